@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Benchmark: TOF single scatter simulation on one MI355X.
+
+One step = one full scatter estimate of the workload with the mu-map, scatter
+points and activity image resident in HBM: trace every scatter-point ->
+detector half path (table build), refresh the emission terms, fused scatter
+sum over (scatter point x LOR x TOF bin), compensated FP32.  Prints one JSON
+line with throughput in scatter-point x LOR x TOF-bin evaluations per second,
+p50/p99 step latency, the roofline of the dominant kernel (the fused scatter
+sum, FP32 VALU), the error against an FP64 run of the same kernels, and a CPU
+baseline (the float64 oracle timed on a bounded scatter-point sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("scatter-pt×LOR×TOF-bin evals/s; p50/p99 latency at ≤1e-4 rel err vs FP64")
+FP32_VALU_PEAK_TFLOPS = 157.3
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument('--gpus', type=int, default=1)
+    p.add_argument('--steps', type=int, default=10)
+    p.add_argument('--warmup', type=int, default=2)
+    p.add_argument('--workload', default='config2', choices=['config1', 'config2', 'small'])
+    p.add_argument('--no-cpu-baseline', action='store_true')
+    p.add_argument('--no-fp64-check', action='store_true')
+    p.add_argument('--cpu-threads', type=int, default=0)
+    return p.parse_args()
+
+
+def make_workload(name):
+    from opensss_amd import phantoms
+    if name == 'config1':
+        return phantoms.config1()
+    if name == 'small':
+        return phantoms.small_tof()
+    return phantoms.config2()
+
+
+def subject(w, precision):
+    from opensss_amd.sss import ScatterSubject
+    water = np.load(os.path.join(ROOT, 'opensss_amd', 'data', 'attenuation_water.npy'))
+    return ScatterSubject(w.attenuation, w.image_size, w.geometry, w.normals, w.det_size, water,
+                          w.energy_resolution, w.energy_threshold, w.n_rings_used, w.n_det_used,
+                          w.tof_resolution, w.tof_range, w.n_bins, scatter_points=w.scatter_points,
+                          precision=precision)
+
+
+def rel_err_above(got, want, frac=0.01):
+    got = np.asarray(got, dtype=np.float64)
+    want = np.asarray(want, dtype=np.float64)
+    peak = np.abs(want).max()
+    big = np.abs(want) >= frac * peak
+    return (float((np.abs(got[big] - want[big]) / np.abs(want[big])).max()),
+            float(np.abs(got - want).max() / peak))
+
+
+def cpu_baseline(w, threads):
+    """Oracle (float64 C restatement) on one scatter point of each of the first
+    `threads` z slices; evals/s scaled by the sample's evaluation count."""
+    from oracle import oracle as ora
+    from opensss_amd.tables import detectors_used, group_by_slice, rings_used
+    ora.build()
+    pts, ptr, _ = group_by_slice(w.scatter_points)
+    take = [ptr[i] for i in range(min(threads, len(ptr) - 1))]
+    sample = pts[take]
+    sp, sptr, _ = group_by_slice(sample)
+    water = np.load(os.path.join(ROOT, 'opensss_amd', 'data', 'attenuation_water.npy'))
+    nru, ndu = w.n_rings_used, w.n_det_used
+    lors = nru * (ndu * (ndu - 1) // 2) + nru * (nru - 1) * (ndu * (ndu + 1) // 2)
+    t0 = time.perf_counter()
+    ora.sss(w.activity, w.attenuation, w.image_size, w.geometry, w.lor_coordinates, w.normals,
+            w.det_size, water, w.energy_resolution, w.energy_threshold,
+            rings_used(w.geometry.shape[0], nru), detectors_used(w.geometry.shape[1], ndu),
+            w.tof_range, w.tof_resolution, w.n_bins, sp, sptr, n_threads=threads, dense=False)
+    dt = time.perf_counter() - t0
+    evals = len(sample) * lors * w.n_bins
+    return {'value': evals / dt, 'unit': 'evals/s', 'cores': threads, 'kind': 'port',
+            'sample': f'{len(sample)} scatter points (one per z slice) of {w.name}, '
+                      f'{evals:.3e} evals in {dt:.2f} s, float64 C oracle, OpenMP over slices'}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
+        print('bench.py runs on one GPU only: the scatter estimate of one instance is measured '
+              'on one MI355X (multi-GPU sharding over z slices is described in DESIGN.md, not run)',
+              file=sys.stderr)
+        sys.exit(2)
+    w = make_workload(args.workload)
+    print(f'[bench] workload {w.name}: {len(w.scatter_points)} sp, grid {w.grid}, '
+          f'{w.n_rings_used}x{w.n_det_used} sampled, {w.n_bins} bins', file=sys.stderr, flush=True)
+    s = subject(w, 'fp32')
+    s.set_activity(w.activity)
+    s.run()
+    s.synchronize()
+    info = s.info
+    for _ in range(args.warmup):
+        s.run()
+    s.synchronize()
+    print('[bench] warmup done', file=sys.stderr, flush=True)
+    lat = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        s.run()
+        s.synchronize()
+        lat.append(time.perf_counter() - t0)
+    elapsed = time.perf_counter() - t_start
+    evals = info['evals']
+    value = evals * args.steps / elapsed
+    print(f'[bench] {args.steps} steps in {elapsed:.3f} s', file=sys.stderr, flush=True)
+
+    # dominant kernel: the fused scatter sum (HIP events on the context stream)
+    kms = []
+    for _ in range(max(3, min(args.steps, 10))):
+        s.scatter()
+        kms.append(s.scatter_kernel_ms())
+    k_ms = float(np.mean(kms))
+    pairs, samples = s.work_stats()
+    flops = 2.0 * w.n_bins * samples + 64.0 * pairs
+    achieved = flops / (k_ms * 1e-3) / 1e12
+    roofline = {'bound': 'valu_fp32', 'achieved': achieved, 'peak': FP32_VALU_PEAK_TFLOPS,
+                'unit': 'TFLOP/s', 'frac': achieved / FP32_VALU_PEAK_TFLOPS, 'traffic': None,
+                'kernel': 'k_scatter', 'kernel_ms': k_ms, 'pairs': pairs, 'samples': samples}
+    out32 = s.output_compact()
+    s.close()
+
+    err = None
+    if not args.no_fp64_check:
+        s64 = subject(w, 'fp64')
+        s64.run(w.activity)
+        out64 = s64.output_compact()
+        s64.close()
+        rel, absn = rel_err_above(out32, out64)
+        err = {'max_rel_err_cells_above_1pct_peak': rel, 'max_abs_err_over_peak': absn}
+        print(f'[bench] FP32 vs FP64: max rel err {rel:.3e}', file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(w, threads)
+        print(f"[bench] cpu baseline {cpu['value']:.3e} evals/s", file=sys.stderr, flush=True)
+
+    lat_ms = np.array(lat) * 1e3
+    line = {
+        'metric': METRIC, 'value': value, 'unit': 'evals/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': elapsed / args.steps * 1e3,
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32',
+        'data': 'synthetic',
+        'config': {'workload': w.name, 'n_scatter_points': int(len(w.scatter_points)),
+                   'lors_per_scatter_point': int(info['n_pairs_enumerated'] // max(1, len(w.scatter_points))),
+                   'tof_bins': w.n_bins, 'evals_per_step': int(evals),
+                   'precision': 'fp32 + compensated sums, fp64 crossings/energy index/TOF offsets',
+                   'parallelism': 'single instance, 1 GPU'},
+        'latency_ms': {'p50': float(np.percentile(lat_ms, 50)), 'p99': float(np.percentile(lat_ms, 99))},
+        'error_vs_fp64': err,
+        'roofline': roofline,
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,147 @@
+/*
+ * sss_hip.h — C ABI of the MI355X TOF single-scatter-simulation library
+ * (libsss_hip.so).  Plain C types only; every function returns an int status
+ * (SSS_OK = 0, negative on error) and never lets an exception cross the ABI.
+ * sss_last_error() returns the message of the last failure on the calling
+ * thread.  A context owns one HIP stream on one device; calls on one context
+ * must not overlap from several threads.
+ *
+ * The reference (r-santo/openSSS) is pure Python; these entry points replace
+ * the numba kernels behind its Python functions:
+ *
+ *   sss_trace_segments     Python/openSSS/RayTracing3DTOF.py:29-180
+ *                          (RayTracing3DTOF, one segment per row)
+ *   sss_subject_create     Python/openSSS/SingleScatterSimulationTOF.py:82-198
+ *   + sss_subject_trace    (set-up tables, half paths of :400-430: traces,
+ *                           attenuation integrals, detector angles)
+ *   sss_subject_set_activity  the activity gathers of :417-418 per request
+ *   sss_subject_scatter    bodySSS :433-557 and the slice sum :193-201
+ *   sss_subject_output_dense  the dense (B, N+1, N/2, NRU^2) layout of :171,551
+ *   sss_interpolate        Python/openSSS/Interpolation.py:95-399
+ *   sss_line_integrals     Python/openSSS/TailsMask.py:84-163,230-309 traces
+ *   sss_backproject_fit    Python/openSSS/BackProjection.py:31-392
+ */
+#ifndef SSS_HIP_H
+#define SSS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SSS_OK 0
+#define SSS_ERR_INVALID -1      /* bad argument or shape */
+#define SSS_ERR_HIP -2          /* HIP runtime error */
+#define SSS_ERR_NOMEM -3        /* device or host allocation failed */
+#define SSS_ERR_UNSUPPORTED -4  /* valid input outside what the kernels implement */
+#define SSS_ERR_TRACE -5        /* a voxel walk did not end where its end voxel says */
+
+#define SSS_PRECISION_FP32 0    /* FP32 sums with compensated accumulation */
+#define SSS_PRECISION_FP64 1    /* all-FP64 validation path */
+
+typedef struct sss_ctx sss_ctx;
+typedef struct sss_subject sss_subject;
+
+/* One scatter-estimation instance: scanner, mu-map and scatter points (host
+ * pointers, copied at create).  Arrays use the reference's layouts: images
+ * are the Fortran-order flattening of (X, Y, Z) (ActivityMap.flatten("F")),
+ * geometry/normals are C-order (NR, N, 3) in cm, scatter points are
+ * positions in cm grouped by normalisation slice (slice_ptr, n_slices+1). */
+typedef struct {
+    int32_t grid[3];
+    double bounds[6];            /* ImageSize [xmin ymin zmin xmax ymax zmax] */
+    const double* mu;            /* X*Y*Z, cm^-1 */
+    int32_t n_rings, n_det;
+    const double* geom;
+    const double* normals;
+    double det_size[2];          /* mm */
+    const double* ratio;         /* n_energy water attenuation ratios */
+    const double* eff;           /* n_energy detection efficiencies */
+    int32_t n_energy;
+    int32_t eff_ref_row;         /* row of 511 keV in the efficiency product (1021) */
+    int32_t nru, ndu;
+    const int32_t* rings;        /* NRU ring ids */
+    const int32_t* dets;         /* NDU detector ids, strictly increasing */
+    int32_t n_sp;
+    const double* sp_xyz;        /* n_sp x 3 */
+    int32_t n_slices;
+    const int32_t* slice_ptr;
+    int32_t n_bins;
+    double tof_bin_width;        /* ps */
+    double tof_kappa;            /* 2 W / w, w = R^2 / (4 ln 2) */
+    double tof_beta;             /* W^2 / w */
+    int32_t tof_half_window;     /* bins either side of the nearest bin */
+    int32_t tof_t_lo, tof_t_hi;  /* non-zero TOF columns, ps */
+    int32_t tof_trivial;         /* one bin, all weights 1 */
+    int32_t precision;           /* SSS_PRECISION_* */
+    int32_t n_chunks;            /* slice chunks for the scatter launch; 0 = auto */
+} sss_subject_desc;
+
+typedef struct {
+    int64_t n_halfpaths;
+    int64_t n_samples;           /* voxel samples over all half paths */
+    int64_t n_cells;             /* compact cells per bin = NRU^2 * NDU(NDU+1)/2 */
+    int64_t n_pairs_enumerated;  /* n_sp * LORs per scatter point */
+    int64_t evals;               /* n_pairs_enumerated * n_bins */
+    int64_t device_bytes;        /* resident device memory of the subject */
+    int32_t n_tiles, n_chunks;
+} sss_subject_info;
+
+const char* sss_last_error(void);
+int sss_version(void);
+
+int sss_ctx_create(int device, sss_ctx** out);
+int sss_ctx_destroy(sss_ctx* ctx);
+int sss_ctx_synchronize(sss_ctx* ctx);
+/* The context's hipStream_t, for callers that time or order work on it. */
+void* sss_ctx_stream(sss_ctx* ctx);
+
+/* Trace n_segs segments (6 doubles each: x1 y1 z1 x2 y2 z2) through the grid.
+ * Outputs have cap = X+Y+Z slots per segment: lengths (f64), 1-based voxel
+ * indices (int32, X + (Y-1)*Ny + (Z-1)*Ny*Nx) and TOF sample positions (f64,
+ * s_k = s_{k-1} + L_k/2).  counts[i] = voxels of segment i (0 if it misses).
+ * Host pointers. */
+int sss_trace_segments(sss_ctx* ctx, const int32_t grid[3], const double bounds[6],
+                       const double* segs, int64_t n_segs, int32_t* counts,
+                       double* lengths, int32_t* indices, double* samples);
+
+/* Per segment: voxel count, sum(mu L) and sum(lam L) (f64).  Host pointers.
+ * lam may be NULL. */
+int sss_line_integrals(sss_ctx* ctx, const int32_t grid[3], const double bounds[6],
+                       const double* mu, const double* lam, const double* segs,
+                       int64_t n_segs, int32_t* counts, double* mu_int, double* lam_int);
+
+int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* desc, sss_subject** out);
+int sss_subject_destroy(sss_subject* s);
+/* Trace every scatter-point -> detector half path and build the resident
+ * table (voxel ids, lengths, TOF sample times, attenuation, angles). */
+int sss_subject_trace(sss_subject* s);
+/* Upload lam (X*Y*Z f64, host) and refresh the per-sample emission terms. */
+int sss_subject_set_activity(sss_subject* s, const double* lam);
+/* Same from a device pointer (X*Y*Z f64), no host copy. */
+int sss_subject_set_activity_device(sss_subject* s, const void* lam_dev);
+/* Scatter sum into the subject's device output (compact layout, float32
+ * [B][NRU^2][NDU(NDU+1)/2], cell (i1 <= i2) at i1*NDU - i1(i1-1)/2 + i2 - i1). */
+int sss_subject_scatter(sss_subject* s);
+/* Trace (if needed) + scatter in one call: the plain one-instance path. */
+int sss_subject_run(sss_subject* s);
+int sss_subject_info_get(const sss_subject* s, sss_subject_info* out);
+/* Pairs passing the reference's filters (A != 0, some activity, efficiency
+ * != 0; SingleScatterSimulationTOF.py:451-509) and the voxel samples their TOF
+ * folds visit, for the algorithmic work count.  Synchronous. */
+int sss_subject_work_stats(sss_subject* s, int64_t* pairs, int64_t* samples);
+/* Duration (HIP events on the context stream) of the last scatter kernel. */
+int sss_subject_scatter_kernel_ms(sss_subject* s, float* ms);
+/* Device pointer of the compact output (valid until destroy). */
+const void* sss_subject_output_device(const sss_subject* s);
+int sss_subject_output_compact(sss_subject* s, float* host_out);
+/* Dense reference layout (B, N+1, N/2, NRU^2) float32 on the host, using
+ * lor_coords (N, N, 2) int16 = (angular, radial). */
+int sss_subject_output_dense(sss_subject* s, const int16_t* lor_coords, float* host_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SSS_HIP_H */

@@ -1,0 +1,110 @@
+// Internal declarations shared by the HIP translation units of libsss_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/sss_hip.h"
+
+namespace sss {
+
+constexpr int kTofHalf = 4000;            // TOF table half width, ps
+constexpr double kLightCmPerPs = 299792458.0 * 1e-10;
+constexpr int kWave = 64;
+constexpr int kTileRows = 8;              // detector tile of one wave: 8 x 8 cells
+constexpr int kTileCols = 8;
+constexpr int kMaxBins = 32;
+constexpr int kMaxHalfWindow = 8;
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define SSS_HIP(call)                                         \
+    do {                                                      \
+        hipError_t e_ = (call);                               \
+        if (e_ != hipSuccess) return ::sss::hip_fail(e_, #call); \
+    } while (0)
+
+// Geometry of the voxel grid, float64 as the reference.
+struct Grid {
+    int nx, ny, nz;
+    double b[6];
+};
+
+// Per-half-path scalars (structure of arrays, one entry per (sp, ring, det)).
+struct HalfPathTable {
+    int64_t* off;      // first sample
+    int32_t* n;        // samples
+    double* dist;      // |D - S|, cm
+    double* att;       // exp(-sum mu L)
+    double* lnatt;     // sum mu L
+    double* act;       // sum lam L (f64, exact-zero tests)
+    float* cosabs;     // |cos| of the angle to the detector normal
+    int32_t* status;   // 0 ok, 1 walk overflow
+};
+
+// Per-sample arrays (compacted, offsets in HalfPathTable::off).
+struct SampleTable {
+    int32_t* vox;      // 0-based voxel (reference linear index - 1)
+    double* len;       // intersection length, cm
+    double* q;         // 2 s_k / c, ps
+    float* a;          // lam * len
+};
+
+struct ScatterArgs {
+    // sizes
+    int n_sp, nru, ndu, n_bins, n_energy, eff_ref_row;
+    int n_tiles, n_chunks;
+    int64_t n_cells_plane;       // NDU (NDU+1)/2
+    // tables
+    const int64_t* hp_off;
+    const int32_t* hp_n;
+    const double* hp_dist;
+    const double* hp_att;
+    const double* hp_lnatt;
+    const double* hp_act;
+    const float* hp_cos;
+    const double* s_q;
+    const float* s_a;
+    const double* detpos;        // (NRU, NDU, 3)
+    const double* eff;           // n_energy
+    const float* ratio;          // n_energy
+    const int4* tiles;           // (r1, r2, i1_0, i2_0)
+    const int32_t* slice_ptr;    // n_slices + 1
+    const int32_t* chunk_slices; // n_chunks + 1 (slice ranges)
+    // TOF kernel
+    float inv_w_bins, mid, bin_w, kappa2, beta2;   // log2 scaled
+    int half_window, t_lo, t_hi;
+    double g_const;              // (Dx Dy 1e-2)^2 / (4 pi)
+    double kn_gamma;
+    // output
+    void* out;                   // [chunk][B][NRU^2][cells], float or double
+};
+
+// Launchers (defined in the .hip files).
+hipError_t launch_halfpath_count(const Grid& g, const double* sp_xyz, int n_sp, const double* detpos,
+                                 const double* detnorm, int nru, int ndu, HalfPathTable t,
+                                 hipStream_t st);
+hipError_t launch_halfpath_fill(const Grid& g, const double* sp_xyz, int n_sp, const double* detpos,
+                                int nru, int ndu, const double* mu, HalfPathTable t, SampleTable s,
+                                hipStream_t st);
+hipError_t launch_activity_refresh(const HalfPathTable t, SampleTable s, int64_t n_hp,
+                                   const double* lam, hipStream_t st);
+hipError_t launch_trace_segments(const Grid& g, const double* segs, int64_t n, int cap,
+                                 int32_t* counts, double* L, int32_t* idx, double* s,
+                                 hipStream_t st);
+hipError_t launch_line_integrals(const Grid& g, const double* segs, int64_t n, const double* mu,
+                                 const double* lam, int32_t* counts, double* mu_int,
+                                 double* lam_int, hipStream_t st);
+hipError_t launch_scatter(const ScatterArgs& a, bool fp64, bool trivial, hipStream_t st);
+hipError_t launch_pair_stats(const ScatterArgs& a, unsigned long long* out, hipStream_t st);
+hipError_t launch_chunk_reduce_typed(const void* partial, int n_chunks, int64_t n, void* out,
+                                     bool fp64, hipStream_t st);
+hipError_t launch_dense_scatter_typed(const void* compact, bool fp64, int n_bins, int nru, int ndu,
+                                      int n_det, const int32_t* dets, const int16_t* lor,
+                                      float* dense, hipStream_t st);
+
+}  // namespace sss

@@ -1,0 +1,640 @@
+// C ABI, contexts and resident scatter subjects of libsss_hip.so.
+//
+// A subject holds, in HBM, everything one scatter estimate reuses across
+// calls: the mu-map, scatter points, sampled-detector positions, the
+// half-path table built by the tracer (voxel ids, lengths, TOF sample
+// times, attenuation integrals, detector angles) and the output buffers.
+// Only the activity image changes per request.
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "sss_internal.h"
+
+namespace sss {
+
+static thread_local std::string g_error;
+
+void set_error(const std::string& msg) { g_error = msg; }
+
+int fail(int code, const std::string& msg)
+{
+    g_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what)
+{
+    g_error = std::string(what) + ": " + hipGetErrorString(e);
+    return SSS_ERR_HIP;
+}
+
+namespace {
+
+__global__ void k_max_status(const int32_t* __restrict__ st, int64_t n, int32_t* out)
+{
+    int32_t m = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        m = st[i] > m ? st[i] : m;
+    if (m) atomicMax(out, m);
+}
+
+template <typename T>
+int dev_alloc(T** p, size_t count, int64_t* tally)
+{
+    *p = nullptr;
+    if (count == 0) return SSS_OK;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(SSS_ERR_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    }
+    if (tally) *tally += (int64_t)(count * sizeof(T));
+    return SSS_OK;
+}
+
+template <typename T>
+void dev_free(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+}  // namespace
+}  // namespace sss
+
+using namespace sss;
+
+struct sss_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+};
+
+struct sss_subject {
+    sss_ctx* ctx = nullptr;
+    Grid grid{};
+    int n_rings = 0, n_det = 0, nru = 0, ndu = 0, n_sp = 0, n_slices = 0;
+    int n_bins = 0, n_energy = 0, eff_ref_row = 0, precision = 0;
+    bool trivial = false;
+    int half_window = 0, t_lo = 0, t_hi = 0;
+    double bin_w = 0, kappa = 0, beta = 0, g_const = 0;
+    int64_t n_vox = 0, n_hp = 0, n_samples = 0, samples_cap = 0, n_cells_plane = 0, out_elems = 0;
+    int n_tiles = 0, n_chunks = 1;
+    int64_t device_bytes = 0;
+    bool traced = false, has_activity = false;
+
+    double* d_mu = nullptr;
+    double* d_lam = nullptr;
+    double* d_sp = nullptr;
+    double* d_detpos = nullptr;
+    double* d_detnorm = nullptr;
+    double* d_eff = nullptr;
+    float* d_ratio = nullptr;
+    int4* d_tiles = nullptr;
+    int32_t* d_slice_ptr = nullptr;
+    int32_t* d_chunk_slices = nullptr;
+    int32_t* d_dets = nullptr;
+    int32_t* d_flag = nullptr;
+    void* d_scan_tmp = nullptr;
+    size_t scan_tmp_bytes = 0;
+    HalfPathTable hp{};
+    SampleTable smp{};
+    void* d_out = nullptr;
+    void* d_partial = nullptr;
+    std::vector<int32_t> dets;
+    hipEvent_t ev_scatter[2] = {nullptr, nullptr};   // around the last scatter kernel
+    bool ev_recorded = false;
+
+    ~sss_subject()
+    {
+        for (auto& e : ev_scatter)
+            if (e) (void)hipEventDestroy(e);
+        dev_free(d_mu); dev_free(d_lam); dev_free(d_sp); dev_free(d_detpos); dev_free(d_detnorm);
+        dev_free(d_eff); dev_free(d_ratio); dev_free(d_tiles); dev_free(d_slice_ptr);
+        dev_free(d_chunk_slices); dev_free(d_dets); dev_free(d_flag); dev_free(d_scan_tmp);
+        dev_free(hp.off); dev_free(hp.n); dev_free(hp.dist); dev_free(hp.att); dev_free(hp.lnatt);
+        dev_free(hp.act); dev_free(hp.cosabs); dev_free(hp.status);
+        dev_free(smp.vox); dev_free(smp.len); dev_free(smp.q); dev_free(smp.a);
+        dev_free(d_out); dev_free(d_partial);
+    }
+};
+
+extern "C" {
+
+const char* sss_last_error(void) { return g_error.c_str(); }
+
+int sss_version(void) { return 1; }
+
+int sss_ctx_create(int device, sss_ctx** out)
+{
+    if (!out) return fail(SSS_ERR_INVALID, "sss_ctx_create: out is NULL");
+    *out = nullptr;
+    int n = 0;
+    SSS_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(SSS_ERR_INVALID, "sss_ctx_create: no such device");
+    SSS_HIP(hipSetDevice(device));
+    sss_ctx* c = new (std::nothrow) sss_ctx;
+    if (!c) return fail(SSS_ERR_NOMEM, "sss_ctx_create: out of host memory");
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return hip_fail(e, "hipStreamCreateWithFlags");
+    }
+    *out = c;
+    return SSS_OK;
+}
+
+int sss_ctx_destroy(sss_ctx* ctx)
+{
+    if (!ctx) return SSS_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SSS_OK;
+}
+
+int sss_ctx_synchronize(sss_ctx* ctx)
+{
+    if (!ctx) return fail(SSS_ERR_INVALID, "sss_ctx_synchronize: NULL context");
+    SSS_HIP(hipSetDevice(ctx->device));
+    SSS_HIP(hipStreamSynchronize(ctx->stream));
+    return SSS_OK;
+}
+
+void* sss_ctx_stream(sss_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+static int check_grid(const int32_t grid[3], const double bounds[6], Grid* g)
+{
+    if (!grid || !bounds) return fail(SSS_ERR_INVALID, "grid/bounds is NULL");
+    if (grid[0] <= 0 || grid[1] <= 0 || grid[2] <= 0)
+        return fail(SSS_ERR_INVALID, "grid dimensions must be positive");
+    if (grid[0] != grid[1])
+        return fail(SSS_ERR_INVALID,
+                    "grid must have Nx == Ny: the reference voxel index uses Ny as the y stride");
+    if (!(bounds[3] > bounds[0] && bounds[4] > bounds[1] && bounds[5] > bounds[2]))
+        return fail(SSS_ERR_INVALID, "bounds must satisfy max > min on every axis");
+    g->nx = grid[0];
+    g->ny = grid[1];
+    g->nz = grid[2];
+    for (int i = 0; i < 6; ++i) g->b[i] = bounds[i];
+    return SSS_OK;
+}
+
+int sss_trace_segments(sss_ctx* ctx, const int32_t grid[3], const double bounds[6],
+                       const double* segs, int64_t n_segs, int32_t* counts, double* lengths,
+                       int32_t* indices, double* samples)
+{
+    if (!ctx) return fail(SSS_ERR_INVALID, "sss_trace_segments: NULL context");
+    Grid g;
+    int rc = check_grid(grid, bounds, &g);
+    if (rc) return rc;
+    if (n_segs < 0 || (n_segs > 0 && (!segs || !counts || !lengths || !indices || !samples)))
+        return fail(SSS_ERR_INVALID, "sss_trace_segments: bad buffers");
+    if (n_segs == 0) return SSS_OK;
+    SSS_HIP(hipSetDevice(ctx->device));
+    const int cap = g.nx + g.ny + g.nz;
+    double *d_segs = nullptr, *d_L = nullptr, *d_s = nullptr;
+    int32_t *d_c = nullptr, *d_i = nullptr;
+    int64_t tally = 0;
+    const size_t slots = (size_t)n_segs * cap;
+    if ((rc = dev_alloc(&d_segs, (size_t)n_segs * 6, &tally)) ||
+        (rc = dev_alloc(&d_L, slots, &tally)) || (rc = dev_alloc(&d_s, slots, &tally)) ||
+        (rc = dev_alloc(&d_c, (size_t)n_segs, &tally)) || (rc = dev_alloc(&d_i, slots, &tally))) {
+        dev_free(d_segs); dev_free(d_L); dev_free(d_s); dev_free(d_c); dev_free(d_i);
+        return rc;
+    }
+    hipStream_t st = ctx->stream;
+    hipError_t e = hipMemcpyAsync(d_segs, segs, (size_t)n_segs * 6 * sizeof(double),
+                                  hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_trace_segments(g, d_segs, n_segs, cap, d_c, d_L, d_i, d_s, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(counts, d_c, (size_t)n_segs * sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(lengths, d_L, slots * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(indices, d_i, slots * sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(samples, d_s, slots * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    dev_free(d_segs); dev_free(d_L); dev_free(d_s); dev_free(d_c); dev_free(d_i);
+    if (e != hipSuccess) return hip_fail(e, "sss_trace_segments");
+    for (int64_t i = 0; i < n_segs; ++i)
+        if (counts[i] < 0) return fail(SSS_ERR_TRACE, "sss_trace_segments: a voxel walk did not end at its end voxel");
+    return SSS_OK;
+}
+
+int sss_line_integrals(sss_ctx* ctx, const int32_t grid[3], const double bounds[6],
+                       const double* mu, const double* lam, const double* segs, int64_t n_segs,
+                       int32_t* counts, double* mu_int, double* lam_int)
+{
+    if (!ctx) return fail(SSS_ERR_INVALID, "sss_line_integrals: NULL context");
+    Grid g;
+    int rc = check_grid(grid, bounds, &g);
+    if (rc) return rc;
+    if (!mu || n_segs < 0 || (n_segs > 0 && (!segs || !counts || !mu_int || !lam_int)))
+        return fail(SSS_ERR_INVALID, "sss_line_integrals: bad buffers");
+    if (n_segs == 0) return SSS_OK;
+    SSS_HIP(hipSetDevice(ctx->device));
+    const size_t nvox = (size_t)g.nx * g.ny * g.nz;
+    double *d_mu = nullptr, *d_lam = nullptr, *d_segs = nullptr, *d_m = nullptr, *d_l = nullptr;
+    int32_t* d_c = nullptr;
+    int64_t tally = 0;
+    if ((rc = dev_alloc(&d_mu, nvox, &tally)) || (lam && (rc = dev_alloc(&d_lam, nvox, &tally))) ||
+        (rc = dev_alloc(&d_segs, (size_t)n_segs * 6, &tally)) ||
+        (rc = dev_alloc(&d_m, (size_t)n_segs, &tally)) || (rc = dev_alloc(&d_l, (size_t)n_segs, &tally)) ||
+        (rc = dev_alloc(&d_c, (size_t)n_segs, &tally))) {
+        dev_free(d_mu); dev_free(d_lam); dev_free(d_segs); dev_free(d_m); dev_free(d_l); dev_free(d_c);
+        return rc;
+    }
+    hipStream_t st = ctx->stream;
+    hipError_t e = hipMemcpyAsync(d_mu, mu, nvox * sizeof(double), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && lam) e = hipMemcpyAsync(d_lam, lam, nvox * sizeof(double), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_segs, segs, (size_t)n_segs * 6 * sizeof(double), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_line_integrals(g, d_segs, n_segs, d_mu, d_lam, d_c, d_m, d_l, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(counts, d_c, (size_t)n_segs * sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(mu_int, d_m, (size_t)n_segs * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(lam_int, d_l, (size_t)n_segs * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    dev_free(d_mu); dev_free(d_lam); dev_free(d_segs); dev_free(d_m); dev_free(d_l); dev_free(d_c);
+    if (e != hipSuccess) return hip_fail(e, "sss_line_integrals");
+    for (int64_t i = 0; i < n_segs; ++i)
+        if (counts[i] < 0) return fail(SSS_ERR_TRACE, "sss_line_integrals: a voxel walk did not end at its end voxel");
+    return SSS_OK;
+}
+
+int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** out)
+{
+    if (!ctx || !d || !out) return fail(SSS_ERR_INVALID, "sss_subject_create: NULL argument");
+    *out = nullptr;
+    Grid g;
+    int rc = check_grid(d->grid, d->bounds, &g);
+    if (rc) return rc;
+    if (!d->mu || !d->geom || !d->normals || !d->ratio || !d->eff || !d->rings || !d->dets ||
+        !d->slice_ptr || (d->n_sp > 0 && !d->sp_xyz))
+        return fail(SSS_ERR_INVALID, "sss_subject_create: NULL array");
+    if (d->n_rings <= 0 || d->n_det <= 1 || d->nru <= 0 || d->ndu <= 0 || d->nru > d->n_rings ||
+        d->ndu > d->n_det)
+        return fail(SSS_ERR_INVALID, "sss_subject_create: bad scanner sizes");
+    if (d->n_bins < 1 || d->n_bins > kMaxBins)
+        return fail(SSS_ERR_UNSUPPORTED, "sss_subject_create: 1 <= n_bins <= 32 is supported");
+    if (d->tof_half_window < 0 || d->tof_half_window > kMaxHalfWindow)
+        return fail(SSS_ERR_UNSUPPORTED, "sss_subject_create: TOF half window above 8 bins");
+    if (d->n_energy <= 0 || d->eff_ref_row < 0 || d->eff_ref_row >= d->n_energy)
+        return fail(SSS_ERR_INVALID, "sss_subject_create: bad energy tables");
+    if (d->precision != SSS_PRECISION_FP32 && d->precision != SSS_PRECISION_FP64)
+        return fail(SSS_ERR_INVALID, "sss_subject_create: bad precision");
+    for (int r = 0; r < d->nru; ++r)
+        if (d->rings[r] < 0 || d->rings[r] >= d->n_rings)
+            return fail(SSS_ERR_INVALID, "sss_subject_create: ring id out of range");
+    for (int i = 0; i < d->ndu; ++i) {
+        if (d->dets[i] < 0 || d->dets[i] >= d->n_det)
+            return fail(SSS_ERR_INVALID, "sss_subject_create: detector id out of range");
+        if (i > 0 && d->dets[i] <= d->dets[i - 1])
+            return fail(SSS_ERR_INVALID, "sss_subject_create: detector ids must increase");
+    }
+    if (d->n_slices < 0 || d->slice_ptr[0] != 0 || d->slice_ptr[d->n_slices] != d->n_sp)
+        return fail(SSS_ERR_INVALID, "sss_subject_create: slice_ptr must run from 0 to n_sp");
+    for (int s = 0; s < d->n_slices; ++s)
+        if (d->slice_ptr[s + 1] < d->slice_ptr[s])
+            return fail(SSS_ERR_INVALID, "sss_subject_create: slice_ptr must not decrease");
+
+    SSS_HIP(hipSetDevice(ctx->device));
+    sss_subject* s = new (std::nothrow) sss_subject;
+    if (!s) return fail(SSS_ERR_NOMEM, "sss_subject_create: out of host memory");
+    s->ctx = ctx;
+    s->grid = g;
+    s->n_rings = d->n_rings; s->n_det = d->n_det; s->nru = d->nru; s->ndu = d->ndu;
+    s->n_sp = d->n_sp; s->n_slices = d->n_slices; s->n_bins = d->n_bins;
+    s->n_energy = d->n_energy; s->eff_ref_row = d->eff_ref_row; s->precision = d->precision;
+    s->trivial = d->tof_trivial != 0 && d->n_bins == 1;
+    s->half_window = d->tof_half_window; s->t_lo = d->tof_t_lo; s->t_hi = d->tof_t_hi;
+    s->bin_w = d->tof_bin_width; s->kappa = d->tof_kappa; s->beta = d->tof_beta;
+    const double ds = d->det_size[0] * d->det_size[1] * 1e-2;
+    s->g_const = ds * ds / (4.0 * M_PI);
+    s->n_vox = (int64_t)g.nx * g.ny * g.nz;
+    s->n_hp = (int64_t)d->n_sp * d->nru * d->ndu;
+    s->n_cells_plane = (int64_t)d->ndu * (d->ndu + 1) / 2;
+    s->out_elems = (int64_t)d->n_bins * d->nru * d->nru * s->n_cells_plane;
+    s->dets.assign(d->dets, d->dets + d->ndu);
+
+    // sampled detector positions; z of a ring is taken from its detector 0
+    // (SingleScatterSimulationTOF.py:402,406-407)
+    std::vector<double> detpos((size_t)d->nru * d->ndu * 3), detnorm((size_t)d->nru * d->ndu * 3);
+    for (int r = 0; r < d->nru; ++r)
+        for (int i = 0; i < d->ndu; ++i) {
+            const size_t ring = (size_t)d->rings[r], det = (size_t)d->dets[i];
+            const size_t o = ((size_t)r * d->ndu + i) * 3;
+            detpos[o + 0] = d->geom[(ring * d->n_det + det) * 3 + 0];
+            detpos[o + 1] = d->geom[(ring * d->n_det + det) * 3 + 1];
+            detpos[o + 2] = d->geom[(ring * d->n_det + 0) * 3 + 2];
+            for (int c = 0; c < 3; ++c) detnorm[o + c] = d->normals[(ring * d->n_det + det) * 3 + c];
+        }
+    // wave tiles of 8 x 8 cells over the upper triangle of every ring-pair plane
+    std::vector<int4> tiles;
+    const int nt = (d->ndu + kTileRows - 1) / kTileRows;
+    for (int r1 = 0; r1 < d->nru; ++r1)
+        for (int r2 = 0; r2 < d->nru; ++r2)
+            for (int ti = 0; ti < nt; ++ti)
+                for (int tj = ti; tj < nt; ++tj)
+                    tiles.push_back(make_int4(r1, r2, ti * kTileRows, tj * kTileCols));
+    s->n_tiles = (int)tiles.size();
+    // slice chunks of about equal scatter-point counts
+    int chunks = d->n_chunks;
+    if (chunks <= 0) chunks = (int)std::max<int64_t>(1, (16384 + s->n_tiles - 1) / std::max(1, s->n_tiles));
+    chunks = std::max(1, std::min(chunks, std::max(1, d->n_slices)));
+    std::vector<int32_t> chunk_slices(1, 0);
+    for (int c = 1; c < chunks; ++c) {
+        const int64_t target = (int64_t)d->n_sp * c / chunks;
+        int sl = chunk_slices.back();
+        while (sl < d->n_slices && d->slice_ptr[sl] < target) ++sl;
+        if (sl > chunk_slices.back() && sl < d->n_slices) chunk_slices.push_back(sl);
+    }
+    chunk_slices.push_back(d->n_slices);
+    s->n_chunks = (int)chunk_slices.size() - 1;
+    std::vector<float> ratio32(d->ratio, d->ratio + d->n_energy);
+
+    int64_t& t = s->device_bytes;
+    const size_t elem = s->precision == SSS_PRECISION_FP64 ? sizeof(double) : sizeof(float);
+    if ((rc = dev_alloc(&s->d_mu, (size_t)s->n_vox, &t)) || (rc = dev_alloc(&s->d_lam, (size_t)s->n_vox, &t)) ||
+        (rc = dev_alloc(&s->d_sp, (size_t)std::max(1, d->n_sp) * 3, &t)) ||
+        (rc = dev_alloc(&s->d_detpos, detpos.size(), &t)) || (rc = dev_alloc(&s->d_detnorm, detnorm.size(), &t)) ||
+        (rc = dev_alloc(&s->d_eff, (size_t)d->n_energy, &t)) || (rc = dev_alloc(&s->d_ratio, (size_t)d->n_energy, &t)) ||
+        (rc = dev_alloc(&s->d_tiles, tiles.size() + 1, &t)) ||
+        (rc = dev_alloc(&s->d_slice_ptr, (size_t)d->n_slices + 1, &t)) ||
+        (rc = dev_alloc(&s->d_chunk_slices, chunk_slices.size(), &t)) ||
+        (rc = dev_alloc(&s->d_dets, (size_t)d->ndu, &t)) || (rc = dev_alloc(&s->d_flag, 1, &t)) ||
+        (rc = dev_alloc(&s->hp.off, (size_t)s->n_hp + 1, &t)) || (rc = dev_alloc(&s->hp.n, (size_t)s->n_hp + 1, &t)) ||
+        (rc = dev_alloc(&s->hp.dist, (size_t)s->n_hp + 1, &t)) || (rc = dev_alloc(&s->hp.att, (size_t)s->n_hp + 1, &t)) ||
+        (rc = dev_alloc(&s->hp.lnatt, (size_t)s->n_hp + 1, &t)) || (rc = dev_alloc(&s->hp.act, (size_t)s->n_hp + 1, &t)) ||
+        (rc = dev_alloc(&s->hp.cosabs, (size_t)s->n_hp + 1, &t)) ||
+        (rc = dev_alloc(&s->hp.status, (size_t)s->n_hp + 1, &t)) ||
+        (rc = dev_alloc(reinterpret_cast<unsigned char**>(&s->d_out), (size_t)s->out_elems * elem + 1, &t)) ||
+        (s->n_chunks > 1 &&
+         (rc = dev_alloc(reinterpret_cast<unsigned char**>(&s->d_partial),
+                         (size_t)s->out_elems * elem * s->n_chunks, &t)))) {
+        delete s;
+        return rc;
+    }
+    hipStream_t st = ctx->stream;
+    // cells no lane owns (i1 == i2 on same-ring planes) are never written: zero once
+    hipError_t e = hipMemsetAsync(s->d_out, 0, (size_t)s->out_elems * elem, st);
+    if (e == hipSuccess && s->d_partial)
+        e = hipMemsetAsync(s->d_partial, 0, (size_t)s->out_elems * elem * s->n_chunks, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s->d_mu, d->mu, (size_t)s->n_vox * sizeof(double), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && d->n_sp > 0)
+        e = hipMemcpyAsync(s->d_sp, d->sp_xyz, (size_t)d->n_sp * 3 * sizeof(double), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->d_detpos, detpos.data(), detpos.size() * sizeof(double), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->d_detnorm, detnorm.data(), detnorm.size() * sizeof(double), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->d_eff, d->eff, (size_t)d->n_energy * sizeof(double), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->d_ratio, ratio32.data(), ratio32.size() * sizeof(float), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && !tiles.empty()) e = hipMemcpyAsync(s->d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->d_slice_ptr, d->slice_ptr, ((size_t)d->n_slices + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->d_chunk_slices, chunk_slices.data(), chunk_slices.size() * sizeof(int32_t), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->d_dets, d->dets, (size_t)d->ndu * sizeof(int32_t), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipEventCreate(&s->ev_scatter[0]);
+    if (e == hipSuccess) e = hipEventCreate(&s->ev_scatter[1]);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        delete s;
+        return hip_fail(e, "sss_subject_create: upload");
+    }
+    *out = s;
+    return SSS_OK;
+}
+
+int sss_subject_destroy(sss_subject* s)
+{
+    if (!s) return SSS_OK;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    delete s;
+    return SSS_OK;
+}
+
+int sss_subject_trace(sss_subject* s)
+{
+    if (!s) return fail(SSS_ERR_INVALID, "sss_subject_trace: NULL subject");
+    SSS_HIP(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->ctx->stream;
+    s->traced = false;
+    if (s->n_hp == 0) {
+        s->n_samples = 0;
+        s->traced = true;
+        return SSS_OK;
+    }
+    SSS_HIP(launch_halfpath_count(s->grid, s->d_sp, s->n_sp, s->d_detpos, s->d_detnorm, s->nru, s->ndu,
+                                  s->hp, st));
+    // exclusive scan of the per-half-path voxel counts (int64 offsets)
+    size_t need = 0;
+    SSS_HIP(rocprim::exclusive_scan(nullptr, need, s->hp.n, s->hp.off, (int64_t)0, (size_t)s->n_hp + 1,
+                                    rocprim::plus<int64_t>(), st));
+    if (need > s->scan_tmp_bytes) {
+        dev_free(s->d_scan_tmp);
+        unsigned char* tmp = nullptr;
+        int rc = dev_alloc(&tmp, need, &s->device_bytes);
+        if (rc) return rc;
+        s->d_scan_tmp = tmp;
+        s->scan_tmp_bytes = need;
+    }
+    // slot n_hp holds a zero count so off[n_hp] is the total
+    SSS_HIP(hipMemsetAsync(s->hp.n + s->n_hp, 0, sizeof(int32_t), st));
+    SSS_HIP(rocprim::exclusive_scan(s->d_scan_tmp, need, s->hp.n, s->hp.off, (int64_t)0, (size_t)s->n_hp + 1,
+                                    rocprim::plus<int64_t>(), st));
+    int64_t total = 0;
+    SSS_HIP(hipMemcpyAsync(&total, s->hp.off + s->n_hp, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    SSS_HIP(hipStreamSynchronize(st));
+    if (total > s->samples_cap) {
+        dev_free(s->smp.vox); dev_free(s->smp.len); dev_free(s->smp.q); dev_free(s->smp.a);
+        s->samples_cap = 0;
+        int rc;
+        if ((rc = dev_alloc(&s->smp.vox, (size_t)total, &s->device_bytes)) ||
+            (rc = dev_alloc(&s->smp.len, (size_t)total, &s->device_bytes)) ||
+            (rc = dev_alloc(&s->smp.q, (size_t)total, &s->device_bytes)) ||
+            (rc = dev_alloc(&s->smp.a, (size_t)total, &s->device_bytes)))
+            return rc;
+        s->samples_cap = total;
+    }
+    s->n_samples = total;
+    SSS_HIP(launch_halfpath_fill(s->grid, s->d_sp, s->n_sp, s->d_detpos, s->nru, s->ndu, s->d_mu, s->hp,
+                                 s->smp, st));
+    SSS_HIP(hipMemsetAsync(s->d_flag, 0, sizeof(int32_t), st));
+    hipLaunchKernelGGL(k_max_status, dim3(256), dim3(256), 0, st, s->hp.status, s->n_hp, s->d_flag);
+    SSS_HIP(hipGetLastError());
+    int32_t flag = 0;
+    SSS_HIP(hipMemcpyAsync(&flag, s->d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    SSS_HIP(hipStreamSynchronize(st));
+    if (flag) return fail(SSS_ERR_TRACE, "sss_subject_trace: a voxel walk did not end at its end voxel");
+    s->traced = true;
+    if (s->has_activity) SSS_HIP(launch_activity_refresh(s->hp, s->smp, s->n_hp, s->d_lam, st));
+    return SSS_OK;
+}
+
+static int refresh_activity(sss_subject* s)
+{
+    s->has_activity = true;
+    if (s->traced) SSS_HIP(launch_activity_refresh(s->hp, s->smp, s->n_hp, s->d_lam, s->ctx->stream));
+    return SSS_OK;
+}
+
+int sss_subject_set_activity(sss_subject* s, const double* lam)
+{
+    if (!s || !lam) return fail(SSS_ERR_INVALID, "sss_subject_set_activity: NULL argument");
+    SSS_HIP(hipSetDevice(s->ctx->device));
+    SSS_HIP(hipMemcpyAsync(s->d_lam, lam, (size_t)s->n_vox * sizeof(double), hipMemcpyHostToDevice,
+                           s->ctx->stream));
+    return refresh_activity(s);
+}
+
+int sss_subject_set_activity_device(sss_subject* s, const void* lam_dev)
+{
+    if (!s || !lam_dev) return fail(SSS_ERR_INVALID, "sss_subject_set_activity_device: NULL argument");
+    SSS_HIP(hipSetDevice(s->ctx->device));
+    SSS_HIP(hipMemcpyAsync(s->d_lam, lam_dev, (size_t)s->n_vox * sizeof(double), hipMemcpyDeviceToDevice,
+                           s->ctx->stream));
+    return refresh_activity(s);
+}
+
+static ScatterArgs scatter_args(const sss_subject* s)
+{
+    ScatterArgs a{};
+    a.n_sp = s->n_sp; a.nru = s->nru; a.ndu = s->ndu; a.n_bins = s->n_bins;
+    a.n_energy = s->n_energy; a.eff_ref_row = s->eff_ref_row;
+    a.n_tiles = s->n_tiles; a.n_chunks = s->n_chunks; a.n_cells_plane = s->n_cells_plane;
+    a.hp_off = s->hp.off; a.hp_n = s->hp.n; a.hp_dist = s->hp.dist; a.hp_att = s->hp.att;
+    a.hp_lnatt = s->hp.lnatt; a.hp_act = s->hp.act; a.hp_cos = s->hp.cosabs;
+    a.s_q = s->smp.q; a.s_a = s->smp.a;
+    a.detpos = s->d_detpos; a.eff = s->d_eff; a.ratio = s->d_ratio; a.tiles = s->d_tiles;
+    a.slice_ptr = s->d_slice_ptr; a.chunk_slices = s->d_chunk_slices;
+    const double log2e = 1.4426950408889634;
+    a.inv_w_bins = (float)(1.0 / s->bin_w);
+    a.mid = (float)((s->n_bins - 1) / 2.0);
+    a.bin_w = (float)s->bin_w;
+    a.kappa2 = (float)(s->kappa * log2e);
+    a.beta2 = (float)(s->beta * log2e);
+    a.half_window = s->half_window; a.t_lo = s->t_lo; a.t_hi = s->t_hi;
+    a.g_const = s->g_const;
+    a.kn_gamma = (511.0e3 * 1.6e-19) / (9.10938356e-31 * (299792458.0 * 299792458.0));
+    a.out = s->n_chunks > 1 ? s->d_partial : s->d_out;
+    return a;
+}
+
+int sss_subject_scatter(sss_subject* s)
+{
+    if (!s) return fail(SSS_ERR_INVALID, "sss_subject_scatter: NULL subject");
+    if (!s->traced) return fail(SSS_ERR_INVALID, "sss_subject_scatter: call sss_subject_trace first");
+    if (!s->has_activity) return fail(SSS_ERR_INVALID, "sss_subject_scatter: no activity image set");
+    SSS_HIP(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->ctx->stream;
+    const bool fp64 = s->precision == SSS_PRECISION_FP64;
+    const ScatterArgs a = scatter_args(s);
+    SSS_HIP(hipEventRecord(s->ev_scatter[0], st));
+    SSS_HIP(launch_scatter(a, fp64, s->trivial, st));
+    SSS_HIP(hipEventRecord(s->ev_scatter[1], st));
+    s->ev_recorded = true;
+    if (s->n_chunks > 1)
+        SSS_HIP(launch_chunk_reduce_typed(s->d_partial, s->n_chunks, s->out_elems, s->d_out, fp64, st));
+    return SSS_OK;
+}
+
+int sss_subject_scatter_kernel_ms(sss_subject* s, float* ms)
+{
+    if (!s || !ms) return fail(SSS_ERR_INVALID, "sss_subject_scatter_kernel_ms: NULL argument");
+    if (!s->ev_recorded) return fail(SSS_ERR_INVALID, "sss_subject_scatter_kernel_ms: no scatter run yet");
+    SSS_HIP(hipSetDevice(s->ctx->device));
+    SSS_HIP(hipEventSynchronize(s->ev_scatter[1]));
+    SSS_HIP(hipEventElapsedTime(ms, s->ev_scatter[0], s->ev_scatter[1]));
+    return SSS_OK;
+}
+
+int sss_subject_work_stats(sss_subject* s, int64_t* pairs, int64_t* samples)
+{
+    if (!s || !pairs || !samples) return fail(SSS_ERR_INVALID, "sss_subject_work_stats: NULL argument");
+    if (!s->traced || !s->has_activity)
+        return fail(SSS_ERR_INVALID, "sss_subject_work_stats: trace and set the activity first");
+    SSS_HIP(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->ctx->stream;
+    unsigned long long* d = nullptr;
+    int rc = dev_alloc(&d, 2, nullptr);
+    if (rc) return rc;
+    unsigned long long h[2] = {0, 0};
+    hipError_t e = hipMemsetAsync(d, 0, sizeof(h), st);
+    if (e == hipSuccess) e = launch_pair_stats(scatter_args(s), d, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    dev_free(d);
+    if (e != hipSuccess) return hip_fail(e, "sss_subject_work_stats");
+    *pairs = (int64_t)h[0];
+    *samples = (int64_t)h[1];
+    return SSS_OK;
+}
+
+int sss_subject_run(sss_subject* s)
+{
+    int rc = sss_subject_trace(s);
+    if (rc) return rc;
+    return sss_subject_scatter(s);
+}
+
+int sss_subject_info_get(const sss_subject* s, sss_subject_info* out)
+{
+    if (!s || !out) return fail(SSS_ERR_INVALID, "sss_subject_info_get: NULL argument");
+    out->n_halfpaths = s->n_hp;
+    out->n_samples = s->n_samples;
+    out->n_cells = (int64_t)s->nru * s->nru * s->n_cells_plane;
+    const int64_t lors = (int64_t)s->nru * (s->ndu * (int64_t)(s->ndu - 1) / 2) +
+                         (int64_t)s->nru * (s->nru - 1) * ((int64_t)s->ndu * (s->ndu + 1) / 2);
+    out->n_pairs_enumerated = lors * s->n_sp;
+    out->evals = out->n_pairs_enumerated * s->n_bins;
+    out->device_bytes = s->device_bytes;
+    out->n_tiles = s->n_tiles;
+    out->n_chunks = s->n_chunks;
+    return SSS_OK;
+}
+
+const void* sss_subject_output_device(const sss_subject* s) { return s ? s->d_out : nullptr; }
+
+int sss_subject_output_compact(sss_subject* s, float* host_out)
+{
+    if (!s || !host_out) return fail(SSS_ERR_INVALID, "sss_subject_output_compact: NULL argument");
+    SSS_HIP(hipSetDevice(s->ctx->device));
+    const size_t elem = s->precision == SSS_PRECISION_FP64 ? sizeof(double) : sizeof(float);
+    SSS_HIP(hipMemcpyAsync(host_out, s->d_out, (size_t)s->out_elems * elem, hipMemcpyDeviceToHost,
+                           s->ctx->stream));
+    SSS_HIP(hipStreamSynchronize(s->ctx->stream));
+    return SSS_OK;
+}
+
+int sss_subject_output_dense(sss_subject* s, const int16_t* lor_coords, float* host_out)
+{
+    if (!s || !lor_coords || !host_out) return fail(SSS_ERR_INVALID, "sss_subject_output_dense: NULL argument");
+    SSS_HIP(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->ctx->stream;
+    const size_t n_dense = (size_t)s->n_bins * (s->n_det + 1) * (s->n_det / 2) * s->nru * s->nru;
+    const size_t n_lor = (size_t)s->n_det * s->n_det * 2;
+    float* d_dense = nullptr;
+    int16_t* d_lor = nullptr;
+    int64_t tally = 0;
+    int rc;
+    if ((rc = dev_alloc(&d_dense, n_dense, &tally)) || (rc = dev_alloc(&d_lor, n_lor, &tally))) {
+        dev_free(d_dense); dev_free(d_lor);
+        return rc;
+    }
+    hipError_t e = hipMemsetAsync(d_dense, 0, n_dense * sizeof(float), st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_lor, lor_coords, n_lor * sizeof(int16_t), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = launch_dense_scatter_typed(s->d_out, s->precision == SSS_PRECISION_FP64, s->n_bins, s->nru,
+                                       s->ndu, s->n_det, s->d_dets, d_lor, d_dense, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(host_out, d_dense, n_dense * sizeof(float), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    dev_free(d_dense); dev_free(d_lor);
+    if (e != hipSuccess) return hip_fail(e, "sss_subject_output_dense");
+    return SSS_OK;
+}
+
+}  // extern "C"

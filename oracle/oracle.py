@@ -1,0 +1,269 @@
+"""ORACLE — test infrastructure only.
+
+Python side of the CPU float64 restatement of the reference TOF-SSS path
+(``oracle/sss_oracle.c``).  Imported by ``tests/``, ``__graft_entry__.smoke()``
+and ``bench.py``'s cpu_baseline leg, and only there, as the checker; the
+shipped package never imports it.
+
+Functions mirror the reference entry points they restate:
+
+* ``trace``          -> ``Python/openSSS/RayTracing3DTOF.py:29-180``
+* ``sss``            -> ``Python/openSSS/SingleScatterSimulationTOF.py:36-208,300-559``
+                        (explicit, slice-grouped scatter-point list; dense output)
+* ``line_integrals`` -> the per-pair trace + sums of ``TailsMask.py:139-148``
+* ``backproject``    -> ``BackProjection.py:31-113`` (mode 0) / ``:217-300`` (mode 1)
+* ``sinogram_coordinates_loops`` -> ``Sinogram.py:27-141`` as plain loops
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, 'build', 'liboracle.so')
+_lib = None
+
+LIGHT_CM_PS = 299792458.0 * 1e-10
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(['make', '-C', _HERE, '-B' if force else 'all'], check=True,
+                       stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _lib.ora_trace.restype = ctypes.c_int
+        _lib.ora_sss.restype = ctypes.c_int
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class _SssArgs(ctypes.Structure):
+    _fields_ = [
+        ('grid', ctypes.c_int32 * 3), ('bounds', ctypes.c_double * 6),
+        ('mu', ctypes.c_void_p), ('lam', ctypes.c_void_p),
+        ('n_rings', ctypes.c_int32), ('n_det', ctypes.c_int32),
+        ('geom', ctypes.c_void_p), ('normals', ctypes.c_void_p),
+        ('det_size', ctypes.c_double * 2),
+        ('ratio', ctypes.c_void_p), ('eff', ctypes.c_void_p),
+        ('n_energy', ctypes.c_int32), ('ref_row', ctypes.c_int32),
+        ('tof', ctypes.c_void_p), ('n_bins', ctypes.c_int32),
+        ('nru', ctypes.c_int32), ('ndu', ctypes.c_int32),
+        ('rings', ctypes.c_void_p), ('dets', ctypes.c_void_p),
+        ('sino', ctypes.c_void_p), ('cell_of', ctypes.c_void_p), ('n_cells', ctypes.c_int32),
+        ('n_sp', ctypes.c_int32), ('sp_xyz', ctypes.c_void_p),
+        ('n_slices', ctypes.c_int32), ('slice_ptr', ctypes.c_void_p),
+        ('n_threads', ctypes.c_int32), ('stat_oor', ctypes.c_void_p),
+    ]
+
+
+def trace(grid_size, grid_bounds, line):
+    """Returns (L f64, idx int32 1-based, s f64) exactly like RayTracing3DTOF."""
+    gs = np.ascontiguousarray(np.asarray(grid_size, dtype=np.int32))
+    gb = np.ascontiguousarray(np.asarray(grid_bounds, dtype=np.float64))
+    lc = np.ascontiguousarray(np.asarray(line, dtype=np.float64))
+    cap = int(gs.sum())
+    L = np.zeros(cap)
+    idx = np.zeros(cap, dtype=np.int32)
+    s = np.zeros(cap)
+    n = lib().ora_trace(_p(gs), _p(gb), _p(lc), _p(L), _p(idx), _p(s), ctypes.c_int(cap))
+    if n < 0:
+        raise RuntimeError('trace exceeded capacity')
+    return L[:n].copy(), idx[:n].copy(), s[:n].copy()
+
+
+def tof_table(n_bins, bin_width, resolution):
+    """The reference's TOFEfficiencyTable (``SingleScatterSimulationTOF.py:260-295``)."""
+    t = np.arange(-4000, 4001, dtype=np.float64)
+    shifts = ((np.arange(1, n_bins + 1) - (n_bins + 1) / 2) * bin_width).reshape(-1, 1)
+    p = t.reshape(1, -1) - shifts
+    p = np.exp(-(p ** 2) / (resolution ** 2 / 4 / np.log(2)))
+    zero = np.sum(p, axis=0) == 0
+    with np.errstate(invalid='ignore', divide='ignore'):
+        p = p / np.sum(p, axis=0)
+    p[:, zero] = 0
+    return p
+
+
+def energy_tables(attenuation_table, energy_resolution, energy_threshold):
+    from scipy.special import erf
+    tab = np.asarray(attenuation_table, dtype=np.float64)
+    ref = int(np.where(tab[:, 0] == 511)[0][0])
+    ratio = tab[:, 1] / tab[ref, 1]
+    eff = 0.5 * (1 - erf((energy_threshold - tab[:, 0]) / (511 * energy_resolution / 2 / np.sqrt(np.log(2)))))
+    return ratio, eff
+
+
+def sampled_cells(sino, dets):
+    """Compact id per (radial, angular) cell hit by a sampled pair (D1 <= D2)."""
+    N = sino.shape[0]
+    cell_of = np.full((N + 1) * (N // 2), -1, dtype=np.int32)
+    n = 0
+    for i1 in range(len(dets)):
+        for i2 in range(i1, len(dets)):
+            ang = int(sino[dets[i1], dets[i2], 0])
+            rad = int(sino[dets[i1], dets[i2], 1])
+            c = rad * (N // 2) + ang
+            if cell_of[c] < 0:
+                cell_of[c] = n
+                n += 1
+    return cell_of, n
+
+
+def sss(activity, attenuation, image_size, geometry, sino, normals, det_size, attenuation_table,
+        energy_resolution, energy_threshold, rings, dets, tof_range, tof_resolution, n_bins,
+        sp_points, slice_ptr, n_threads=1, dense=True, return_stats=False):
+    """Oracle TOF-SSS.  ``sp_points`` (n, 3) int voxel indices grouped by z (``slice_ptr``).
+
+    Returns float32 (B, N+1, N//2, NRU**2) when ``dense`` else the compact
+    (B, NRU**2, n_cells) array plus the (radial, angular) -> cell map.
+    """
+    X, Y, Z = attenuation.shape
+    mu = np.ascontiguousarray(np.asarray(attenuation, dtype=np.float64).flatten(order='F'))
+    lam = np.ascontiguousarray(np.asarray(activity, dtype=np.float64).flatten(order='F'))
+    geom = np.ascontiguousarray(geometry, dtype=np.float64)
+    nv = np.ascontiguousarray(normals, dtype=np.float64)
+    ratio, eff = energy_tables(attenuation_table, energy_resolution, energy_threshold)
+    ratio = np.ascontiguousarray(ratio)
+    eff = np.ascontiguousarray(eff)
+    bin_width = 2 * tof_range / n_bins
+    tof = np.ascontiguousarray(tof_table(n_bins, bin_width, tof_resolution))
+    rings = np.ascontiguousarray(np.asarray(rings, dtype=np.int32))
+    dets = np.ascontiguousarray(np.asarray(dets, dtype=np.int32))
+    sino16 = np.ascontiguousarray(np.asarray(sino, dtype=np.int16))
+    cell_of, n_cells = sampled_cells(sino16, dets)
+    isz = np.asarray(image_size, dtype=np.float64)
+    radius = -isz[0]
+    width = -isz[2]
+    zvox = (isz[5] - isz[2]) / Z
+    zstart = zvox / 2 - width
+    p = np.asarray(sp_points, dtype=np.int64).reshape(-1, 3)
+    xyz = np.empty((p.shape[0], 3))
+    xyz[:, 0] = ((p[:, 0] + 1) * 2 * radius / X) - radius
+    xyz[:, 1] = ((p[:, 1] + 1) * 2 * radius / Y) - radius
+    xyz[:, 2] = zstart + p[:, 2] * zvox
+    xyz = np.ascontiguousarray(xyz)
+    sptr = np.ascontiguousarray(np.asarray(slice_ptr, dtype=np.int32))
+    oor = np.zeros(1, dtype=np.int64)
+    a = _SssArgs()
+    a.grid[:] = [X, Y, Z]
+    a.bounds[:] = [float(v) for v in isz]
+    a.mu, a.lam = mu.ctypes.data, lam.ctypes.data
+    a.n_rings, a.n_det = geom.shape[0], geom.shape[1]
+    a.geom, a.normals = geom.ctypes.data, nv.ctypes.data
+    a.det_size[:] = [float(det_size[0]), float(det_size[1])]
+    a.ratio, a.eff = ratio.ctypes.data, eff.ctypes.data
+    a.n_energy = ratio.shape[0]
+    a.ref_row = 511 * 2 - 1
+    a.tof, a.n_bins = tof.ctypes.data, n_bins
+    a.nru, a.ndu = len(rings), len(dets)
+    a.rings, a.dets = rings.ctypes.data, dets.ctypes.data
+    a.sino, a.cell_of, a.n_cells = sino16.ctypes.data, cell_of.ctypes.data, n_cells
+    a.n_sp, a.sp_xyz = xyz.shape[0], xyz.ctypes.data
+    a.n_slices, a.slice_ptr = len(sptr) - 1, sptr.ctypes.data
+    a.n_threads = n_threads
+    a.stat_oor = oor.ctypes.data
+    nru = len(rings)
+    out = np.zeros((n_bins, nru * nru, n_cells), dtype=np.float32)
+    rc = lib().ora_sss(ctypes.byref(a), _p(out))
+    if rc != 0:
+        raise RuntimeError(f'ora_sss failed: {rc}')
+    stats = {'offsets_out_of_range': int(oor[0])}
+    if dense:
+        N = geom.shape[1]
+        res = np.zeros((n_bins, N + 1, N // 2, nru * nru), dtype=np.float32)
+        cells = np.nonzero(cell_of >= 0)[0]
+        rad, ang = cells // (N // 2), cells % (N // 2)
+        res[:, rad, ang, :] = np.transpose(out[:, :, cell_of[cells]], (0, 2, 1))
+        return (res, stats) if return_stats else res
+    return (out, cell_of, stats) if return_stats else (out, cell_of)
+
+
+def line_integrals(grid_size, grid_bounds, mu_flat_f, lam_flat_f, segs):
+    gs = np.ascontiguousarray(np.asarray(grid_size, dtype=np.int32))
+    gb = np.ascontiguousarray(np.asarray(grid_bounds, dtype=np.float64))
+    mu = np.ascontiguousarray(mu_flat_f, dtype=np.float64)
+    lam = np.ascontiguousarray(lam_flat_f, dtype=np.float64)
+    segs = np.ascontiguousarray(segs, dtype=np.float64).reshape(-1, 6)
+    n = np.zeros(segs.shape[0], dtype=np.int32)
+    m = np.zeros(segs.shape[0])
+    l = np.zeros(segs.shape[0])
+    lib().ora_line_integrals(_p(gs), _p(gb), _p(mu), _p(lam), _p(segs),
+                             ctypes.c_int64(segs.shape[0]), _p(n), _p(m), _p(l))
+    return n, m, l
+
+
+def backproject(grid_size, grid_bounds, segs, v_prompt, v_scatter, v_random, mode):
+    gs = np.ascontiguousarray(np.asarray(grid_size, dtype=np.int32))
+    gb = np.ascontiguousarray(np.asarray(grid_bounds, dtype=np.float64))
+    segs = np.ascontiguousarray(segs, dtype=np.float64).reshape(-1, 6)
+    vp = np.ascontiguousarray(v_prompt, dtype=np.float32)
+    vs = np.ascontiguousarray(v_scatter, dtype=np.float32)
+    vr = np.ascontiguousarray(v_random, dtype=np.float32)
+    nvox = int(np.prod(gs))
+    imgs = [np.zeros(nvox, dtype=np.float32) for _ in range(4)]
+    lib().ora_backproject(_p(gs), _p(gb), _p(segs), ctypes.c_int64(segs.shape[0]),
+                          _p(vp), _p(vs), _p(vr), ctypes.c_int(mode),
+                          *[_p(i) for i in imgs])
+    return imgs
+
+
+def sinogram_coordinates_loops(n_sectors_trans, n_sectors_axial, n_modules_axial, n_modules_trans,
+                               n_crystals_trans, n_crystals_axial, min_sector_difference=0,
+                               id_shift=True):
+    """Plain-loop restatement of ``Sinogram.py:55-141`` (small scanners only)."""
+    NR = n_sectors_axial * n_modules_axial * n_crystals_axial
+    N = n_sectors_trans * n_modules_trans * n_crystals_trans
+    mcd = min_sector_difference * n_modules_trans * n_crystals_trans
+    radial_size = N - 2 * (mcd - 1) - 1
+    shift = (n_modules_trans * n_crystals_trans) // 2 if id_shift else 0
+    lor = np.zeros((N, N, 2)) - 1
+    for d1 in range(N):
+        c1 = d1 % N - shift
+        for d2 in range(N):
+            c2 = d2 % N - shift
+            if c1 < 0:
+                c1 += N
+            if c2 < 0:
+                c2 += N
+            ida, idb = (c1, c2) if c1 < c2 else (c2, c1)
+            if idb - ida < mcd:
+                continue
+            if ida + idb >= (3 * N) // 2 or ida + idb < N // 2:
+                radial = -N // 2 if ida == idb else ((idb - ida - 1) // 2) - ((N - (idb - ida + 1)) // 2)
+            else:
+                radial = N // 2 if ida == idb else ((N - (idb - ida + 1)) // 2) - ((idb - ida - 1) // 2)
+            if ida + idb < N // 2:
+                angular = (2 * ida + N + radial) // 2
+            elif ida + idb >= (3 * N) // 2:
+                angular = (2 * ida - N + radial) // 2
+            else:
+                angular = (2 * ida - radial) // 2
+            lor[d1, d2, 0] = angular
+            lor[d1, d2, 1] = radial + radial_size // 2
+    sino = np.zeros((NR, NR))
+    for r1 in range(NR):
+        for r2 in range(NR):
+            if r1 == r2:
+                idx = r1
+            elif r2 < r1:
+                d = r1 - r2
+                idx = (2 * d - 1) * NR - d * (d - 1) + r2
+            else:
+                d = r2 - r1
+                idx = (2 * d - 1) * NR - d * (d - 1) + (NR - d) + r1
+            sino[r1, r2] = idx
+    return lor.astype(np.int16), sino.astype(np.int16), shift
