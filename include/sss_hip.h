@@ -86,6 +86,7 @@ typedef struct {
     int64_t evals;               /* n_pairs_enumerated * n_bins */
     int64_t device_bytes;        /* resident device memory of the subject */
     int32_t n_tiles, n_chunks;
+    int32_t n_truncated_walks;   /* half paths whose walk stopped early (voxel-face runs) */
 } sss_subject_info;
 
 const char* sss_last_error(void);
@@ -126,7 +127,7 @@ int sss_subject_set_activity_device(sss_subject* s, const void* lam_dev);
 int sss_subject_scatter(sss_subject* s);
 /* Trace (if needed) + scatter in one call: the plain one-instance path. */
 int sss_subject_run(sss_subject* s);
-int sss_subject_info_get(const sss_subject* s, sss_subject_info* out);
+int sss_subject_info_get(sss_subject* s, sss_subject_info* out);
 /* Pairs passing the reference's filters (A != 0, some activity, efficiency
  * != 0; SingleScatterSimulationTOF.py:451-509) and the voxel samples their TOF
  * folds visit, for the algorithmic work count.  Synchronous. */
@@ -139,6 +140,49 @@ int sss_subject_output_compact(sss_subject* s, float* host_out);
 /* Dense reference layout (B, N+1, N/2, NRU^2) float32 on the host, using
  * lor_coords (N, N, 2) int16 = (angular, radial). */
 int sss_subject_output_dense(sss_subject* s, const int16_t* lor_coords, float* host_out);
+
+/* Device buffers on the context's device (synchronous copies). */
+int sss_buffer_alloc(sss_ctx* ctx, int64_t bytes, void** dev);
+int sss_buffer_free(sss_ctx* ctx, void* dev);
+int sss_buffer_upload(sss_ctx* ctx, void* dev, const void* host, int64_t bytes);
+int sss_buffer_download(sss_ctx* ctx, void* host, const void* dev, int64_t bytes);
+
+/* Sampled -> full sinogram (Python/openSSS/Interpolation.py:95-399), device
+ * pointers, asynchronous on the context stream.  sampled is the compact
+ * [B][n_sampled_planes][cells] estimate; cell_map[c] (c = radial * n_ang +
+ * angular of the output) is the compact cell of the nearest sampled detector
+ * pair, plane_map[s] the sampled plane of the nearest sampled ring pair of
+ * output plane s (-1: zero).  Writes out[b][c][s] = max(value, 0) (may be
+ * NULL) and sum[c][s] = sum over bins (may be NULL). */
+int sss_interpolate(sss_ctx* ctx, const float* sampled, int n_bins, int n_sampled_planes,
+                    int64_t cells_per_sampled_plane, const int32_t* cell_map, int64_t n_cells_out,
+                    const int32_t* plane_map, int n_out_planes, float* out, float* sum);
+
+/* Tail mask (Python/openSSS/TailsMask.py:84-163 no span, :230-309 span), host
+ * pointers.  For output plane p the LORs run from detector d1 of geometry ring
+ * plane_rings[2p] to d2 of ring plane_rings[2p+1], d1 <= d2 on the
+ * accel-strided detector grid; mask[(radial * N/2 + angular) * n_planes + p]
+ * = 1 where 1/exp(-sum mu L) == 1 in float64 (use_attenuation) or where
+ * sum lam L <= 0, else 0; cells no LOR reaches stay 0. */
+int sss_tail_mask(sss_ctx* ctx, const int32_t grid[3], const double bounds[6], const double* image,
+                  int use_attenuation, int n_rings_geom, int n_det, const double* geom,
+                  const int16_t* lor_coords, int n_planes, const int32_t* plane_rings, int accel,
+                  uint8_t* mask_out);
+
+/* Backprojection tail fit (Python/openSSS/BackProjection.py:31-392), host
+ * pointers.  Entry e traces d1 of ring plane_rings[3e] to d2 of ring
+ * plane_rings[3e+1] and reads its values from plane plane_rings[3e+2] of the
+ * (N+1, N/2, n_value_planes) sinograms; LORs whose masked scatter is 0 are
+ * skipped.  Values are spread value/n per voxel, or value*L/sum(L) when
+ * length_weighted, into float64 images (prompts, scatter, randoms,
+ * sensitivity).  sums_out = (sum S_bp T, sum S_bp^2) over voxels with
+ * S_bp = scatter/sens, T = max(prompts/sens - randoms/sens, 0); the fitted
+ * scale is sums_out[0] / sums_out[1].  images_out (4 * X*Y*Z) may be NULL. */
+int sss_tail_fit(sss_ctx* ctx, const int32_t grid[3], const double bounds[6], int n_rings_geom, int n_det,
+                 const double* geom, const int16_t* lor_coords, int n_entries, const int32_t* plane_rings,
+                 int n_value_planes, int accel, int length_weighted, const float* scatter,
+                 const float* prompts, const float* randoms, const float* mask, double* sums_out,
+                 double* images_out);
 
 #ifdef __cplusplus
 }

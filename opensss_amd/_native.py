@@ -59,6 +59,7 @@ class SubjectInfo(ctypes.Structure):
         ('n_cells', ctypes.c_int64), ('n_pairs_enumerated', ctypes.c_int64),
         ('evals', ctypes.c_int64), ('device_bytes', ctypes.c_int64),
         ('n_tiles', ctypes.c_int32), ('n_chunks', ctypes.c_int32),
+        ('n_truncated_walks', ctypes.c_int32),
     ]
 
 
@@ -92,7 +93,67 @@ SIGNATURES = [
     ('sss_subject_output_device', ctypes.c_void_p, [ctypes.c_void_p]),
     ('sss_subject_output_compact', ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ('sss_subject_output_dense', ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ('sss_buffer_alloc', ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)]),
+    ('sss_buffer_free', ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ('sss_buffer_upload', ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
+    ('sss_buffer_download', ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
+    ('sss_interpolate', ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_void_p,
+      ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    ('sss_tail_mask', ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+      ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+      ctypes.c_void_p]),
+    ('sss_tail_fit', ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.c_void_p]),
 ]
+
+
+class DeviceBuffer:
+    """A device allocation owned by a context (``sss_buffer_*``)."""
+
+    def __init__(self, ctx: 'Context', nbytes: int):
+        self._lib = load_library()
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        check(self._lib.sss_buffer_alloc(ctx.handle, self.nbytes, ctypes.byref(p)), 'sss_buffer_alloc')
+        self.ptr = p
+
+    @classmethod
+    def from_array(cls, ctx, arr: np.ndarray) -> 'DeviceBuffer':
+        a = np.ascontiguousarray(arr)
+        buf = cls(ctx, a.nbytes)
+        buf.upload(a)
+        return buf
+
+    def upload(self, arr: np.ndarray) -> None:
+        a = np.ascontiguousarray(arr)
+        if a.nbytes > self.nbytes:
+            raise ValueError('array larger than the device buffer')
+        check(self._lib.sss_buffer_upload(self.ctx.handle, self.ptr, a.ctypes.data_as(ctypes.c_void_p),
+                                          a.nbytes), 'sss_buffer_upload')
+
+    def download(self, out: np.ndarray) -> np.ndarray:
+        if not out.flags['C_CONTIGUOUS'] or out.nbytes > self.nbytes:
+            raise ValueError('bad host buffer')
+        check(self._lib.sss_buffer_download(self.ctx.handle, out.ctypes.data_as(ctypes.c_void_p),
+                                            self.ptr, out.nbytes), 'sss_buffer_download')
+        return out
+
+    def free(self) -> None:
+        if getattr(self, 'ptr', None):
+            self._lib.sss_buffer_free(self.ctx.handle, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 _lib = None
 _lock = threading.Lock()

@@ -51,7 +51,8 @@ struct SampleTable {
     int32_t* vox;      // 0-based voxel (reference linear index - 1)
     double* len;       // intersection length, cm
     double* q;         // 2 s_k / c, ps
-    float* a;          // lam * len
+    float* a;          // lam * len (FP32 fold input)
+    double* a64;       // lam * len in float64 (FP64 validation path only, else null)
 };
 
 struct ScatterArgs {
@@ -67,17 +68,21 @@ struct ScatterArgs {
     const double* hp_lnatt;
     const double* hp_act;
     const float* hp_cos;
-    const double* s_q;
-    const float* s_a;
+    const double* s_q;           // TOF sample times 2 s_k / c (float64, both paths)
+    const float* s_a;            // FP32 path
+    const double* s_a64;         // FP64 path
     const double* detpos;        // (NRU, NDU, 3)
     const double* eff;           // n_energy
     const float* ratio;          // n_energy
     const int4* tiles;           // (r1, r2, i1_0, i2_0)
     const int32_t* slice_ptr;    // n_slices + 1
     const int32_t* chunk_slices; // n_chunks + 1 (slice ranges)
-    // TOF kernel
-    float inv_w_bins, mid, bin_w, kappa2, beta2;   // log2 scaled
+    // TOF kernel: bins b at s_b = (b - mid) W; weights of bin j0 + d relative to
+    // the nearest bin j0 are 2^(d ku - d^2 beta2), ku = kappa2 (t - s_j0)
+    double inv_w_bins, mid, bin_w, kappa2, beta2;  // log2 scaled kappa, beta
+    double c_beta;                                 // 2^-beta2
     int half_window, t_lo, t_hi;
+    int single_exp;                                // h = 1 and beta2 small: one exp2 per sample
     double g_const;              // (Dx Dy 1e-2)^2 / (4 pi)
     double kn_gamma;
     // output
@@ -103,6 +108,16 @@ hipError_t launch_scatter(const ScatterArgs& a, bool fp64, bool trivial, hipStre
 hipError_t launch_pair_stats(const ScatterArgs& a, unsigned long long* out, hipStream_t st);
 hipError_t launch_chunk_reduce_typed(const void* partial, int n_chunks, int64_t n, void* out,
                                      bool fp64, hipStream_t st);
+hipError_t launch_interp(const float* sampled, int n_bins, int n_sampled_planes, int64_t cells_s,
+                         const int32_t* cell_map, int64_t n_cells_out, const int32_t* plane_map,
+                         int n_out_planes, float* out, float* sum, hipStream_t st);
+hipError_t launch_tail_mask(const Grid& g, const double* image, int use_attenuation, const double* geom,
+                            int n_det, const int16_t* lor, int n_planes, const int32_t* plane_rings,
+                            int accel, uint8_t* mask, hipStream_t st);
+hipError_t launch_fit(const Grid& g, const double* geom, int n_det, const int16_t* lor, int n_entries,
+                      const int32_t* plane_rings, int n_value_planes, int accel, int length_weighted,
+                      const float* scatter, const float* prompts, const float* randoms, const float* mask,
+                      double* img, double* sums, hipStream_t st);
 hipError_t launch_dense_scatter_typed(const void* compact, bool fp64, int n_bins, int nru, int ndu,
                                       int n_det, const int32_t* dets, const int16_t* lor,
                                       float* dense, hipStream_t st);

@@ -37,13 +37,14 @@ int hip_fail(hipError_t e, const char* what)
 
 namespace {
 
-__global__ void k_max_status(const int32_t* __restrict__ st, int64_t n, int32_t* out)
+// Number of half paths whose voxel walk was truncated (status 1).
+__global__ void k_count_status(const int32_t* __restrict__ st, int64_t n, int32_t* out)
 {
     int32_t m = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
-        m = st[i] > m ? st[i] : m;
-    if (m) atomicMax(out, m);
+        m += st[i];
+    if (m) atomicAdd(out, m);
 }
 
 template <typename T>
@@ -121,7 +122,7 @@ struct sss_subject {
         dev_free(d_chunk_slices); dev_free(d_dets); dev_free(d_flag); dev_free(d_scan_tmp);
         dev_free(hp.off); dev_free(hp.n); dev_free(hp.dist); dev_free(hp.att); dev_free(hp.lnatt);
         dev_free(hp.act); dev_free(hp.cosabs); dev_free(hp.status);
-        dev_free(smp.vox); dev_free(smp.len); dev_free(smp.q); dev_free(smp.a);
+        dev_free(smp.vox); dev_free(smp.len); dev_free(smp.q); dev_free(smp.a); dev_free(smp.a64);
         dev_free(d_out); dev_free(d_partial);
     }
 };
@@ -451,25 +452,25 @@ int sss_subject_trace(sss_subject* s)
     SSS_HIP(hipStreamSynchronize(st));
     if (total > s->samples_cap) {
         dev_free(s->smp.vox); dev_free(s->smp.len); dev_free(s->smp.q); dev_free(s->smp.a);
+        dev_free(s->smp.a64);
         s->samples_cap = 0;
         int rc;
         if ((rc = dev_alloc(&s->smp.vox, (size_t)total, &s->device_bytes)) ||
             (rc = dev_alloc(&s->smp.len, (size_t)total, &s->device_bytes)) ||
             (rc = dev_alloc(&s->smp.q, (size_t)total, &s->device_bytes)) ||
-            (rc = dev_alloc(&s->smp.a, (size_t)total, &s->device_bytes)))
+            (rc = dev_alloc(&s->smp.a, (size_t)total, &s->device_bytes)) ||
+            (s->precision == SSS_PRECISION_FP64 &&
+             (rc = dev_alloc(&s->smp.a64, (size_t)total, &s->device_bytes))))
             return rc;
         s->samples_cap = total;
     }
     s->n_samples = total;
     SSS_HIP(launch_halfpath_fill(s->grid, s->d_sp, s->n_sp, s->d_detpos, s->nru, s->ndu, s->d_mu, s->hp,
                                  s->smp, st));
+    // truncated walks are counted (read by sss_subject_info_get), not an error
     SSS_HIP(hipMemsetAsync(s->d_flag, 0, sizeof(int32_t), st));
-    hipLaunchKernelGGL(k_max_status, dim3(256), dim3(256), 0, st, s->hp.status, s->n_hp, s->d_flag);
+    hipLaunchKernelGGL(k_count_status, dim3(256), dim3(256), 0, st, s->hp.status, s->n_hp, s->d_flag);
     SSS_HIP(hipGetLastError());
-    int32_t flag = 0;
-    SSS_HIP(hipMemcpyAsync(&flag, s->d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    SSS_HIP(hipStreamSynchronize(st));
-    if (flag) return fail(SSS_ERR_TRACE, "sss_subject_trace: a voxel walk did not end at its end voxel");
     s->traced = true;
     if (s->has_activity) SSS_HIP(launch_activity_refresh(s->hp, s->smp, s->n_hp, s->d_lam, st));
     return SSS_OK;
@@ -508,16 +509,19 @@ static ScatterArgs scatter_args(const sss_subject* s)
     a.n_tiles = s->n_tiles; a.n_chunks = s->n_chunks; a.n_cells_plane = s->n_cells_plane;
     a.hp_off = s->hp.off; a.hp_n = s->hp.n; a.hp_dist = s->hp.dist; a.hp_att = s->hp.att;
     a.hp_lnatt = s->hp.lnatt; a.hp_act = s->hp.act; a.hp_cos = s->hp.cosabs;
-    a.s_q = s->smp.q; a.s_a = s->smp.a;
+    a.s_q = s->smp.q; a.s_a = s->smp.a; a.s_a64 = s->smp.a64;
     a.detpos = s->d_detpos; a.eff = s->d_eff; a.ratio = s->d_ratio; a.tiles = s->d_tiles;
     a.slice_ptr = s->d_slice_ptr; a.chunk_slices = s->d_chunk_slices;
     const double log2e = 1.4426950408889634;
-    a.inv_w_bins = (float)(1.0 / s->bin_w);
-    a.mid = (float)((s->n_bins - 1) / 2.0);
-    a.bin_w = (float)s->bin_w;
-    a.kappa2 = (float)(s->kappa * log2e);
-    a.beta2 = (float)(s->beta * log2e);
+    a.inv_w_bins = 1.0 / s->bin_w;
+    a.mid = (s->n_bins - 1) / 2.0;
+    a.bin_w = s->bin_w;
+    a.kappa2 = s->kappa * log2e;
+    a.beta2 = s->beta * log2e;
+    a.c_beta = std::exp2(-a.beta2);
     a.half_window = s->half_window; a.t_lo = s->t_lo; a.t_hi = s->t_hi;
+    // one exp2 per sample needs E^2 = 2^(2 ku) finite inside the bins: ku <= beta2
+    a.single_exp = (s->half_window == 1 && a.beta2 <= 60.0) ? 1 : 0;
     a.g_const = s->g_const;
     a.kn_gamma = (511.0e3 * 1.6e-19) / (9.10938356e-31 * (299792458.0 * 299792458.0));
     a.out = s->n_chunks > 1 ? s->d_partial : s->d_out;
@@ -581,7 +585,7 @@ int sss_subject_run(sss_subject* s)
     return sss_subject_scatter(s);
 }
 
-int sss_subject_info_get(const sss_subject* s, sss_subject_info* out)
+int sss_subject_info_get(sss_subject* s, sss_subject_info* out)
 {
     if (!s || !out) return fail(SSS_ERR_INVALID, "sss_subject_info_get: NULL argument");
     out->n_halfpaths = s->n_hp;
@@ -594,6 +598,13 @@ int sss_subject_info_get(const sss_subject* s, sss_subject_info* out)
     out->device_bytes = s->device_bytes;
     out->n_tiles = s->n_tiles;
     out->n_chunks = s->n_chunks;
+    int32_t trunc = 0;
+    if (s->traced) {
+        SSS_HIP(hipSetDevice(s->ctx->device));
+        SSS_HIP(hipMemcpyAsync(&trunc, s->d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s->ctx->stream));
+        SSS_HIP(hipStreamSynchronize(s->ctx->stream));
+    }
+    out->n_truncated_walks = trunc;
     return SSS_OK;
 }
 
@@ -634,6 +645,176 @@ int sss_subject_output_dense(sss_subject* s, const int16_t* lor_coords, float* h
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     dev_free(d_dense); dev_free(d_lor);
     if (e != hipSuccess) return hip_fail(e, "sss_subject_output_dense");
+    return SSS_OK;
+}
+
+/* ---- device buffers ---------------------------------------------------- */
+
+int sss_buffer_alloc(sss_ctx* ctx, int64_t bytes, void** dev)
+{
+    if (!ctx || !dev || bytes < 0) return fail(SSS_ERR_INVALID, "sss_buffer_alloc: bad argument");
+    SSS_HIP(hipSetDevice(ctx->device));
+    unsigned char* p = nullptr;
+    int rc = dev_alloc(&p, (size_t)(bytes > 0 ? bytes : 1), nullptr);
+    if (rc) return rc;
+    *dev = p;
+    return SSS_OK;
+}
+
+int sss_buffer_free(sss_ctx* ctx, void* dev)
+{
+    if (!ctx) return fail(SSS_ERR_INVALID, "sss_buffer_free: NULL context");
+    if (!dev) return SSS_OK;
+    SSS_HIP(hipSetDevice(ctx->device));
+    SSS_HIP(hipStreamSynchronize(ctx->stream));
+    SSS_HIP(hipFree(dev));
+    return SSS_OK;
+}
+
+int sss_buffer_upload(sss_ctx* ctx, void* dev, const void* host, int64_t bytes)
+{
+    if (!ctx || (bytes > 0 && (!dev || !host)) || bytes < 0)
+        return fail(SSS_ERR_INVALID, "sss_buffer_upload: bad argument");
+    if (bytes == 0) return SSS_OK;
+    SSS_HIP(hipSetDevice(ctx->device));
+    SSS_HIP(hipMemcpyAsync(dev, host, (size_t)bytes, hipMemcpyHostToDevice, ctx->stream));
+    SSS_HIP(hipStreamSynchronize(ctx->stream));
+    return SSS_OK;
+}
+
+int sss_buffer_download(sss_ctx* ctx, void* host, const void* dev, int64_t bytes)
+{
+    if (!ctx || (bytes > 0 && (!dev || !host)) || bytes < 0)
+        return fail(SSS_ERR_INVALID, "sss_buffer_download: bad argument");
+    if (bytes == 0) return SSS_OK;
+    SSS_HIP(hipSetDevice(ctx->device));
+    SSS_HIP(hipMemcpyAsync(host, dev, (size_t)bytes, hipMemcpyDeviceToHost, ctx->stream));
+    SSS_HIP(hipStreamSynchronize(ctx->stream));
+    return SSS_OK;
+}
+
+/* ---- full-sinogram stages ---------------------------------------------- */
+
+int sss_interpolate(sss_ctx* ctx, const float* sampled, int n_bins, int n_sampled_planes,
+                    int64_t cells_per_sampled_plane, const int32_t* cell_map, int64_t n_cells_out,
+                    const int32_t* plane_map, int n_out_planes, float* out, float* sum)
+{
+    if (!ctx || !sampled || !cell_map || !plane_map || (!out && !sum) || n_bins < 1 ||
+        n_sampled_planes < 1 || cells_per_sampled_plane < 1 || n_cells_out < 0 || n_out_planes < 0)
+        return fail(SSS_ERR_INVALID, "sss_interpolate: bad argument");
+    SSS_HIP(hipSetDevice(ctx->device));
+    SSS_HIP(launch_interp(sampled, n_bins, n_sampled_planes, cells_per_sampled_plane, cell_map,
+                          n_cells_out, plane_map, n_out_planes, out, sum, ctx->stream));
+    return SSS_OK;
+}
+
+static int check_planes(const int32_t* plane_rings, int n, int stride, int n_rings_geom, int n_value_planes)
+{
+    for (int p = 0; p < n; ++p) {
+        const int ra = plane_rings[stride * p], rb = plane_rings[stride * p + 1];
+        if (ra < 0 || ra >= n_rings_geom || rb < 0 || rb >= n_rings_geom)
+            return fail(SSS_ERR_INVALID, "plane ring out of range");
+        if (stride == 3 && (plane_rings[3 * p + 2] < 0 || plane_rings[3 * p + 2] >= n_value_planes))
+            return fail(SSS_ERR_INVALID, "value plane out of range");
+    }
+    return SSS_OK;
+}
+
+int sss_tail_mask(sss_ctx* ctx, const int32_t grid[3], const double bounds[6], const double* image,
+                  int use_attenuation, int n_rings_geom, int n_det, const double* geom,
+                  const int16_t* lor_coords, int n_planes, const int32_t* plane_rings, int accel,
+                  uint8_t* mask_out)
+{
+    if (!ctx || !image || !geom || !lor_coords || !plane_rings || !mask_out || n_det < 2 ||
+        n_rings_geom < 1 || n_planes < 0 || accel < 1)
+        return fail(SSS_ERR_INVALID, "sss_tail_mask: bad argument");
+    Grid g;
+    int rc = check_grid(grid, bounds, &g);
+    if (rc) return rc;
+    if ((rc = check_planes(plane_rings, n_planes, 2, n_rings_geom, 0))) return rc;
+    SSS_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const size_t nvox = (size_t)g.nx * g.ny * g.nz;
+    const size_t ngeom = (size_t)n_rings_geom * n_det * 3, nlor = (size_t)n_det * n_det * 2;
+    const size_t nmask = (size_t)(n_det + 1) * (n_det / 2) * n_planes;
+    double *d_img = nullptr, *d_geom = nullptr;
+    int16_t* d_lor = nullptr;
+    int32_t* d_pr = nullptr;
+    uint8_t* d_mask = nullptr;
+    int64_t tally = 0;
+    if ((rc = dev_alloc(&d_img, nvox, &tally)) || (rc = dev_alloc(&d_geom, ngeom, &tally)) ||
+        (rc = dev_alloc(&d_lor, nlor, &tally)) || (rc = dev_alloc(&d_pr, (size_t)2 * n_planes + 1, &tally)) ||
+        (rc = dev_alloc(&d_mask, nmask + 1, &tally))) {
+        dev_free(d_img); dev_free(d_geom); dev_free(d_lor); dev_free(d_pr); dev_free(d_mask);
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(d_img, image, nvox * sizeof(double), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_geom, geom, ngeom * sizeof(double), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_lor, lor_coords, nlor * sizeof(int16_t), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && n_planes > 0)
+        e = hipMemcpyAsync(d_pr, plane_rings, (size_t)2 * n_planes * sizeof(int32_t), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemsetAsync(d_mask, 0, nmask + 1, st);
+    if (e == hipSuccess)
+        e = launch_tail_mask(g, d_img, use_attenuation, d_geom, n_det, d_lor, n_planes, d_pr, accel, d_mask, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(mask_out, d_mask, nmask, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    dev_free(d_img); dev_free(d_geom); dev_free(d_lor); dev_free(d_pr); dev_free(d_mask);
+    if (e != hipSuccess) return hip_fail(e, "sss_tail_mask");
+    return SSS_OK;
+}
+
+int sss_tail_fit(sss_ctx* ctx, const int32_t grid[3], const double bounds[6], int n_rings_geom, int n_det,
+                 const double* geom, const int16_t* lor_coords, int n_entries, const int32_t* plane_rings,
+                 int n_value_planes, int accel, int length_weighted, const float* scatter,
+                 const float* prompts, const float* randoms, const float* mask, double* sums_out,
+                 double* images_out)
+{
+    if (!ctx || !geom || !lor_coords || !plane_rings || !scatter || !prompts || !randoms || !mask ||
+        !sums_out || n_det < 2 || n_rings_geom < 1 || n_entries < 0 || n_value_planes < 1 || accel < 1)
+        return fail(SSS_ERR_INVALID, "sss_tail_fit: bad argument");
+    Grid g;
+    int rc = check_grid(grid, bounds, &g);
+    if (rc) return rc;
+    if ((rc = check_planes(plane_rings, n_entries, 3, n_rings_geom, n_value_planes))) return rc;
+    SSS_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const size_t nvox = (size_t)g.nx * g.ny * g.nz;
+    const size_t ngeom = (size_t)n_rings_geom * n_det * 3, nlor = (size_t)n_det * n_det * 2;
+    const size_t nsino = (size_t)(n_det + 1) * (n_det / 2) * n_value_planes;
+    double *d_geom = nullptr, *d_img = nullptr, *d_sums = nullptr;
+    int16_t* d_lor = nullptr;
+    int32_t* d_pr = nullptr;
+    float *d_s = nullptr, *d_p = nullptr, *d_r = nullptr, *d_m = nullptr;
+    int64_t tally = 0;
+    if ((rc = dev_alloc(&d_geom, ngeom, &tally)) || (rc = dev_alloc(&d_img, 4 * nvox, &tally)) ||
+        (rc = dev_alloc(&d_sums, 2, &tally)) || (rc = dev_alloc(&d_lor, nlor, &tally)) ||
+        (rc = dev_alloc(&d_pr, (size_t)3 * n_entries + 1, &tally)) || (rc = dev_alloc(&d_s, nsino, &tally)) ||
+        (rc = dev_alloc(&d_p, nsino, &tally)) || (rc = dev_alloc(&d_r, nsino, &tally)) ||
+        (rc = dev_alloc(&d_m, nsino, &tally))) {
+        dev_free(d_geom); dev_free(d_img); dev_free(d_sums); dev_free(d_lor); dev_free(d_pr);
+        dev_free(d_s); dev_free(d_p); dev_free(d_r); dev_free(d_m);
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(d_geom, geom, ngeom * sizeof(double), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_lor, lor_coords, nlor * sizeof(int16_t), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && n_entries > 0)
+        e = hipMemcpyAsync(d_pr, plane_rings, (size_t)3 * n_entries * sizeof(int32_t), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_s, scatter, nsino * sizeof(float), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_p, prompts, nsino * sizeof(float), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_r, randoms, nsino * sizeof(float), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_m, mask, nsino * sizeof(float), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemsetAsync(d_img, 0, 4 * nvox * sizeof(double), st);
+    if (e == hipSuccess) e = hipMemsetAsync(d_sums, 0, 2 * sizeof(double), st);
+    if (e == hipSuccess)
+        e = launch_fit(g, d_geom, n_det, d_lor, n_entries, d_pr, n_value_planes, accel, length_weighted, d_s,
+                       d_p, d_r, d_m, d_img, d_sums, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(sums_out, d_sums, 2 * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && images_out)
+        e = hipMemcpyAsync(images_out, d_img, 4 * nvox * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    dev_free(d_geom); dev_free(d_img); dev_free(d_sums); dev_free(d_lor); dev_free(d_pr);
+    dev_free(d_s); dev_free(d_p); dev_free(d_r); dev_free(d_m);
+    if (e != hipSuccess) return hip_fail(e, "sss_tail_fit");
     return SSS_OK;
 }
 

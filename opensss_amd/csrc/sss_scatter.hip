@@ -1,25 +1,29 @@
 // Fused single-scatter sum (the bodySSS LOR loop, SingleScatterSimulationTOF.py:433-557).
 //
 // Work decomposition: one wavefront owns an 8 x 8 tile of cells (i1, i2) of
-// one ring-pair plane (r1, r2) and keeps that tile for a contiguous range of
-// normalisation slices (a chunk).  Each lane accumulates its cell over every
-// scatter point of a slice, then adds sum/count of the slice (the per-slice
-// normalisation of :555-557) into its running output.  No atomics on global
-// memory and a fixed reduction order: results do not depend on what else
-// runs on the device.
+// one ring-pair plane (r1, r2) for a contiguous range of normalisation
+// slices (a chunk).  Each lane accumulates its own cell over the scatter
+// points of a slice, then adds sum/count of the slice (the per-slice
+// normalisation of :555-557) into its running output.  No global atomics and
+// a fixed reduction order: results do not depend on what else runs.
+//
+// Every lane walks the slice's scatter points with its own cursor and skips
+// the pairs the reference filters out (attenuation 0, no activity on either
+// path, detection efficiency 0 -- about half of them are backscatter beyond
+// the energy threshold), so the TOF fold only ever runs on lanes with work.
 //
 // Per (scatter point, cell) the lane evaluates, in float64: the distances,
-// the Compton energy index floor(511/(2 + cos_arg) + 0.5) (the same value as
-// the reference's acos/degrees/cos round trip at :495-501) and the pair's
-// TOF time shift; in R (float or double): Klein-Nishina, efficiency, solid
-// angle, attenuation scaling and the TOF fold.  The fold visits every voxel
-// sample of both half paths, rounds the time offset to the reference's
-// integer picosecond (:520,526), and adds the sample into the nearest TOF
-// bin and its +-h neighbours with the normalised Gaussian weights (the rows
-// of the reference's TOF table, evaluated analytically).  Fold accumulators
-// live in LDS, one column per lane, so per-lane bin indices need no
-// register indexing.  Sums over scatter points and slices are compensated
-// (Kahan) in the float32 path.
+// the Compton energy index floor(511/(2 + cos_arg) + 0.5) (the value of the
+// reference's acos/degrees/cos round trip at :495-501) and the pair's TOF
+// time shift; in R (float or double): Klein-Nishina, efficiency, solid angle,
+// attenuation scaling and the TOF fold.  The fold visits every voxel sample
+// of both half paths in one loop, rounds the time offset to the reference's
+// integer picosecond in float64 (:520,526) and adds the sample into the nearest TOF bin and
+// its +-h neighbours with the normalised Gaussian weights (the columns of the
+// reference's TOF table, evaluated analytically).  Fold accumulators live in
+// LDS, one column per lane, so per-lane bin indices need no register
+// indexing.  Sums over scatter points and slices are compensated (Kahan) in
+// the FP32 path.
 #include "sss_internal.h"
 
 namespace sss {
@@ -52,58 +56,202 @@ __device__ __forceinline__ R fast_rcp(R x)
     else return 1.0 / x;
 }
 
-// Adds coef * a_k * T[b, t_k] for every sample of one half path into the
-// lane's LDS column (rows b + h).  sign = -1 for the D1 path, +1 for D2.
-template <typename R, int HM>
-__device__ __forceinline__ void tof_fold(const ScatterArgs& a, R* __restrict__ col, int64_t base,
-                                         int n, double shift, double sign, R coef, int n_bins)
+template <typename R>
+__device__ __forceinline__ R clampr(R x, R lo, R hi)
 {
-    const int h = a.half_window;
-    const R inv_w = (R)a.inv_w_bins, mid = (R)a.mid, bw = (R)a.bin_w;
-    const R kappa2 = (R)a.kappa2, beta2 = (R)a.beta2;
-    const R bmax = (R)(n_bins - 1);
-    for (int k = 0; k < n; ++k) {
-        const double q = a.s_q[base + k];
-        const R av = (R)a.s_a[base + k];
-        const double fl = floor(shift + sign * q);
-        bool ok = (fl >= 0.0) && (fl <= 2.0 * kTofHalf);
-        const int t = (ok ? (int)fl : kTofHalf) - kTofHalf;
-        ok = ok && (t >= a.t_lo) && (t <= a.t_hi);
-        const R tf = (R)t;
-        R jf = rint(tf * inv_w + mid);
-        jf = jf < (R)0 ? (R)0 : (jf > bmax ? bmax : jf);
-        const int j0 = (int)jf;
-        const R ku = kappa2 * (tf - (jf - mid) * bw);
-        R e[2 * HM + 1];
+    if constexpr (sizeof(R) == 4) return __builtin_amdgcn_fmed3f(x, lo, hi);
+    else return x < lo ? lo : (x > hi ? hi : x);
+}
+
+// Geometry and coefficients of one (scatter point, cell) pair.
+template <typename R>
+struct Pair {
+    R coef1, coef2;          // G eff KN A1 A2^r, G eff KN A2 A1^r
+    double shift;            // (d1 - d2)/c + 4000.5 ps
+    int n1, n2;             // samples folded on each path (0 when its activity is 0)
+    int64_t b1, b2;          // first sample of each path
+    R act1, act2;            // activity integrals (one-bin path)
+    float counts;
+};
+
+// The reference's per-pair filters and scalar terms (:451-540).  Returns
+// false for pairs it skips.
+template <typename R>
+__device__ __forceinline__ bool pair_eval(const ScatterArgs& a, int sp, int d1, int d2, double lor2,
+                                          double eff_ref, Pair<R>& p)
+{
+    const int64_t per_sp = (int64_t)a.nru * a.ndu;
+    const int64_t h1 = sp * per_sp + d1;
+    const int64_t h2 = sp * per_sp + d2;
+    const double att1 = a.hp_att[h1], att2 = a.hp_att[h2];
+    const double act1 = a.hp_act[h1], act2 = a.hp_act[h2];
+    if (att1 == 0.0 || att2 == 0.0 || (act1 == 0.0 && act2 == 0.0)) return false;
+    const double dist1 = a.hp_dist[h1], dist2 = a.hp_dist[h2];
+    // law of cosines at the scatter point, energy of the scattered photon
+    const double s1q = dist1 * dist1, s2q = dist2 * dist2;
+    double carg = (s1q + s2q - lor2) / (2.0 * sqrt(s1q * s2q));
+    carg = carg < -1.0 ? -1.0 : (carg > 1.0 ? 1.0 : carg);
+    const double e_kev = floor(511.0 / (2.0 + carg) + 0.5);
+    int erow = (int)(e_kev * 2.0) - 1;
+    erow = erow < 0 ? 0 : (erow >= a.n_energy ? a.n_energy - 1 : erow);
+    const double eff = a.eff[erow] * eff_ref;
+    if (eff == 0.0) return false;
+    const R ratio = (R)a.ratio[erow];
+    const R log2e = (R)1.4426950408889634;
+    const R a1s = fast_exp2(-ratio * (R)a.hp_lnatt[h1] * log2e);
+    const R a2s = fast_exp2(-ratio * (R)a.hp_lnatt[h2] * log2e);
+    const R c = (R)carg;   // cos(angle at S) = -cos(scattering angle)
+    const R rho = (R)1 / ((R)1 + (R)a.kn_gamma * ((R)1 + c));
+    const R kn = rho * rho * (rho + (R)1 / rho - ((R)1 - c * c));
+    const R geo = (R)a.g_const * (R)a.hp_cos[h1] * (R)a.hp_cos[h2] / ((R)s1q * (R)s2q);
+    const R pre = geo * (R)eff * kn;
+    p.coef1 = pre * (R)att1 * a2s;
+    p.coef2 = pre * (R)att2 * a1s;
+    p.shift = (dist1 - dist2) * (1.0 / kLightCmPerPs) + (kTofHalf + 0.5);
+    p.n1 = act1 != 0.0 ? a.hp_n[h1] : 0;
+    p.n2 = act2 != 0.0 ? a.hp_n[h2] : 0;
+    p.b1 = a.hp_off[h1];
+    p.b2 = a.hp_off[h2];
+    p.act1 = (R)act1;
+    p.act2 = (R)act2;
+    p.counts = (act1 != 0.0 && act2 != 0.0) ? 2.0f : 1.0f;
+    return true;
+}
+
+// TOF weights of one voxel sample at integer offset off (t = off - 4000 ps):
+// w[H + d] = wsample * T[j0 + d, t] for |d| <= h, zero outside the bins or
+// the window; returns the nearest bin j0.
+template <typename R, int H, bool SINGLE_EXP>
+__device__ __forceinline__ int tof_weights(const ScatterArgs& a, uint32_t off, R wsample, int n_bins,
+                                           R (&w)[2 * H + 1])
+{
+    const R tf = (R)((int)off - kTofHalf);
+    const R jf = clampr(rint(tf * (R)a.inv_w_bins + (R)a.mid), (R)0, (R)(n_bins - 1));
+    const int j0 = (int)jf;
+    const R u = tf - (jf - (R)a.mid) * (R)a.bin_w;
+    R ku = u * (R)a.kappa2;
+    if constexpr (SINGLE_EXP) {
+        // h = 1: e+ = c E, e- = c / E with E = 2^ku, c = 2^-beta2; with
+        // D = E (1 + e+ + e-) the weights are E/D, c E^2/D and c/D.
+        ku = clampr(ku, (R)-100, (R)100);
+        const R E = fast_exp2(ku);
+        const R c = (R)a.c_beta;
+        const R cp = (j0 < n_bins - 1) ? c * E : (R)0;
+        const R cm = (j0 > 0) ? c : (R)0;
+        const R D = cp * E + (E + cm);
+        const R g = wsample * fast_rcp(D);
+        w[1] = g * E;
+        w[2] = w[1] * cp;
+        w[0] = g * cm;
+    } else {
+        const int h = a.half_window;
         R z = (R)1;
-        e[HM] = (R)1;
+        w[H] = (R)1;
 #pragma unroll
-        for (int d = 1; d <= HM; ++d) {
+        for (int d = 1; d <= H; ++d) {
             R ep = (R)0, em = (R)0;
             if (d <= h) {
-                const R dd = (R)(d * d) * beta2;
+                const R dd = (R)(d * d) * (R)a.beta2;
                 ep = (j0 + d < n_bins) ? fast_exp2((R)d * ku - dd) : (R)0;
                 em = (j0 - d >= 0) ? fast_exp2(-(R)d * ku - dd) : (R)0;
             }
-            e[HM + d] = ep;
-            e[HM - d] = em;
+            w[H + d] = ep;
+            w[H - d] = em;
             z += ep + em;
         }
-        const R wgt = ok ? coef * av * fast_rcp(z) : (R)0;
-        R* row = col + (j0 + h) * kWave;
-        atomicAdd(row, wgt);
+        const R g = wsample * fast_rcp(z);
 #pragma unroll
-        for (int d = 1; d <= HM; ++d) {
-            if (d <= h) {
-                atomicAdd(row + d * kWave, wgt * e[HM + d]);
-                atomicAdd(row - d * kWave, wgt * e[HM - d]);
-            }
-        }
+        for (int i = 0; i < 2 * H + 1; ++i) w[i] *= g;
     }
+    return j0;
 }
 
-template <typename R, int BMAX, int HM, bool TRIVIAL>
-__global__ __launch_bounds__(256) void k_scatter(ScatterArgs a)
+// Register window over the 2H+1 TOF bins around the current nearest bin.
+// Along one half path the nearest bin changes only every few tens of
+// samples (a bin is hundreds of ps, a voxel sample tens), so consecutive
+// samples add into the same registers; when the nearest bin changes the
+// window is added once into the lane's LDS column (rows b + H, padded so no
+// bounds test is needed; a plain read-add-write, the lane owns its column).
+template <typename R, int H>
+struct BinWindow {
+    R v[2 * H + 1];
+    int cur;        // nearest bin of the samples in the window
+
+    __device__ __forceinline__ void reset(int j)
+    {
+#pragma unroll
+        for (int i = 0; i < 2 * H + 1; ++i) v[i] = (R)0;
+        cur = j;
+    }
+    __device__ __forceinline__ void spill(R* col) const
+    {
+        // window bins cur-H .. cur+H live in rows cur .. cur+2H
+        R* base = col + cur * kWave;
+#pragma unroll
+        for (int i = 0; i < 2 * H + 1; ++i) base[i * kWave] += v[i];
+    }
+    __device__ __forceinline__ void add(R* col, int j0, const R (&w)[2 * H + 1])
+    {
+        if (j0 != cur) {
+            spill(col);
+            reset(j0);
+        }
+#pragma unroll
+        for (int i = 0; i < 2 * H + 1; ++i) v[i] += w[i];
+    }
+};
+
+// Fold the voxel samples of one half path into the lane's column:
+// offset floor(shift + sign * q) (SIGN = -1: path to D1, P - q; +1: path to
+// D2, P + q), weight a_k * coef * T[b, t].  Offsets outside the non-zero TOF
+// columns weigh 0.  The samples two ahead are in flight.
+template <typename R, int H, bool SINGLE_EXP, int SIGN>
+__device__ __forceinline__ void fold_path(const ScatterArgs& a, R* __restrict__ col, int64_t base, int n,
+                                          double shift, R coef, int n_bins)
+{
+    const uint32_t lo = (uint32_t)(a.t_lo + kTofHalf);
+    const uint32_t span = (uint32_t)(a.t_hi - a.t_lo);
+    const double* __restrict__ qp = a.s_q + base;
+    BinWindow<R, H> win;
+    win.reset(0);
+    double q0 = 0.0, q1 = 0.0;
+    R a0 = (R)0, a1 = (R)0;
+    auto load = [&](int k, double& q, R& av) {
+        q = qp[k];
+        if constexpr (sizeof(R) == 4) av = a.s_a[base + k];
+        else av = a.s_a64[base + k];
+    };
+    if (n > 0) load(0, q0, a0);
+    if (n > 1) load(1, q1, a1);
+    for (int k = 0; k < n; ++k) {
+        const double q = q0;
+        const R av = a0;
+        q0 = q1;
+        a0 = a1;
+        if (k + 2 < n) load(k + 2, q1, a1);
+        // integer picosecond offset, float64 as the reference (:520,526)
+        const double fl = floor(SIGN < 0 ? shift - q : shift + q);
+        const uint32_t off = (uint32_t)fl;   // saturates below 0
+        const bool ok = (fl >= 0.0) && (off - lo) <= span;
+        R w[2 * H + 1];
+        const int j0 = tof_weights<R, H, SINGLE_EXP>(a, ok ? off : lo, ok ? av * coef : (R)0, n_bins, w);
+        win.add(col, j0, w);
+    }
+    win.spill(col);
+}
+
+// Fold both half paths of a pair (path 1: offsets P - q; path 2: P + q).
+template <typename R, int H, bool SINGLE_EXP>
+__device__ __forceinline__ void fold_pair(const ScatterArgs& a, R* __restrict__ col, const Pair<R>& p,
+                                          int n_bins)
+{
+    if (a.t_hi < a.t_lo) return;   // no non-zero TOF column at all: every weight is zero
+    fold_path<R, H, SINGLE_EXP, -1>(a, col, p.b1, p.n1, p.shift, p.coef1, n_bins);
+    fold_path<R, H, SINGLE_EXP, +1>(a, col, p.b2, p.n2, p.shift, p.coef2, n_bins);
+}
+
+template <typename R, int BMAX, int HM, bool SINGLE_EXP, bool TRIVIAL>
+__global__ __launch_bounds__(256, 3) void k_scatter(ScatterArgs a)
 {
     extern __shared__ unsigned char smem_raw[];
     const int wave = threadIdx.x / kWave;
@@ -112,7 +260,7 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a)
     if (tile >= a.n_tiles) return;
     const int chunk = blockIdx.y;
     const int B = a.n_bins;
-    const int rows = B + 2 * a.half_window;
+    const int rows = B + 2 * HM;   // bins -HM .. B-1+HM (the window's reach)
     R* col = reinterpret_cast<R*>(smem_raw) + (size_t)wave * rows * kWave + lane;
     if constexpr (!TRIVIAL) {
         for (int r = 0; r < rows; ++r) col[r * kWave] = (R)0;
@@ -124,10 +272,8 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a)
     const int i1 = tl.z + lane / kTileCols;
     const int i2 = tl.w + lane % kTileCols;
     const bool valid = (i1 < ndu) && (i2 < ndu) && (i1 < i2 || (i1 == i2 && r1 != r2));
-    const int i1c = i1 < ndu ? i1 : ndu - 1;
-    const int i2c = i2 < ndu ? i2 : ndu - 1;
-    const int d1 = r1 * ndu + i1c;
-    const int d2 = r2 * ndu + i2c;
+    const int d1 = r1 * ndu + (i1 < ndu ? i1 : ndu - 1);
+    const int d2 = r2 * ndu + (i2 < ndu ? i2 : ndu - 1);
     double lor2;
     {
         const double dx = a.detpos[3 * d1 + 0] - a.detpos[3 * d2 + 0];
@@ -136,8 +282,6 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a)
         lor2 = ((dx * dx) + (dy * dy)) + (dz * dz);
     }
     const double eff_ref = a.eff[a.eff_ref_row];
-    const int64_t per_sp = (int64_t)nru * ndu;
-    const double inv_c = 1.0 / kLightCmPerPs;
 
     R out[BMAX], outc[BMAX], num[BMAX], numc[BMAX];
 #pragma unroll
@@ -149,56 +293,35 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a)
 #pragma unroll
         for (int b = 0; b < BMAX; ++b) { num[b] = numc[b] = (R)0; }
         float cnt = 0.0f;
-        const int p_begin = a.slice_ptr[sl];
         const int p_end = a.slice_ptr[sl + 1];
-        for (int sp = p_begin; sp < p_end; ++sp) {
-            const int64_t h1 = sp * per_sp + d1;
-            const int64_t h2 = sp * per_sp + d2;
-            const double dist1 = a.hp_dist[h1], dist2 = a.hp_dist[h2];
-            const double att1 = a.hp_att[h1], att2 = a.hp_att[h2];
-            const double act1 = a.hp_act[h1], act2 = a.hp_act[h2];
-            bool ok = valid && att1 != 0.0 && att2 != 0.0 && !(act1 == 0.0 && act2 == 0.0);
-            // law of cosines at the scatter point, energy of the scattered photon
-            const double s1q = dist1 * dist1, s2q = dist2 * dist2;
-            double carg = (s1q + s2q - lor2) / (2.0 * sqrt(s1q * s2q));
-            carg = carg < -1.0 ? -1.0 : (carg > 1.0 ? 1.0 : carg);
-            const double e_kev = floor(511.0 / (2.0 + carg) + 0.5);
-            int erow = (int)(e_kev * 2.0) - 1;
-            erow = erow < 0 ? 0 : (erow >= a.n_energy ? a.n_energy - 1 : erow);
-            const double eff = a.eff[erow] * eff_ref;
-            ok = ok && (eff != 0.0);
-            if (!ok) continue;
-            const R ratio = (R)a.ratio[erow];
-            const R ln1 = (R)a.hp_lnatt[h1], ln2 = (R)a.hp_lnatt[h2];
-            const R log2e = (R)1.4426950408889634;
-            const R a1s = fast_exp2(-ratio * ln1 * log2e);
-            const R a2s = fast_exp2(-ratio * ln2 * log2e);
-            const R c = (R)carg;
-            const R rho = (R)1 / ((R)1 + (R)a.kn_gamma * ((R)1 + c));
-            const R kn = rho * rho * (rho + (R)1 / rho - ((R)1 - c * c));
-            const R geo = (R)a.g_const * (R)a.hp_cos[h1] * (R)a.hp_cos[h2] / ((R)s1q * (R)s2q);
-            const R pre = geo * (R)eff * kn;
-            const R coef1 = pre * (R)att1 * a2s;
-            const R coef2 = pre * (R)att2 * a1s;
-            if constexpr (TRIVIAL) {
-                const R v = coef1 * (R)act1 + coef2 * (R)act2;
-                kahan_add(num[0], numc[0], v);
-            } else {
-                const double shift = (dist1 - dist2) * inv_c + (kTofHalf + 0.5);
-                if (act1 != 0.0) tof_fold<R, HM>(a, col, a.hp_off[h1], a.hp_n[h1], shift, -1.0, coef1, B);
-                if (act2 != 0.0) tof_fold<R, HM>(a, col, a.hp_off[h2], a.hp_n[h2], shift, 1.0, coef2, B);
-                const int h = a.half_window;
+        int cursor = valid ? a.slice_ptr[sl] : p_end;
+        while (true) {
+            // next scatter point of this lane's cell that the reference keeps
+            Pair<R> p;
+            bool found = false;
+            while (cursor < p_end) {
+                found = pair_eval<R>(a, cursor, d1, d2, lor2, eff_ref, p);
+                ++cursor;
+                if (found) break;
+            }
+            if (!__any(found)) break;
+            if (found) {
+                cnt += p.counts;
+                if constexpr (TRIVIAL) {
+                    kahan_add(num[0], numc[0], p.coef1 * p.act1 + p.coef2 * p.act2);
+                } else {
+                    fold_pair<R, HM, SINGLE_EXP>(a, col, p, B);
 #pragma unroll
-                for (int b = 0; b < BMAX; ++b) {
-                    if (b < B) {
-                        R* p = col + (b + h) * kWave;
-                        const R v = *p;
-                        *p = (R)0;
-                        kahan_add(num[b], numc[b], v);
+                    for (int b = 0; b < BMAX; ++b) {
+                        if (b < B) {
+                            R* q = col + (b + HM) * kWave;
+                            const R v = *q;
+                            *q = (R)0;
+                            kahan_add(num[b], numc[b], v);
+                        }
                     }
                 }
             }
-            cnt += (act1 != 0.0 && act2 != 0.0) ? 2.0f : 1.0f;
         }
         if (cnt > 0.0f) {
 #pragma unroll
@@ -229,7 +352,7 @@ __global__ __launch_bounds__(256) void k_pair_stats(ScatterArgs a, unsigned long
     if (tile >= a.n_tiles) return;
     const int4 tl = a.tiles[tile];
     const int r1 = tl.x, r2 = tl.y;
-    const int ndu = a.ndu, nru = a.nru;
+    const int ndu = a.ndu;
     const int i1 = tl.z + lane / kTileCols;
     const int i2 = tl.w + lane % kTileCols;
     const bool valid = (i1 < ndu) && (i2 < ndu) && (i1 < i2 || (i1 == i2 && r1 != r2));
@@ -244,24 +367,13 @@ __global__ __launch_bounds__(256) void k_pair_stats(ScatterArgs a, unsigned long
         lor2 = ((dx * dx) + (dy * dy)) + (dz * dz);
     }
     const double eff_ref = a.eff[a.eff_ref_row];
-    const int64_t per_sp = (int64_t)nru * ndu;
     unsigned long long pairs = 0, samples = 0;
     const int p_end = a.slice_ptr[a.chunk_slices[a.n_chunks]];
     for (int sp = a.slice_ptr[a.chunk_slices[0]]; sp < p_end; ++sp) {
-        const int64_t h1 = sp * per_sp + d1;
-        const int64_t h2 = sp * per_sp + d2;
-        const double dist1 = a.hp_dist[h1], dist2 = a.hp_dist[h2];
-        const double act1 = a.hp_act[h1], act2 = a.hp_act[h2];
-        if (a.hp_att[h1] == 0.0 || a.hp_att[h2] == 0.0 || (act1 == 0.0 && act2 == 0.0)) continue;
-        const double s1q = dist1 * dist1, s2q = dist2 * dist2;
-        double carg = (s1q + s2q - lor2) / (2.0 * sqrt(s1q * s2q));
-        carg = carg < -1.0 ? -1.0 : (carg > 1.0 ? 1.0 : carg);
-        int erow = (int)(floor(511.0 / (2.0 + carg) + 0.5) * 2.0) - 1;
-        erow = erow < 0 ? 0 : (erow >= a.n_energy ? a.n_energy - 1 : erow);
-        if (a.eff[erow] * eff_ref == 0.0) continue;
+        Pair<float> p;
+        if (!pair_eval<float>(a, sp, d1, d2, lor2, eff_ref, p)) continue;
         ++pairs;
-        samples += (act1 != 0.0 ? (unsigned long long)a.hp_n[h1] : 0ull) +
-                   (act2 != 0.0 ? (unsigned long long)a.hp_n[h2] : 0ull);
+        samples += (unsigned long long)(p.n1 + p.n2);
     }
     atomicAdd(out + 0, pairs);
     atomicAdd(out + 1, samples);
@@ -303,13 +415,14 @@ __global__ void k_dense_scatter(const R* __restrict__ compact, int n_bins, int n
     dense[((b * (n_det + 1) + rad) * na + ang) * planes + plane] = (float)compact[i];
 }
 
-template <typename R, int BMAX, int HM, bool TRIVIAL>
+template <typename R, int BMAX, int HM, bool SINGLE_EXP, bool TRIVIAL>
 hipError_t launch_variant(const ScatterArgs& a, hipStream_t st)
 {
     const int waves_per_block = 4;
     dim3 grid((a.n_tiles + waves_per_block - 1) / waves_per_block, a.n_chunks);
-    size_t lds = TRIVIAL ? 0 : (size_t)waves_per_block * (a.n_bins + 2 * a.half_window) * kWave * sizeof(R);
-    hipLaunchKernelGGL((k_scatter<R, BMAX, HM, TRIVIAL>), grid, dim3(waves_per_block * kWave), lds, st, a);
+    size_t lds = TRIVIAL ? 0 : (size_t)waves_per_block * (a.n_bins + 2 * HM) * kWave * sizeof(R);
+    hipLaunchKernelGGL((k_scatter<R, BMAX, HM, SINGLE_EXP, TRIVIAL>), grid, dim3(waves_per_block * kWave),
+                       lds, st, a);
     return hipGetLastError();
 }
 
@@ -317,16 +430,20 @@ template <typename R, int BMAX>
 hipError_t launch_hm(const ScatterArgs& a, hipStream_t st)
 {
     const int h = a.half_window;
-    if (h <= 1) return launch_variant<R, BMAX, 1, false>(a, st);
-    if (h <= 2) return launch_variant<R, BMAX, 2, false>(a, st);
-    if (h <= 4) return launch_variant<R, BMAX, 4, false>(a, st);
-    return launch_variant<R, BMAX, 8, false>(a, st);
+    if (h == 0) return launch_variant<R, BMAX, 0, false, false>(a, st);
+    if (h == 1) {
+        if (a.single_exp) return launch_variant<R, BMAX, 1, true, false>(a, st);
+        return launch_variant<R, BMAX, 1, false, false>(a, st);
+    }
+    if (h <= 2) return launch_variant<R, BMAX, 2, false, false>(a, st);
+    if (h <= 4) return launch_variant<R, BMAX, 4, false, false>(a, st);
+    return launch_variant<R, BMAX, 8, false, false>(a, st);
 }
 
 template <typename R>
 hipError_t launch_typed(const ScatterArgs& a, bool trivial, hipStream_t st)
 {
-    if (trivial) return launch_variant<R, 1, 1, true>(a, st);
+    if (trivial) return launch_variant<R, 1, 1, false, true>(a, st);
     if (a.n_bins <= 4) return launch_hm<R, 4>(a, st);
     if (a.n_bins <= 8) return launch_hm<R, 8>(a, st);
     if (a.n_bins <= 16) return launch_hm<R, 16>(a, st);

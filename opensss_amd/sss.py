@@ -56,7 +56,9 @@ class ScatterSubject:
         self._lib = _native.load_library()
 
         ratio, eff, _ = energy_tables(AttenuationTable, EnergyResolution, EnergyThreshold)
-        self.tof = tof_kernel(self.n_bins, float(TOFRange), float(TOFResolution))
+        # the FP64 validation path keeps every bin above 1e-16 of the nearest one
+        self.tof = tof_kernel(self.n_bins, float(TOFRange), float(TOFResolution),
+                              window_tol=1e-16 if precision == 'fp64' else 1e-9)
         self.rings = _i32(rings_used(self.n_rings, self.nru))
         self.dets = _i32(detectors_used(self.n_det, self.ndu))
         if scatter_points is None:

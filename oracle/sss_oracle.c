@@ -112,13 +112,14 @@ int ora_trace(const int32_t* gs, const double* gb, const double* lc,
     double prev = 0;
     while ((X != xEnd || Y != yEnd || Z != zEnd) && n < n_max) {
         double len;
-        if (xTM < yTM) {
-            if (xTM < zTM) { X += xStep; len = xTM; xTM += xTD; }
-            else           { Z += zStep; len = zTM; zTM += zTD; }
-        } else {
-            if (yTM < zTM) { Y += yStep; len = yTM; yTM += yTD; }
-            else           { Z += zStep; len = zTM; zTM += zTD; }
-        }
+        /* same axis choice as the reference; a step that would leave the grid
+         * (only possible in the non-terminating case above) ends the walk */
+        int axis = (xTM < yTM) ? (xTM < zTM ? 0 : 2) : (yTM < zTM ? 1 : 2);
+        double nX = axis == 0 ? X + xStep : X, nY = axis == 1 ? Y + yStep : Y, nZ = axis == 2 ? Z + zStep : Z;
+        if (nX < 1 || nX > Nx || nY < 1 || nY > Ny || nZ < 1 || nZ > Nz) break;
+        if (axis == 0) { X = nX; len = xTM; xTM += xTD; }
+        else if (axis == 1) { Y = nY; len = yTM; yTM += yTD; }
+        else { Z = nZ; len = zTM; zTM += zTD; }
         double dx = px - xVec * len, dy = py - yVec * len, dz = pz - zVec * len;
         L[n - 1] = sqrt(dx * dx + dy * dy + dz * dz);
         s[n - 1] = prev + L[n - 1] / 2;

@@ -106,6 +106,22 @@ def test_small_tof_workload_vs_oracle(oracle, precision, tol):
     assert rel <= tol and absn <= 1e-6
 
 
+@pytest.mark.parametrize('n_bins', [15, 21, 1, 3])
+def test_tof_kernel_variants_vs_oracle(oracle, n_bins):
+    """15 bins: one-exp window (h = 1); 21 bins: h = 2; one bin of a 214 ps
+    kernel: zero TOF columns beyond +-3508 ps; 3 bins: narrow kernel (two-exp)."""
+    w = phantoms.small_tof(seed=12, n_bins=n_bins, n_sp=30)
+    got = gpu_compact(w, 'fp32')
+    want = oracle_compact(oracle, w)
+    rel, absn = rel_err_above(got, want)
+    print(f'{w.name}: max rel err {rel:.3e}, max abs/peak {absn:.3e}')
+    assert rel <= FP32_TOL and absn <= 1e-6
+    got64 = gpu_compact(w, 'fp64')
+    rel, absn = rel_err_above(got64, want)
+    # the oracle stores its per-slice sums in float32 as the reference does
+    assert rel <= FP64_TOL and absn <= 1e-6
+
+
 def test_config1_nontof_full_vs_oracle(oracle):
     w = phantoms.config1()
     got = gpu_compact(w, 'fp32')
