@@ -4,7 +4,7 @@
 One step = one full scatter estimate of the workload with the mu-map, scatter
 points and activity image resident in HBM: trace every scatter-point ->
 detector half path (table build), refresh the emission terms, fused scatter
-sum over (scatter point x LOR x TOF bin), compensated FP32.  Prints one JSON
+sum over (scatter point x LOR x TOF bin) in FP32.  Prints one JSON
 line with throughput in scatter-point x LOR x TOF-bin evaluations per second,
 p50/p99 step latency, the roofline of the dominant kernel (the fused scatter
 sum, FP32 VALU), the error against an FP64 run of the same kernels, and a CPU
@@ -34,7 +34,22 @@ def parse():
     p.add_argument('--no-cpu-baseline', action='store_true')
     p.add_argument('--no-fp64-check', action='store_true')
     p.add_argument('--cpu-threads', type=int, default=0)
+    p.add_argument('--calibrate-copy', action='store_true',
+                   help='run only a 1 GiB device copy (3 launches) to scale FETCH_SIZE/WRITE_SIZE')
     return p.parse_args()
+
+
+def calibrate_copy():
+    """Reference kernel for the HBM counters: 1 GiB read + 1 GiB written per launch."""
+    import torch
+    n = (1 << 30) // 4
+    src = torch.ones(n, dtype=torch.float32, device='cuda')
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({'calibration': 'torch.mul copy', 'bytes_read': n * 4, 'bytes_written': n * 4,
+                      'launches': 3}))
 
 
 def make_workload(name):
@@ -96,6 +111,9 @@ def main():
               'on one MI355X (multi-GPU sharding over z slices is described in DESIGN.md, not run)',
               file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     w = make_workload(args.workload)
     print(f'[bench] workload {w.name}: {len(w.scatter_points)} sp, grid {w.grid}, '
           f'{w.n_rings_used}x{w.n_det_used} sampled, {w.n_bins} bins', file=sys.stderr, flush=True)
@@ -160,7 +178,8 @@ def main():
         'config': {'workload': w.name, 'n_scatter_points': int(len(w.scatter_points)),
                    'lors_per_scatter_point': int(info['n_pairs_enumerated'] // max(1, len(w.scatter_points))),
                    'tof_bins': w.n_bins, 'evals_per_step': int(evals),
-                   'precision': 'fp32 + compensated sums, fp64 crossings/energy index/TOF offsets',
+                   'precision': 'fp32 fold and sums; fp64 crossings, distances, energy index; '
+                                '32.32 fixed-point TOF offsets',
                    'parallelism': 'single instance, 1 GPU'},
         'latency_ms': {'p50': float(np.percentile(lat_ms, 50)), 'p99': float(np.percentile(lat_ms, 99))},
         'error_vs_fp64': err,

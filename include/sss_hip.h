@@ -37,7 +37,7 @@ extern "C" {
 #define SSS_ERR_UNSUPPORTED -4  /* valid input outside what the kernels implement */
 #define SSS_ERR_TRACE -5        /* a voxel walk did not end where its end voxel says */
 
-#define SSS_PRECISION_FP32 0    /* FP32 sums with compensated accumulation */
+#define SSS_PRECISION_FP32 0    /* FP32 fold and sums; FP64 geometry, energy index, TOF offsets */
 #define SSS_PRECISION_FP64 1    /* all-FP64 validation path */
 
 typedef struct sss_ctx sss_ctx;

@@ -46,12 +46,23 @@ struct HalfPathTable {
     int32_t* status;   // 0 ok, 1 walk overflow
 };
 
-// Per-sample arrays (compacted, offsets in HalfPathTable::off).
+// TOF time offsets are carried in 32.32 fixed point: one integer add and the
+// high word give floor(X - q) exactly for X, q quantised to 2^-32 ps.
+constexpr double kFixOne = 4294967296.0;   // 2^32
+constexpr int kSamplePad = 4;              // records after the last one (fold prefetch reads 3 ahead)
+
+// One voxel sample of a half path, read by the fold with a single 16-byte load.
+struct alignas(16) SampleRec {
+    int64_t nq;        // -round(2^32 q), q = 2 s_k / c in ps
+    float a;           // lam * len (FP32 fold input, rewritten per activity image)
+    int32_t vox;       // 0-based voxel (reference linear index - 1)
+};
+
+// Per-sample arrays (compacted, offsets in HalfPathTable::off; kSamplePad
+// spare records at the end).
 struct SampleTable {
-    int32_t* vox;      // 0-based voxel (reference linear index - 1)
+    SampleRec* rec;
     double* len;       // intersection length, cm
-    double* q;         // 2 s_k / c, ps
-    float* a;          // lam * len (FP32 fold input)
     double* a64;       // lam * len in float64 (FP64 validation path only, else null)
 };
 
@@ -68,9 +79,8 @@ struct ScatterArgs {
     const double* hp_lnatt;
     const double* hp_act;
     const float* hp_cos;
-    const double* s_q;           // TOF sample times 2 s_k / c (float64, both paths)
-    const float* s_a;            // FP32 path
-    const double* s_a64;         // FP64 path
+    const SampleRec* s_rec;      // fixed-point TOF offsets, FP32 emission terms
+    const double* s_a64;         // FP64 emission terms (FP64 path)
     const double* detpos;        // (NRU, NDU, 3)
     const double* eff;           // n_energy
     const float* ratio;          // n_energy
@@ -83,11 +93,18 @@ struct ScatterArgs {
     double c_beta;                                 // 2^-beta2
     int half_window, t_lo, t_hi;
     int single_exp;                                // h = 1 and beta2 small: one exp2 per sample
+    int range_check;                               // offsets may leave [t_lo, t_hi]: test each sample
     double g_const;              // (Dx Dy 1e-2)^2 / (4 pi)
     double kn_gamma;
     // output
     void* out;                   // [chunk][B][NRU^2][cells], float or double
 };
+
+// Largest |TOF offset| in ps the fold can produce for a subject: the
+// detector-distance difference of any pair plus the largest sample time
+// (s_k <= half the clipped path, so q <= box diagonal / c), plus rounding.
+double tof_offset_bound(const Grid& g, const double* sp_xyz, int n_sp, const double* detpos,
+                        int n_det_pos);
 
 // Launchers (defined in the .hip files).
 hipError_t launch_halfpath_count(const Grid& g, const double* sp_xyz, int n_sp, const double* detpos,

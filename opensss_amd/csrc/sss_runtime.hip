@@ -69,6 +69,28 @@ void dev_free(T*& p)
 }
 
 }  // namespace
+
+double tof_offset_bound(const Grid& g, const double* sp_xyz, int n_sp, const double* detpos,
+                        int n_det_pos)
+{
+    const double dx = g.b[3] - g.b[0], dy = g.b[4] - g.b[1], dz = g.b[5] - g.b[2];
+    const double qmax = std::sqrt(dx * dx + dy * dy + dz * dz) / kLightCmPerPs;
+    double dd = 0;
+    for (int p = 0; p < n_sp; ++p) {
+        double lo = HUGE_VAL, hi = 0;
+        for (int i = 0; i < n_det_pos; ++i) {
+            const double ex = detpos[3 * i + 0] - sp_xyz[3 * p + 0];
+            const double ey = detpos[3 * i + 1] - sp_xyz[3 * p + 1];
+            const double ez = detpos[3 * i + 2] - sp_xyz[3 * p + 2];
+            const double r = std::sqrt(ex * ex + ey * ey + ez * ez);
+            lo = std::min(lo, r);
+            hi = std::max(hi, r);
+        }
+        dd = std::max(dd, hi - lo);
+    }
+    return dd / kLightCmPerPs + qmax + 1.0;
+}
+
 }  // namespace sss
 
 using namespace sss;
@@ -85,6 +107,7 @@ struct sss_subject {
     int n_bins = 0, n_energy = 0, eff_ref_row = 0, precision = 0;
     bool trivial = false;
     int half_window = 0, t_lo = 0, t_hi = 0;
+    bool range_check = true;          // TOF offsets may leave the non-zero table columns
     double bin_w = 0, kappa = 0, beta = 0, g_const = 0;
     int64_t n_vox = 0, n_hp = 0, n_samples = 0, samples_cap = 0, n_cells_plane = 0, out_elems = 0;
     int n_tiles = 0, n_chunks = 1;
@@ -122,7 +145,7 @@ struct sss_subject {
         dev_free(d_chunk_slices); dev_free(d_dets); dev_free(d_flag); dev_free(d_scan_tmp);
         dev_free(hp.off); dev_free(hp.n); dev_free(hp.dist); dev_free(hp.att); dev_free(hp.lnatt);
         dev_free(hp.act); dev_free(hp.cosabs); dev_free(hp.status);
-        dev_free(smp.vox); dev_free(smp.len); dev_free(smp.q); dev_free(smp.a); dev_free(smp.a64);
+        dev_free(smp.rec); dev_free(smp.len); dev_free(smp.a64);
         dev_free(d_out); dev_free(d_partial);
     }
 };
@@ -335,6 +358,11 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
             detpos[o + 2] = d->geom[(ring * d->n_det + 0) * 3 + 2];
             for (int c = 0; c < 3; ++c) detnorm[o + c] = d->normals[(ring * d->n_det + det) * 3 + c];
         }
+    // the per-sample window test can go when every offset provably stays
+    // inside a table whose columns are all non-zero
+    s->range_check = !(s->t_lo == -kTofHalf && s->t_hi == kTofHalf &&
+                       tof_offset_bound(g, d->sp_xyz, d->n_sp, detpos.data(), d->nru * d->ndu) <
+                           kTofHalf - 2);
     // wave tiles of 8 x 8 cells over the upper triangle of every ring-pair plane
     std::vector<int4> tiles;
     const int nt = (d->ndu + kTileRows - 1) / kTileRows;
@@ -344,9 +372,16 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
                 for (int tj = ti; tj < nt; ++tj)
                     tiles.push_back(make_int4(r1, r2, ti * kTileRows, tj * kTileCols));
     s->n_tiles = (int)tiles.size();
-    // slice chunks of about equal scatter-point counts
+    // slice chunks of about equal scatter-point counts.  By default one chunk
+    // per slice: the launch walks chunk-major, so the waves resident at one
+    // time share one slice's half paths in cache; the per-chunk partial
+    // outputs are capped at 4 GiB.
     int chunks = d->n_chunks;
-    if (chunks <= 0) chunks = (int)std::max<int64_t>(1, (16384 + s->n_tiles - 1) / std::max(1, s->n_tiles));
+    if (chunks <= 0) {
+        const int64_t out_bytes = std::max<int64_t>(
+            1, s->out_elems * (d->precision == SSS_PRECISION_FP64 ? 8 : 4));
+        chunks = (int)std::max<int64_t>(1, std::min<int64_t>(d->n_slices, (int64_t(4) << 30) / out_bytes));
+    }
     chunks = std::max(1, std::min(chunks, std::max(1, d->n_slices)));
     std::vector<int32_t> chunk_slices(1, 0);
     for (int c = 1; c < chunks; ++c) {
@@ -450,17 +485,15 @@ int sss_subject_trace(sss_subject* s)
     int64_t total = 0;
     SSS_HIP(hipMemcpyAsync(&total, s->hp.off + s->n_hp, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     SSS_HIP(hipStreamSynchronize(st));
-    if (total > s->samples_cap) {
-        dev_free(s->smp.vox); dev_free(s->smp.len); dev_free(s->smp.q); dev_free(s->smp.a);
-        dev_free(s->smp.a64);
+    if (total > s->samples_cap || !s->smp.rec) {
+        dev_free(s->smp.rec); dev_free(s->smp.len); dev_free(s->smp.a64);
         s->samples_cap = 0;
         int rc;
-        if ((rc = dev_alloc(&s->smp.vox, (size_t)total, &s->device_bytes)) ||
+        // spare records after the last sample: the fold prefetches two ahead
+        if ((rc = dev_alloc(&s->smp.rec, (size_t)total + kSamplePad, &s->device_bytes)) ||
             (rc = dev_alloc(&s->smp.len, (size_t)total, &s->device_bytes)) ||
-            (rc = dev_alloc(&s->smp.q, (size_t)total, &s->device_bytes)) ||
-            (rc = dev_alloc(&s->smp.a, (size_t)total, &s->device_bytes)) ||
             (s->precision == SSS_PRECISION_FP64 &&
-             (rc = dev_alloc(&s->smp.a64, (size_t)total, &s->device_bytes))))
+             (rc = dev_alloc(&s->smp.a64, (size_t)total + kSamplePad, &s->device_bytes))))
             return rc;
         s->samples_cap = total;
     }
@@ -509,7 +542,7 @@ static ScatterArgs scatter_args(const sss_subject* s)
     a.n_tiles = s->n_tiles; a.n_chunks = s->n_chunks; a.n_cells_plane = s->n_cells_plane;
     a.hp_off = s->hp.off; a.hp_n = s->hp.n; a.hp_dist = s->hp.dist; a.hp_att = s->hp.att;
     a.hp_lnatt = s->hp.lnatt; a.hp_act = s->hp.act; a.hp_cos = s->hp.cosabs;
-    a.s_q = s->smp.q; a.s_a = s->smp.a; a.s_a64 = s->smp.a64;
+    a.s_rec = s->smp.rec; a.s_a64 = s->smp.a64;
     a.detpos = s->d_detpos; a.eff = s->d_eff; a.ratio = s->d_ratio; a.tiles = s->d_tiles;
     a.slice_ptr = s->d_slice_ptr; a.chunk_slices = s->d_chunk_slices;
     const double log2e = 1.4426950408889634;
@@ -522,6 +555,7 @@ static ScatterArgs scatter_args(const sss_subject* s)
     a.half_window = s->half_window; a.t_lo = s->t_lo; a.t_hi = s->t_hi;
     // one exp2 per sample needs E^2 = 2^(2 ku) finite inside the bins: ku <= beta2
     a.single_exp = (s->half_window == 1 && a.beta2 <= 60.0) ? 1 : 0;
+    a.range_check = s->range_check ? 1 : 0;
     a.g_const = s->g_const;
     a.kn_gamma = (511.0e3 * 1.6e-19) / (9.10938356e-31 * (299792458.0 * 299792458.0));
     a.out = s->n_chunks > 1 ? s->d_partial : s->d_out;

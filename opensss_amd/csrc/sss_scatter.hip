@@ -4,7 +4,7 @@
 // one ring-pair plane (r1, r2) for a contiguous range of normalisation
 // slices (a chunk).  Each lane accumulates its own cell over the scatter
 // points of a slice, then adds sum/count of the slice (the per-slice
-// normalisation of :555-557) into its running output.  No global atomics and
+// normalisation of :555-557) into the chunk's output.  No global atomics and
 // a fixed reduction order: results do not depend on what else runs.
 //
 // Every lane walks the slice's scatter points with its own cursor and skips
@@ -16,31 +16,23 @@
 // the Compton energy index floor(511/(2 + cos_arg) + 0.5) (the value of the
 // reference's acos/degrees/cos round trip at :495-501) and the pair's TOF
 // time shift; in R (float or double): Klein-Nishina, efficiency, solid angle,
-// attenuation scaling and the TOF fold.  The fold visits every voxel sample
-// of both half paths in one loop, rounds the time offset to the reference's
-// integer picosecond in float64 (:520,526) and adds the sample into the nearest TOF bin and
-// its +-h neighbours with the normalised Gaussian weights (the columns of the
-// reference's TOF table, evaluated analytically).  Fold accumulators live in
-// LDS, one column per lane, so per-lane bin indices need no register
-// indexing.  Sums over scatter points and slices are compensated (Kahan) in
-// the FP32 path.
+// attenuation scaling and the TOF fold.
+//
+// The fold (:516-530) visits the voxel samples of both half paths.  The
+// reference's integer picosecond offset floor((d1 - d2)/c - +2 s_k/c + 0.5)
+// is formed in 32.32 fixed point (one 64-bit add, the high word is the floor)
+// from the pair's shift and the sample's time, both rounded to 2^-32 ps.
+// Each sample adds into the nearest TOF bin and its +-h neighbours with the
+// normalised Gaussian weights (the columns of the reference's TOF table,
+// evaluated analytically).  Along a half path the nearest bin changes only
+// every few tens of samples, so the weights collect in a register window
+// that is added into the lane's LDS column (one row per bin, padded by h on
+// both sides) when the nearest bin changes.  The column holds the slice's
+// sums; at the slice end it is divided by the slice's pair count.
 #include "sss_internal.h"
 
 namespace sss {
 namespace {
-
-template <typename R>
-__device__ __forceinline__ void kahan_add(R& s, R& c, R v)
-{
-    if constexpr (sizeof(R) == 4) {
-        const R y = v - c;
-        const R t = s + y;
-        c = (t - s) - y;
-        s = t;
-    } else {
-        s += v;
-    }
-}
 
 template <typename R>
 __device__ __forceinline__ R fast_exp2(R x)
@@ -63,12 +55,19 @@ __device__ __forceinline__ R clampr(R x, R lo, R hi)
     else return x < lo ? lo : (x > hi ? hi : x);
 }
 
+// LDS add without a returned value (the lane owns its column: no conflicts).
+template <typename R>
+__device__ __forceinline__ void lds_add(R* p, R v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // Geometry and coefficients of one (scatter point, cell) pair.
 template <typename R>
 struct Pair {
     R coef1, coef2;          // G eff KN A1 A2^r, G eff KN A2 A1^r
-    double shift;            // (d1 - d2)/c + 4000.5 ps
-    int n1, n2;             // samples folded on each path (0 when its activity is 0)
+    int64_t shift;           // ((d1 - d2)/c + 1/2) ps in 32.32 fixed point
+    int n1, n2;              // samples folded on each path (0 when its activity is 0)
     int64_t b1, b2;          // first sample of each path
     R act1, act2;            // activity integrals (one-bin path)
     float counts;
@@ -107,7 +106,7 @@ __device__ __forceinline__ bool pair_eval(const ScatterArgs& a, int sp, int d1, 
     const R pre = geo * (R)eff * kn;
     p.coef1 = pre * (R)att1 * a2s;
     p.coef2 = pre * (R)att2 * a1s;
-    p.shift = (dist1 - dist2) * (1.0 / kLightCmPerPs) + (kTofHalf + 0.5);
+    p.shift = (int64_t)rint(((dist1 - dist2) * (1.0 / kLightCmPerPs) + 0.5) * kFixOne);
     p.n1 = act1 != 0.0 ? a.hp_n[h1] : 0;
     p.n2 = act2 != 0.0 ? a.hp_n[h2] : 0;
     p.b1 = a.hp_off[h1];
@@ -118,26 +117,34 @@ __device__ __forceinline__ bool pair_eval(const ScatterArgs& a, int sp, int d1, 
     return true;
 }
 
-// TOF weights of one voxel sample at integer offset off (t = off - 4000 ps):
-// w[H + d] = wsample * T[j0 + d, t] for |d| <= h, zero outside the bins or
-// the window; returns the nearest bin j0.
+// TOF constants of one fold direction, in R.
+template <typename R>
+struct TofDir {
+    R inv_w;      // +-1/W: nearest bin j0 = rint(t / W + mid), sign folded in
+    R kap;        // +-kappa2
+    R mid, jmax;  // (B-1)/2, B-1
+    R wk, mid_wk; // W kappa2, mid W kappa2
+    R c;          // 2^-beta2
+};
+
+// TOF weights of one voxel sample whose offset is +-hi ps (sign in d):
+// w[H + j] = wsample * T[j0 + j, t] for |j| <= h, zero outside the bins or
+// the window; returns the nearest bin j0 (as R).
 template <typename R, int H, bool SINGLE_EXP>
-__device__ __forceinline__ int tof_weights(const ScatterArgs& a, uint32_t off, R wsample, int n_bins,
-                                           R (&w)[2 * H + 1])
+__device__ __forceinline__ R tof_weights(const ScatterArgs& a, const TofDir<R>& d, int hi, R wsample,
+                                         R (&w)[2 * H + 1])
 {
-    const R tf = (R)((int)off - kTofHalf);
-    const R jf = clampr(rint(tf * (R)a.inv_w_bins + (R)a.mid), (R)0, (R)(n_bins - 1));
-    const int j0 = (int)jf;
-    const R u = tf - (jf - (R)a.mid) * (R)a.bin_w;
-    R ku = u * (R)a.kappa2;
+    const R hf = (R)hi;
+    const R jf = clampr(rint(hf * d.inv_w + d.mid), (R)0, d.jmax);
+    // ku = kappa2 (t - (j0 - mid) W)
+    R ku = hf * d.kap + (d.mid_wk - jf * d.wk);
     if constexpr (SINGLE_EXP) {
         // h = 1: e+ = c E, e- = c / E with E = 2^ku, c = 2^-beta2; with
         // D = E (1 + e+ + e-) the weights are E/D, c E^2/D and c/D.
         ku = clampr(ku, (R)-100, (R)100);
         const R E = fast_exp2(ku);
-        const R c = (R)a.c_beta;
-        const R cp = (j0 < n_bins - 1) ? c * E : (R)0;
-        const R cm = (j0 > 0) ? c : (R)0;
+        const R cp = (jf < d.jmax) ? d.c * E : (R)0;
+        const R cm = (jf > (R)0) ? d.c : (R)0;
         const R D = cp * E + (E + cm);
         const R g = wsample * fast_rcp(D);
         w[1] = g * E;
@@ -148,36 +155,33 @@ __device__ __forceinline__ int tof_weights(const ScatterArgs& a, uint32_t off, R
         R z = (R)1;
         w[H] = (R)1;
 #pragma unroll
-        for (int d = 1; d <= H; ++d) {
+        for (int j = 1; j <= H; ++j) {
             R ep = (R)0, em = (R)0;
-            if (d <= h) {
-                const R dd = (R)(d * d) * (R)a.beta2;
-                ep = (j0 + d < n_bins) ? fast_exp2((R)d * ku - dd) : (R)0;
-                em = (j0 - d >= 0) ? fast_exp2(-(R)d * ku - dd) : (R)0;
+            if (j <= h) {
+                const R dd = (R)(j * j) * (R)a.beta2;
+                ep = (jf + (R)j <= d.jmax) ? fast_exp2((R)j * ku - dd) : (R)0;
+                em = (jf - (R)j >= (R)0) ? fast_exp2(-(R)j * ku - dd) : (R)0;
             }
-            w[H + d] = ep;
-            w[H - d] = em;
+            w[H + j] = ep;
+            w[H - j] = em;
             z += ep + em;
         }
         const R g = wsample * fast_rcp(z);
 #pragma unroll
         for (int i = 0; i < 2 * H + 1; ++i) w[i] *= g;
     }
-    return j0;
+    return jf;
 }
 
-// Register window over the 2H+1 TOF bins around the current nearest bin.
-// Along one half path the nearest bin changes only every few tens of
-// samples (a bin is hundreds of ps, a voxel sample tens), so consecutive
-// samples add into the same registers; when the nearest bin changes the
-// window is added once into the lane's LDS column (rows b + H, padded so no
-// bounds test is needed; a plain read-add-write, the lane owns its column).
+// Register window over the 2H+1 TOF bins around the current nearest bin; it
+// is added into the lane's LDS column (rows b + H) when the nearest bin
+// changes and at the end of a slice.
 template <typename R, int H>
 struct BinWindow {
     R v[2 * H + 1];
-    int cur;        // nearest bin of the samples in the window
+    R cur;          // nearest bin of the samples in the window
 
-    __device__ __forceinline__ void reset(int j)
+    __device__ __forceinline__ void reset(R j)
     {
 #pragma unroll
         for (int i = 0; i < 2 * H + 1; ++i) v[i] = (R)0;
@@ -186,11 +190,11 @@ struct BinWindow {
     __device__ __forceinline__ void spill(R* col) const
     {
         // window bins cur-H .. cur+H live in rows cur .. cur+2H
-        R* base = col + cur * kWave;
+        R* base = col + (int)cur * kWave;
 #pragma unroll
-        for (int i = 0; i < 2 * H + 1; ++i) base[i * kWave] += v[i];
+        for (int i = 0; i < 2 * H + 1; ++i) lds_add(base + i * kWave, v[i]);
     }
-    __device__ __forceinline__ void add(R* col, int j0, const R (&w)[2 * H + 1])
+    __device__ __forceinline__ void add(R* col, R j0, const R (&w)[2 * H + 1])
     {
         if (j0 != cur) {
             spill(col);
@@ -201,57 +205,81 @@ struct BinWindow {
     }
 };
 
-// Fold the voxel samples of one half path into the lane's column:
-// offset floor(shift + sign * q) (SIGN = -1: path to D1, P - q; +1: path to
-// D2, P + q), weight a_k * coef * T[b, t].  Offsets outside the non-zero TOF
-// columns weigh 0.  The samples two ahead are in flight.
-template <typename R, int H, bool SINGLE_EXP, int SIGN>
-__device__ __forceinline__ void fold_path(const ScatterArgs& a, R* __restrict__ col, int64_t base, int n,
-                                          double shift, R coef, int n_bins)
+// One voxel sample of a half path (see fold_path).
+template <typename R, int H, bool SINGLE_EXP, bool RANGE, int SIGN>
+__device__ __forceinline__ void fold_sample(const ScatterArgs& a, const TofDir<R>& d, R* __restrict__ col,
+                                            BinWindow<R, H>& win, int64_t x, int64_t nq, R av, R coef)
 {
-    const uint32_t lo = (uint32_t)(a.t_lo + kTofHalf);
-    const uint32_t span = (uint32_t)(a.t_hi - a.t_lo);
-    const double* __restrict__ qp = a.s_q + base;
-    BinWindow<R, H> win;
-    win.reset(0);
-    double q0 = 0.0, q1 = 0.0;
-    R a0 = (R)0, a1 = (R)0;
-    auto load = [&](int k, double& q, R& av) {
-        q = qp[k];
-        if constexpr (sizeof(R) == 4) av = a.s_a[base + k];
-        else av = a.s_a64[base + k];
-    };
-    if (n > 0) load(0, q0, a0);
-    if (n > 1) load(1, q1, a1);
-    for (int k = 0; k < n; ++k) {
-        const double q = q0;
-        const R av = a0;
-        q0 = q1;
-        a0 = a1;
-        if (k + 2 < n) load(k + 2, q1, a1);
-        // integer picosecond offset, float64 as the reference (:520,526)
-        const double fl = floor(SIGN < 0 ? shift - q : shift + q);
-        const uint32_t off = (uint32_t)fl;   // saturates below 0
-        const bool ok = (fl >= 0.0) && (off - lo) <= span;
-        R w[2 * H + 1];
-        const int j0 = tof_weights<R, H, SINGLE_EXP>(a, ok ? off : lo, ok ? av * coef : (R)0, n_bins, w);
-        win.add(col, j0, w);
+    int hi = (int)((x + nq) >> 32);
+    R ws = av * coef;
+    if constexpr (RANGE) {
+        // a zero-weight sample is moved to t_lo so its exponentials stay finite
+        const uint32_t span = (uint32_t)(a.t_hi - a.t_lo);
+        const uint32_t rel = SIGN < 0 ? (uint32_t)(hi - a.t_lo) : (uint32_t)(-a.t_lo - hi);
+        const bool ok = rel <= span;
+        ws = ok ? ws : (R)0;
+        hi = ok ? hi : (SIGN < 0 ? a.t_lo : -a.t_lo);
     }
-    win.spill(col);
+    R w[2 * H + 1];
+    const R j0 = tof_weights<R, H, SINGLE_EXP>(a, d, hi, ws, w);
+    win.add(col, j0, w);
 }
 
-// Fold both half paths of a pair (path 1: offsets P - q; path 2: P + q).
-template <typename R, int H, bool SINGLE_EXP>
-__device__ __forceinline__ void fold_pair(const ScatterArgs& a, R* __restrict__ col, const Pair<R>& p,
-                                          int n_bins)
+// Fold the voxel samples of one half path.  SIGN = -1 (path to D1): offset
+// t = floor(X - q) = hi32(shift + nq); SIGN = +1 (path to D2): t = floor(X + q)
+// = -ceil(-X - q) = -hi32((2^32 - 1 - shift) + nq).  Two samples per trip,
+// each record loaded right after its predecessor in the same slot is used;
+// the loads run up to three records past the path (the table has spare
+// records after its end).  With RANGE, offsets outside
+// the non-zero TOF columns [t_lo, t_hi] weigh 0.
+template <typename R, int H, bool SINGLE_EXP, bool RANGE, int SIGN>
+__device__ __forceinline__ void fold_path(const ScatterArgs& a, const TofDir<R>& d, R* __restrict__ col,
+                                          BinWindow<R, H>& win, int64_t base, int n, int64_t shift,
+                                          R coef)
 {
-    if (a.t_hi < a.t_lo) return;   // no non-zero TOF column at all: every weight is zero
-    fold_path<R, H, SINGLE_EXP, -1>(a, col, p.b1, p.n1, p.shift, p.coef1, n_bins);
-    fold_path<R, H, SINGLE_EXP, +1>(a, col, p.b2, p.n2, p.shift, p.coef2, n_bins);
+    const SampleRec* __restrict__ p = a.s_rec + base;
+    const SampleRec* const end = p + n;
+    int64_t x = SIGN < 0 ? shift : ((int64_t)0xffffffffLL - shift);
+    asm volatile("" : "+v"(x));   // keep x + nq as one 64-bit add per sample
+    SampleRec ra = p[0], rb = p[1];
+    if constexpr (sizeof(R) == 8) {
+        const double* __restrict__ ap = a.s_a64 + base;
+        double aa = ap[0], ab = ap[1];
+        for (; p < end; p += 2, ap += 2) {
+            fold_sample<R, H, SINGLE_EXP, RANGE, SIGN>(a, d, col, win, x, ra.nq, aa, coef);
+            ra = p[2];
+            aa = ap[2];
+            if (p + 1 < end) fold_sample<R, H, SINGLE_EXP, RANGE, SIGN>(a, d, col, win, x, rb.nq, ab, coef);
+            rb = p[3];
+            ab = ap[3];
+        }
+    } else {
+        // both loads unconditional, so the wait before a sample covers only its own record
+        for (; p < end; p += 2) {
+            fold_sample<R, H, SINGLE_EXP, RANGE, SIGN>(a, d, col, win, x, ra.nq, ra.a, coef);
+            ra = p[2];
+            if (p + 1 < end) fold_sample<R, H, SINGLE_EXP, RANGE, SIGN>(a, d, col, win, x, rb.nq, rb.a, coef);
+            rb = p[3];
+        }
+    }
 }
 
-template <typename R, int BMAX, int HM, bool SINGLE_EXP, bool TRIVIAL>
-__global__ __launch_bounds__(256, 3) void k_scatter(ScatterArgs a)
+template <typename R>
+__device__ __forceinline__ TofDir<R> tof_dir(const ScatterArgs& a, int sign)
+{
+    TofDir<R> d;
+    d.inv_w = (R)(sign * a.inv_w_bins);
+    d.kap = (R)(sign * a.kappa2);
+    d.mid = (R)a.mid;
+    d.jmax = (R)(a.n_bins - 1);
+    d.wk = (R)(a.bin_w * a.kappa2);
+    d.mid_wk = (R)(a.mid * a.bin_w * a.kappa2);
+    d.c = (R)a.c_beta;
+    return d;
+}
+
+template <typename R, int BMAX, int HM, bool SINGLE_EXP, bool TRIVIAL, bool RANGE>
+__global__ __launch_bounds__(256) void k_scatter(ScatterArgs a)
 {
     extern __shared__ unsigned char smem_raw[];
     const int wave = threadIdx.x / kWave;
@@ -282,17 +310,22 @@ __global__ __launch_bounds__(256, 3) void k_scatter(ScatterArgs a)
         lor2 = ((dx * dx) + (dy * dy)) + (dz * dz);
     }
     const double eff_ref = a.eff[a.eff_ref_row];
+    // path to D1: t = +hi; path to D2: t = -hi
+    const TofDir<R> dt1 = tof_dir<R>(a, +1), dt2 = tof_dir<R>(a, -1);
 
-    R out[BMAX], outc[BMAX], num[BMAX], numc[BMAX];
-#pragma unroll
-    for (int b = 0; b < BMAX; ++b) { out[b] = outc[b] = (R)0; }
+    const int plane = r2 + r1 * nru;
+    const int64_t cell = (int64_t)i1 * ndu - (int64_t)i1 * (i1 - 1) / 2 + (i2 - i1);
+    const int64_t planes = (int64_t)nru * nru;
+    R* dst = reinterpret_cast<R*>(a.out) + ((int64_t)chunk * B * planes + plane) * a.n_cells_plane + cell;
+    const int64_t bin_stride = planes * a.n_cells_plane;
 
     const int s_begin = a.chunk_slices[chunk];
     const int s_end = a.chunk_slices[chunk + 1];
     for (int sl = s_begin; sl < s_end; ++sl) {
-#pragma unroll
-        for (int b = 0; b < BMAX; ++b) { num[b] = numc[b] = (R)0; }
+        R num = (R)0;   // one-bin path
         float cnt = 0.0f;
+        BinWindow<R, HM> win;
+        win.reset((R)0);
         const int p_end = a.slice_ptr[sl + 1];
         int cursor = valid ? a.slice_ptr[sl] : p_end;
         while (true) {
@@ -308,36 +341,37 @@ __global__ __launch_bounds__(256, 3) void k_scatter(ScatterArgs a)
             if (found) {
                 cnt += p.counts;
                 if constexpr (TRIVIAL) {
-                    kahan_add(num[0], numc[0], p.coef1 * p.act1 + p.coef2 * p.act2);
-                } else {
-                    fold_pair<R, HM, SINGLE_EXP>(a, col, p, B);
-#pragma unroll
-                    for (int b = 0; b < BMAX; ++b) {
-                        if (b < B) {
-                            R* q = col + (b + HM) * kWave;
-                            const R v = *q;
-                            *q = (R)0;
-                            kahan_add(num[b], numc[b], v);
-                        }
-                    }
+                    num += p.coef1 * p.act1 + p.coef2 * p.act2;
+                } else if (a.t_hi >= a.t_lo) {   // else every TOF weight is zero
+                    fold_path<R, HM, SINGLE_EXP, RANGE, -1>(a, dt1, col, win, p.b1, p.n1, p.shift, p.coef1);
+                    fold_path<R, HM, SINGLE_EXP, RANGE, +1>(a, dt2, col, win, p.b2, p.n2, p.shift, p.coef2);
                 }
             }
         }
-        if (cnt > 0.0f) {
-#pragma unroll
-            for (int b = 0; b < BMAX; ++b)
-                if (b < B) kahan_add(out[b], outc[b], num[b] / (R)cnt);
+        // slice sum / slice count into the chunk's output (0/0 -> 0, :201)
+        if constexpr (TRIVIAL) {
+            if (valid) {
+                const R v = cnt > 0.0f ? num / (R)cnt : (R)0;
+                dst[0] = sl == s_begin ? v : dst[0] + v;
+            }
+        } else {
+            win.spill(col);
+            const R inv = cnt > 0.0f ? (R)1 / (R)cnt : (R)0;
+            for (int b = 0; b < B; ++b) {
+                R* q = col + (b + HM) * kWave;
+                const R v = *q * inv;
+                *q = (R)0;
+                if (valid) {
+                    R* o = dst + b * bin_stride;
+                    *o = sl == s_begin ? v : *o + v;
+                }
+            }
+            // window reach rows (bins beyond the ends) are dropped
+            for (int r = 0; r < HM; ++r) {
+                col[r * kWave] = (R)0;
+                col[(B + HM + r) * kWave] = (R)0;
+            }
         }
-    }
-    if (!valid) return;
-    const int plane = r2 + r1 * nru;
-    const int64_t cell = (int64_t)i1 * ndu - (int64_t)i1 * (i1 - 1) / 2 + (i2 - i1);
-    const int64_t planes = (int64_t)nru * nru;
-    R* dst = reinterpret_cast<R*>(a.out);
-#pragma unroll
-    for (int b = 0; b < BMAX; ++b) {
-        if (b < B)
-            dst[(((int64_t)chunk * B + b) * planes + plane) * a.n_cells_plane + cell] = out[b];
     }
 }
 
@@ -415,39 +449,45 @@ __global__ void k_dense_scatter(const R* __restrict__ compact, int n_bins, int n
     dense[((b * (n_det + 1) + rad) * na + ang) * planes + plane] = (float)compact[i];
 }
 
-template <typename R, int BMAX, int HM, bool SINGLE_EXP, bool TRIVIAL>
+template <typename R, int BMAX, int HM, bool SINGLE_EXP, bool TRIVIAL, bool RANGE>
 hipError_t launch_variant(const ScatterArgs& a, hipStream_t st)
 {
     const int waves_per_block = 4;
     dim3 grid((a.n_tiles + waves_per_block - 1) / waves_per_block, a.n_chunks);
     size_t lds = TRIVIAL ? 0 : (size_t)waves_per_block * (a.n_bins + 2 * HM) * kWave * sizeof(R);
-    hipLaunchKernelGGL((k_scatter<R, BMAX, HM, SINGLE_EXP, TRIVIAL>), grid, dim3(waves_per_block * kWave),
-                       lds, st, a);
+    hipLaunchKernelGGL((k_scatter<R, BMAX, HM, SINGLE_EXP, TRIVIAL, RANGE>), grid,
+                       dim3(waves_per_block * kWave), lds, st, a);
     return hipGetLastError();
 }
 
-template <typename R, int BMAX>
+template <typename R, int BMAX, bool RANGE>
 hipError_t launch_hm(const ScatterArgs& a, hipStream_t st)
 {
     const int h = a.half_window;
-    if (h == 0) return launch_variant<R, BMAX, 0, false, false>(a, st);
+    if (h == 0) return launch_variant<R, BMAX, 0, false, false, RANGE>(a, st);
     if (h == 1) {
-        if (a.single_exp) return launch_variant<R, BMAX, 1, true, false>(a, st);
-        return launch_variant<R, BMAX, 1, false, false>(a, st);
+        if (a.single_exp) return launch_variant<R, BMAX, 1, true, false, RANGE>(a, st);
+        return launch_variant<R, BMAX, 1, false, false, RANGE>(a, st);
     }
-    if (h <= 2) return launch_variant<R, BMAX, 2, false, false>(a, st);
-    if (h <= 4) return launch_variant<R, BMAX, 4, false, false>(a, st);
-    return launch_variant<R, BMAX, 8, false, false>(a, st);
+    if (h <= 2) return launch_variant<R, BMAX, 2, false, false, RANGE>(a, st);
+    if (h <= 4) return launch_variant<R, BMAX, 4, false, false, RANGE>(a, st);
+    return launch_variant<R, BMAX, 8, false, false, RANGE>(a, st);
+}
+
+template <typename R, bool RANGE>
+hipError_t launch_range(const ScatterArgs& a, hipStream_t st)
+{
+    if (a.n_bins <= 4) return launch_hm<R, 4, RANGE>(a, st);
+    if (a.n_bins <= 8) return launch_hm<R, 8, RANGE>(a, st);
+    if (a.n_bins <= 16) return launch_hm<R, 16, RANGE>(a, st);
+    return launch_hm<R, 32, RANGE>(a, st);
 }
 
 template <typename R>
 hipError_t launch_typed(const ScatterArgs& a, bool trivial, hipStream_t st)
 {
-    if (trivial) return launch_variant<R, 1, 1, false, true>(a, st);
-    if (a.n_bins <= 4) return launch_hm<R, 4>(a, st);
-    if (a.n_bins <= 8) return launch_hm<R, 8>(a, st);
-    if (a.n_bins <= 16) return launch_hm<R, 16>(a, st);
-    return launch_hm<R, 32>(a, st);
+    if (trivial) return launch_variant<R, 1, 1, false, true, false>(a, st);
+    return a.range_check ? launch_range<R, true>(a, st) : launch_range<R, false>(a, st);
 }
 
 }  // namespace

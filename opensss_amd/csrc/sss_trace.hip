@@ -72,9 +72,13 @@ __global__ void k_halfpath_fill(Grid g, const double* __restrict__ sp_xyz, int n
         const double two_over_c = 2.0 / kLightCmPerPs;
         bool truncated = false;
         const int got = walk_run(g, w, [&](int k, int vox, double L, double smp) {
-            s.vox[base + k] = vox;
+            SampleRec r;
+            // q * 2^32 is exact in float64; rint drops bits below 2^-32 ps
+            r.nq = -(int64_t)rint(smp * two_over_c * kFixOne);
+            r.a = 0.0f;
+            r.vox = vox;
+            s.rec[base + k] = r;
             s.len[base + k] = L;
-            s.q[base + k] = smp * two_over_c;
             mu_int += mu[vox] * L;
         }, &truncated);
         if (truncated) status = 1;
@@ -94,8 +98,8 @@ __global__ void k_activity_refresh(HalfPathTable t, SampleTable s, int64_t n_hp,
     const int n = t.n[h];
     double act = 0;
     for (int k = 0; k < n; ++k) {
-        const double ar = lam[s.vox[base + k]] * s.len[base + k];
-        s.a[base + k] = (float)ar;
+        const double ar = lam[s.rec[base + k].vox] * s.len[base + k];
+        s.rec[base + k].a = (float)ar;
         if (s.a64) s.a64[base + k] = ar;
         act += ar;   // sequential float64 sum, as the reference's np.sum under numba
     }
