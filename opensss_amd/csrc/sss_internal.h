@@ -15,6 +15,7 @@ constexpr double kLightCmPerPs = 299792458.0 * 1e-10;
 constexpr int kWave = 64;
 constexpr int kTileRows = 8;              // detector tile of one wave: 8 x 8 cells
 constexpr int kTileCols = 8;
+constexpr int kScatterWaves = 4;          // waves per workgroup of the scatter launch
 constexpr int kMaxBins = 32;
 constexpr int kMaxHalfWindow = 8;
 
@@ -46,8 +47,11 @@ struct HalfPathTable {
     int32_t* status;   // 0 ok, 1 walk overflow
 };
 
-// TOF time offsets are carried in 32.32 fixed point: one integer add and the
-// high word give floor(X - q) exactly for X, q quantised to 2^-32 ps.
+// TOF time offsets are carried in 32.32 fixed point: one 64-bit add and the
+// high word give floor(X - q) exactly for X, q quantised to 2^-32 ps.  The
+// reference's offsets fall within 1e-6 ps of an integer for about one
+// sample in a thousand (the fractions of (d1 - d2)/c and 2 s_k/c are often
+// nearly complementary), so the fraction needs far more than 20 bits.
 constexpr double kFixOne = 4294967296.0;   // 2^32
 constexpr int kSamplePad = 4;              // records after the last one (fold prefetch reads 3 ahead)
 
@@ -59,7 +63,7 @@ struct alignas(16) SampleRec {
 };
 
 // Per-sample arrays (compacted, offsets in HalfPathTable::off; kSamplePad
-// spare records at the end).
+// spare records at the end of rec and a64).
 struct SampleTable {
     SampleRec* rec;
     double* len;       // intersection length, cm
@@ -85,6 +89,8 @@ struct ScatterArgs {
     const double* eff;           // n_energy
     const float* ratio;          // n_energy
     const int4* tiles;           // (r1, r2, i1_0, i2_0)
+    const int2* items;           // per wave of the scatter launch: (tile or -1, chunk)
+    int n_items;
     const int32_t* slice_ptr;    // n_slices + 1
     const int32_t* chunk_slices; // n_chunks + 1 (slice ranges)
     // TOF kernel: bins b at s_b = (b - mid) W; weights of bin j0 + d relative to
@@ -94,6 +100,8 @@ struct ScatterArgs {
     int half_window, t_lo, t_hi;
     int single_exp;                                // h = 1 and beta2 small: one exp2 per sample
     int range_check;                               // offsets may leave [t_lo, t_hi]: test each sample
+    // the same constants rounded to float once on the host (uniform registers)
+    float f_inv_w, f_kap, f_mid, f_jmax, f_wk, f_mid_wk, f_c;
     double g_const;              // (Dx Dy 1e-2)^2 / (4 pi)
     double kn_gamma;
     // output

@@ -91,6 +91,43 @@ double tof_offset_bound(const Grid& g, const double* sp_xyz, int n_sp, const dou
     return dd / kLightCmPerPs + qmax + 1.0;
 }
 
+// Wave work items of the scatter launch.  Workgroups go to the 8 XCDs of an
+// MI355X round-robin by workgroup id, and each XCD has its own L2; a plane
+// (r1, r2) reads the half paths of rings r1 and r2, so each XCD is given the
+// planes of one pair of ring-group blocks {G_i x G_j, G_j x G_i} (rings in 4
+// groups; the diagonal blocks pair up as {G0 x G0, G1 x G1}, {G2 x G2, G3 x G3})
+// and reads half the rings.  Within an XCD the items run chunk-major, so the
+// waves resident at one time share one chunk's scatter points.
+std::vector<int2> xcd_work_items(const std::vector<int4>& tiles, int nru, int n_chunks,
+                                 int waves_per_block)
+{
+    constexpr int kXcd = 8;
+    auto group = [nru](int r) { return std::min(3, r * 4 / std::max(1, nru)); };
+    auto xcd_of = [](int g1, int g2) {
+        if (g1 == g2) return 6 + g1 / 2;
+        const int i = std::min(g1, g2), j = std::max(g1, g2);
+        static const int pair_id[4][4] = {{-1, 0, 1, 2}, {0, -1, 3, 4}, {1, 3, -1, 5}, {2, 4, 5, -1}};
+        return pair_id[i][j];
+    };
+    std::vector<std::vector<int>> lists(kXcd);
+    for (size_t t = 0; t < tiles.size(); ++t)
+        lists[xcd_of(group(tiles[t].x), group(tiles[t].y))].push_back((int)t);
+    size_t per_xcd = 0;
+    for (const auto& l : lists) per_xcd = std::max(per_xcd, l.size() * (size_t)n_chunks);
+    const size_t blocks_per_xcd = (per_xcd + waves_per_block - 1) / waves_per_block;
+    std::vector<int2> items(blocks_per_xcd * kXcd * waves_per_block, make_int2(-1, 0));
+    for (int x = 0; x < kXcd; ++x) {
+        size_t k = 0;
+        for (int c = 0; c < n_chunks; ++c)
+            for (int t : lists[x]) {
+                const size_t blk = (k / waves_per_block) * kXcd + x;   // workgroup id on XCD x
+                items[blk * waves_per_block + k % waves_per_block] = make_int2(t, c);
+                ++k;
+            }
+    }
+    return items;
+}
+
 }  // namespace sss
 
 using namespace sss;
@@ -122,6 +159,8 @@ struct sss_subject {
     double* d_eff = nullptr;
     float* d_ratio = nullptr;
     int4* d_tiles = nullptr;
+    int2* d_items = nullptr;          // scatter launch: (tile, chunk) per wave, XCD-aware order
+    int n_items = 0;
     int32_t* d_slice_ptr = nullptr;
     int32_t* d_chunk_slices = nullptr;
     int32_t* d_dets = nullptr;
@@ -146,7 +185,7 @@ struct sss_subject {
         dev_free(hp.off); dev_free(hp.n); dev_free(hp.dist); dev_free(hp.att); dev_free(hp.lnatt);
         dev_free(hp.act); dev_free(hp.cosabs); dev_free(hp.status);
         dev_free(smp.rec); dev_free(smp.len); dev_free(smp.a64);
-        dev_free(d_out); dev_free(d_partial);
+        dev_free(d_out); dev_free(d_partial); dev_free(d_items);
     }
 };
 
@@ -360,9 +399,8 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
         }
     // the per-sample window test can go when every offset provably stays
     // inside a table whose columns are all non-zero
-    s->range_check = !(s->t_lo == -kTofHalf && s->t_hi == kTofHalf &&
-                       tof_offset_bound(g, d->sp_xyz, d->n_sp, detpos.data(), d->nru * d->ndu) <
-                           kTofHalf - 2);
+    const double tof_bound = tof_offset_bound(g, d->sp_xyz, d->n_sp, detpos.data(), d->nru * d->ndu);
+    s->range_check = !(s->t_lo == -kTofHalf && s->t_hi == kTofHalf && tof_bound < kTofHalf - 2);
     // wave tiles of 8 x 8 cells over the upper triangle of every ring-pair plane
     std::vector<int4> tiles;
     const int nt = (d->ndu + kTileRows - 1) / kTileRows;
@@ -384,14 +422,20 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
     }
     chunks = std::max(1, std::min(chunks, std::max(1, d->n_slices)));
     std::vector<int32_t> chunk_slices(1, 0);
-    for (int c = 1; c < chunks; ++c) {
-        const int64_t target = (int64_t)d->n_sp * c / chunks;
-        int sl = chunk_slices.back();
-        while (sl < d->n_slices && d->slice_ptr[sl] < target) ++sl;
-        if (sl > chunk_slices.back() && sl < d->n_slices) chunk_slices.push_back(sl);
+    if (chunks >= d->n_slices) {
+        for (int sl = 1; sl < d->n_slices; ++sl) chunk_slices.push_back(sl);
+    } else {
+        for (int c = 1; c < chunks; ++c) {
+            const int64_t target = (int64_t)d->n_sp * c / chunks;
+            int sl = chunk_slices.back();
+            while (sl < d->n_slices && d->slice_ptr[sl] < target) ++sl;
+            if (sl > chunk_slices.back() && sl < d->n_slices) chunk_slices.push_back(sl);
+        }
     }
     chunk_slices.push_back(d->n_slices);
     s->n_chunks = (int)chunk_slices.size() - 1;
+    const std::vector<int2> items = xcd_work_items(tiles, d->nru, s->n_chunks, kScatterWaves);
+    s->n_items = (int)items.size();
     std::vector<float> ratio32(d->ratio, d->ratio + d->n_energy);
 
     int64_t& t = s->device_bytes;
@@ -401,6 +445,7 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
         (rc = dev_alloc(&s->d_detpos, detpos.size(), &t)) || (rc = dev_alloc(&s->d_detnorm, detnorm.size(), &t)) ||
         (rc = dev_alloc(&s->d_eff, (size_t)d->n_energy, &t)) || (rc = dev_alloc(&s->d_ratio, (size_t)d->n_energy, &t)) ||
         (rc = dev_alloc(&s->d_tiles, tiles.size() + 1, &t)) ||
+        (rc = dev_alloc(&s->d_items, items.size() + 1, &t)) ||
         (rc = dev_alloc(&s->d_slice_ptr, (size_t)d->n_slices + 1, &t)) ||
         (rc = dev_alloc(&s->d_chunk_slices, chunk_slices.size(), &t)) ||
         (rc = dev_alloc(&s->d_dets, (size_t)d->ndu, &t)) || (rc = dev_alloc(&s->d_flag, 1, &t)) ||
@@ -430,6 +475,7 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
     if (e == hipSuccess) e = hipMemcpyAsync(s->d_eff, d->eff, (size_t)d->n_energy * sizeof(double), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(s->d_ratio, ratio32.data(), ratio32.size() * sizeof(float), hipMemcpyHostToDevice, st);
     if (e == hipSuccess && !tiles.empty()) e = hipMemcpyAsync(s->d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && !items.empty()) e = hipMemcpyAsync(s->d_items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(s->d_slice_ptr, d->slice_ptr, ((size_t)d->n_slices + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(s->d_chunk_slices, chunk_slices.data(), chunk_slices.size() * sizeof(int32_t), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(s->d_dets, d->dets, (size_t)d->ndu * sizeof(int32_t), hipMemcpyHostToDevice, st);
@@ -489,7 +535,7 @@ int sss_subject_trace(sss_subject* s)
         dev_free(s->smp.rec); dev_free(s->smp.len); dev_free(s->smp.a64);
         s->samples_cap = 0;
         int rc;
-        // spare records after the last sample: the fold prefetches two ahead
+        // spare records after the last sample: the fold prefetches ahead
         if ((rc = dev_alloc(&s->smp.rec, (size_t)total + kSamplePad, &s->device_bytes)) ||
             (rc = dev_alloc(&s->smp.len, (size_t)total, &s->device_bytes)) ||
             (s->precision == SSS_PRECISION_FP64 &&
@@ -544,6 +590,7 @@ static ScatterArgs scatter_args(const sss_subject* s)
     a.hp_lnatt = s->hp.lnatt; a.hp_act = s->hp.act; a.hp_cos = s->hp.cosabs;
     a.s_rec = s->smp.rec; a.s_a64 = s->smp.a64;
     a.detpos = s->d_detpos; a.eff = s->d_eff; a.ratio = s->d_ratio; a.tiles = s->d_tiles;
+    a.items = s->d_items; a.n_items = s->n_items;
     a.slice_ptr = s->d_slice_ptr; a.chunk_slices = s->d_chunk_slices;
     const double log2e = 1.4426950408889634;
     a.inv_w_bins = 1.0 / s->bin_w;
@@ -556,6 +603,13 @@ static ScatterArgs scatter_args(const sss_subject* s)
     // one exp2 per sample needs E^2 = 2^(2 ku) finite inside the bins: ku <= beta2
     a.single_exp = (s->half_window == 1 && a.beta2 <= 60.0) ? 1 : 0;
     a.range_check = s->range_check ? 1 : 0;
+    a.f_inv_w = (float)a.inv_w_bins;
+    a.f_kap = (float)a.kappa2;
+    a.f_mid = (float)a.mid;
+    a.f_jmax = (float)(s->n_bins - 1);
+    a.f_wk = (float)(a.bin_w * a.kappa2);
+    a.f_mid_wk = (float)(a.mid * a.bin_w * a.kappa2);
+    a.f_c = (float)a.c_beta;
     a.g_const = s->g_const;
     a.kn_gamma = (511.0e3 * 1.6e-19) / (9.10938356e-31 * (299792458.0 * 299792458.0));
     a.out = s->n_chunks > 1 ? s->d_partial : s->d_out;

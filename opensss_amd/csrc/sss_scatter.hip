@@ -2,7 +2,8 @@
 //
 // Work decomposition: one wavefront owns an 8 x 8 tile of cells (i1, i2) of
 // one ring-pair plane (r1, r2) for a contiguous range of normalisation
-// slices (a chunk).  Each lane accumulates its own cell over the scatter
+// slices (a chunk); the (tile, chunk) items are listed per wave in an
+// XCD-aware order built on the host.  Each lane accumulates its own cell over the scatter
 // points of a slice, then adds sum/count of the slice (the per-slice
 // normalisation of :555-557) into the chunk's output.  No global atomics and
 // a fixed reduction order: results do not depend on what else runs.
@@ -117,24 +118,38 @@ __device__ __forceinline__ bool pair_eval(const ScatterArgs& a, int sp, int d1, 
     return true;
 }
 
-// TOF constants of one fold direction, in R.
+// TOF kernel constants in R (for float: rounded once on the host, so they
+// stay in scalar registers).
 template <typename R>
-struct TofDir {
-    R inv_w;      // +-1/W: nearest bin j0 = rint(t / W + mid), sign folded in
-    R kap;        // +-kappa2
+struct TofConst {
+    R inv_w;      // 1/W: nearest bin j0 = rint(t / W + mid)
+    R kap;        // kappa2
     R mid, jmax;  // (B-1)/2, B-1
     R wk, mid_wk; // W kappa2, mid W kappa2
     R c;          // 2^-beta2
 };
 
-// TOF weights of one voxel sample whose offset is +-hi ps (sign in d):
+template <typename R>
+__device__ __forceinline__ TofConst<R> tof_const(const ScatterArgs& a)
+{
+    TofConst<R> d;
+    if constexpr (sizeof(R) == 4) {
+        d.inv_w = a.f_inv_w; d.kap = a.f_kap; d.mid = a.f_mid; d.jmax = a.f_jmax;
+        d.wk = a.f_wk; d.mid_wk = a.f_mid_wk; d.c = a.f_c;
+    } else {
+        d.inv_w = a.inv_w_bins; d.kap = a.kappa2; d.mid = a.mid; d.jmax = (R)(a.n_bins - 1);
+        d.wk = a.bin_w * a.kappa2; d.mid_wk = a.mid * a.bin_w * a.kappa2; d.c = a.c_beta;
+    }
+    return d;
+}
+
+// TOF weights of one voxel sample at offset tf ps:
 // w[H + j] = wsample * T[j0 + j, t] for |j| <= h, zero outside the bins or
 // the window; returns the nearest bin j0 (as R).
 template <typename R, int H, bool SINGLE_EXP>
-__device__ __forceinline__ R tof_weights(const ScatterArgs& a, const TofDir<R>& d, int hi, R wsample,
+__device__ __forceinline__ R tof_weights(const ScatterArgs& a, const TofConst<R>& d, R hf, R wsample,
                                          R (&w)[2 * H + 1])
 {
-    const R hf = (R)hi;
     const R jf = clampr(rint(hf * d.inv_w + d.mid), (R)0, d.jmax);
     // ku = kappa2 (t - (j0 - mid) W)
     R ku = hf * d.kap + (d.mid_wk - jf * d.wk);
@@ -207,21 +222,24 @@ struct BinWindow {
 
 // One voxel sample of a half path (see fold_path).
 template <typename R, int H, bool SINGLE_EXP, bool RANGE, int SIGN>
-__device__ __forceinline__ void fold_sample(const ScatterArgs& a, const TofDir<R>& d, R* __restrict__ col,
+__device__ __forceinline__ void fold_sample(const ScatterArgs& a, const TofConst<R>& d, R* __restrict__ col,
                                             BinWindow<R, H>& win, int64_t x, int64_t nq, R av, R coef)
 {
-    int hi = (int)((x + nq) >> 32);
+    const int hi = (int)((x + nq) >> 32);   // high word: floor
     R ws = av * coef;
+    R tf;
     if constexpr (RANGE) {
         // a zero-weight sample is moved to t_lo so its exponentials stay finite
-        const uint32_t span = (uint32_t)(a.t_hi - a.t_lo);
-        const uint32_t rel = SIGN < 0 ? (uint32_t)(hi - a.t_lo) : (uint32_t)(-a.t_lo - hi);
-        const bool ok = rel <= span;
+        int t = SIGN < 0 ? hi : -hi;
+        const bool ok = (uint32_t)(t - a.t_lo) <= (uint32_t)(a.t_hi - a.t_lo);
         ws = ok ? ws : (R)0;
-        hi = ok ? hi : (SIGN < 0 ? a.t_lo : -a.t_lo);
+        t = ok ? t : a.t_lo;
+        tf = (R)t;
+    } else {
+        tf = SIGN < 0 ? (R)hi : -(R)hi;
     }
     R w[2 * H + 1];
-    const R j0 = tof_weights<R, H, SINGLE_EXP>(a, d, hi, ws, w);
+    const R j0 = tof_weights<R, H, SINGLE_EXP>(a, d, tf, ws, w);
     win.add(col, j0, w);
 }
 
@@ -230,10 +248,10 @@ __device__ __forceinline__ void fold_sample(const ScatterArgs& a, const TofDir<R
 // = -ceil(-X - q) = -hi32((2^32 - 1 - shift) + nq).  Two samples per trip,
 // each record loaded right after its predecessor in the same slot is used;
 // the loads run up to three records past the path (the table has spare
-// records after its end).  With RANGE, offsets outside
-// the non-zero TOF columns [t_lo, t_hi] weigh 0.
+// records after its end).  With RANGE, offsets outside the non-zero TOF
+// columns [t_lo, t_hi] weigh 0.
 template <typename R, int H, bool SINGLE_EXP, bool RANGE, int SIGN>
-__device__ __forceinline__ void fold_path(const ScatterArgs& a, const TofDir<R>& d, R* __restrict__ col,
+__device__ __forceinline__ void fold_path(const ScatterArgs& a, const TofConst<R>& d, R* __restrict__ col,
                                           BinWindow<R, H>& win, int64_t base, int n, int64_t shift,
                                           R coef)
 {
@@ -264,29 +282,20 @@ __device__ __forceinline__ void fold_path(const ScatterArgs& a, const TofDir<R>&
     }
 }
 
-template <typename R>
-__device__ __forceinline__ TofDir<R> tof_dir(const ScatterArgs& a, int sign)
-{
-    TofDir<R> d;
-    d.inv_w = (R)(sign * a.inv_w_bins);
-    d.kap = (R)(sign * a.kappa2);
-    d.mid = (R)a.mid;
-    d.jmax = (R)(a.n_bins - 1);
-    d.wk = (R)(a.bin_w * a.kappa2);
-    d.mid_wk = (R)(a.mid * a.bin_w * a.kappa2);
-    d.c = (R)a.c_beta;
-    return d;
-}
-
+// At least 6 waves per SIMD (<= 80 VGPRs): the values that do not fit are
+// spilled outside the sample loops (per slice and per pair only).  Measured
+// on config 2: 4 waves 240 ms, 5 waves 203 ms, 6 waves 189 ms.
 template <typename R, int BMAX, int HM, bool SINGLE_EXP, bool TRIVIAL, bool RANGE>
-__global__ __launch_bounds__(256) void k_scatter(ScatterArgs a)
+__global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArgs a)
 {
     extern __shared__ unsigned char smem_raw[];
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
-    const int tile = blockIdx.x * (blockDim.x / kWave) + wave;
-    if (tile >= a.n_tiles) return;
-    const int chunk = blockIdx.y;
+    // (tile, chunk) of this wave; padding items (tile -1) exit at once
+    const int2 item = a.items[blockIdx.x * kScatterWaves + wave];
+    if (item.x < 0) return;
+    const int tile = item.x;
+    const int chunk = item.y;
     const int B = a.n_bins;
     const int rows = B + 2 * HM;   // bins -HM .. B-1+HM (the window's reach)
     R* col = reinterpret_cast<R*>(smem_raw) + (size_t)wave * rows * kWave + lane;
@@ -310,8 +319,7 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a)
         lor2 = ((dx * dx) + (dy * dy)) + (dz * dz);
     }
     const double eff_ref = a.eff[a.eff_ref_row];
-    // path to D1: t = +hi; path to D2: t = -hi
-    const TofDir<R> dt1 = tof_dir<R>(a, +1), dt2 = tof_dir<R>(a, -1);
+    const TofConst<R> tc = tof_const<R>(a);
 
     const int plane = r2 + r1 * nru;
     const int64_t cell = (int64_t)i1 * ndu - (int64_t)i1 * (i1 - 1) / 2 + (i2 - i1);
@@ -343,8 +351,8 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a)
                 if constexpr (TRIVIAL) {
                     num += p.coef1 * p.act1 + p.coef2 * p.act2;
                 } else if (a.t_hi >= a.t_lo) {   // else every TOF weight is zero
-                    fold_path<R, HM, SINGLE_EXP, RANGE, -1>(a, dt1, col, win, p.b1, p.n1, p.shift, p.coef1);
-                    fold_path<R, HM, SINGLE_EXP, RANGE, +1>(a, dt2, col, win, p.b2, p.n2, p.shift, p.coef2);
+                    fold_path<R, HM, SINGLE_EXP, RANGE, -1>(a, tc, col, win, p.b1, p.n1, p.shift, p.coef1);
+                    fold_path<R, HM, SINGLE_EXP, RANGE, +1>(a, tc, col, win, p.b2, p.n2, p.shift, p.coef2);
                 }
             }
         }
@@ -452,11 +460,11 @@ __global__ void k_dense_scatter(const R* __restrict__ compact, int n_bins, int n
 template <typename R, int BMAX, int HM, bool SINGLE_EXP, bool TRIVIAL, bool RANGE>
 hipError_t launch_variant(const ScatterArgs& a, hipStream_t st)
 {
-    const int waves_per_block = 4;
-    dim3 grid((a.n_tiles + waves_per_block - 1) / waves_per_block, a.n_chunks);
-    size_t lds = TRIVIAL ? 0 : (size_t)waves_per_block * (a.n_bins + 2 * HM) * kWave * sizeof(R);
+    // one workgroup per kScatterWaves work items (the item list is padded to that)
+    dim3 grid((unsigned)(a.n_items / kScatterWaves));
+    size_t lds = TRIVIAL ? 0 : (size_t)kScatterWaves * (a.n_bins + 2 * HM) * kWave * sizeof(R);
     hipLaunchKernelGGL((k_scatter<R, BMAX, HM, SINGLE_EXP, TRIVIAL, RANGE>), grid,
-                       dim3(waves_per_block * kWave), lds, st, a);
+                       dim3(kScatterWaves * kWave), lds, st, a);
     return hipGetLastError();
 }
 
@@ -494,7 +502,7 @@ hipError_t launch_typed(const ScatterArgs& a, bool trivial, hipStream_t st)
 
 hipError_t launch_scatter(const ScatterArgs& a, bool fp64, bool trivial, hipStream_t st)
 {
-    if (a.n_tiles == 0) return hipSuccess;
+    if (a.n_tiles == 0 || a.n_items < kScatterWaves) return hipSuccess;
     return fp64 ? launch_typed<double>(a, trivial, st) : launch_typed<float>(a, trivial, st);
 }
 
