@@ -75,7 +75,8 @@ typedef struct {
     int32_t tof_t_lo, tof_t_hi;  /* non-zero TOF columns, ps */
     int32_t tof_trivial;         /* one bin, all weights 1 */
     int32_t precision;           /* SSS_PRECISION_* */
-    int32_t n_chunks;            /* slice chunks for the scatter launch; 0 = auto */
+    int32_t n_chunks;            /* target number of scatter-point chunks (each inside one
+                                    slice) for the scatter launch; 0 = auto */
 } sss_subject_desc;
 
 typedef struct {

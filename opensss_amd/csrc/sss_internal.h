@@ -16,6 +16,7 @@ constexpr int kWave = 64;
 constexpr int kTileRows = 8;              // detector tile of one wave: 8 x 8 cells
 constexpr int kTileCols = 8;
 constexpr int kScatterWaves = 4;          // waves per workgroup of the scatter launch
+constexpr int kAutoChunkPoints = 8;       // scatter points per chunk by default
 constexpr int kMaxBins = 32;
 constexpr int kMaxHalfWindow = 8;
 
@@ -92,7 +93,7 @@ struct ScatterArgs {
     const int2* items;           // per wave of the scatter launch: (tile or -1, chunk)
     int n_items;
     const int32_t* slice_ptr;    // n_slices + 1
-    const int32_t* chunk_slices; // n_chunks + 1 (slice ranges)
+    const int32_t* chunk_sp;     // n_chunks + 1: scatter-point range of each chunk (inside one slice)
     // TOF kernel: bins b at s_b = (b - mid) W; weights of bin j0 + d relative to
     // the nearest bin j0 are 2^(d ku - d^2 beta2), ku = kappa2 (t - s_j0)
     double inv_w_bins, mid, bin_w, kappa2, beta2;  // log2 scaled kappa, beta
@@ -104,8 +105,10 @@ struct ScatterArgs {
     float f_inv_w, f_kap, f_mid, f_jmax, f_wk, f_mid_wk, f_c;
     double g_const;              // (Dx Dy 1e-2)^2 / (4 pi)
     double kn_gamma;
-    // output
-    void* out;                   // [chunk][B][NRU^2][cells], float or double
+    // output: per chunk, the un-normalised sums and the pair counts; the
+    // chunk reduce divides per slice and sums the slices
+    void* part_sum;              // [chunk][B][NRU^2][cells], float or double
+    float* part_cnt;             // [chunk][NRU^2][cells]
 };
 
 // Largest |TOF offset| in ps the fold can produce for a subject: the
@@ -131,8 +134,9 @@ hipError_t launch_line_integrals(const Grid& g, const double* segs, int64_t n, c
                                  double* lam_int, hipStream_t st);
 hipError_t launch_scatter(const ScatterArgs& a, bool fp64, bool trivial, hipStream_t st);
 hipError_t launch_pair_stats(const ScatterArgs& a, unsigned long long* out, hipStream_t st);
-hipError_t launch_chunk_reduce_typed(const void* partial, int n_chunks, int64_t n, void* out,
-                                     bool fp64, hipStream_t st);
+hipError_t launch_chunk_reduce_typed(const void* part_sum, const float* part_cnt,
+                                     const int32_t* chunk_slice, int n_chunks, int64_t n,
+                                     int64_t n_cells, void* out, bool fp64, hipStream_t st);
 hipError_t launch_interp(const float* sampled, int n_bins, int n_sampled_planes, int64_t cells_s,
                          const int32_t* cell_map, int64_t n_cells_out, const int32_t* plane_map,
                          int n_out_planes, float* out, float* sum, hipStream_t st);

@@ -147,6 +147,7 @@ struct sss_subject {
     bool range_check = true;          // TOF offsets may leave the non-zero table columns
     double bin_w = 0, kappa = 0, beta = 0, g_const = 0;
     int64_t n_vox = 0, n_hp = 0, n_samples = 0, samples_cap = 0, n_cells_plane = 0, out_elems = 0;
+    int64_t n_cells = 0;                  // cells of all planes
     int n_tiles = 0, n_chunks = 1;
     int64_t device_bytes = 0;
     bool traced = false, has_activity = false;
@@ -162,7 +163,8 @@ struct sss_subject {
     int2* d_items = nullptr;          // scatter launch: (tile, chunk) per wave, XCD-aware order
     int n_items = 0;
     int32_t* d_slice_ptr = nullptr;
-    int32_t* d_chunk_slices = nullptr;
+    int32_t* d_chunk_sp = nullptr;        // n_chunks + 1 scatter-point bounds
+    int32_t* d_chunk_slice = nullptr;     // slice of each chunk
     int32_t* d_dets = nullptr;
     int32_t* d_flag = nullptr;
     void* d_scan_tmp = nullptr;
@@ -170,7 +172,8 @@ struct sss_subject {
     HalfPathTable hp{};
     SampleTable smp{};
     void* d_out = nullptr;
-    void* d_partial = nullptr;
+    void* d_part_sum = nullptr;           // [chunk][B][planes][cells]
+    float* d_part_cnt = nullptr;          // [chunk][planes][cells]
     std::vector<int32_t> dets;
     hipEvent_t ev_scatter[2] = {nullptr, nullptr};   // around the last scatter kernel
     bool ev_recorded = false;
@@ -181,11 +184,11 @@ struct sss_subject {
             if (e) (void)hipEventDestroy(e);
         dev_free(d_mu); dev_free(d_lam); dev_free(d_sp); dev_free(d_detpos); dev_free(d_detnorm);
         dev_free(d_eff); dev_free(d_ratio); dev_free(d_tiles); dev_free(d_slice_ptr);
-        dev_free(d_chunk_slices); dev_free(d_dets); dev_free(d_flag); dev_free(d_scan_tmp);
+        dev_free(d_chunk_sp); dev_free(d_chunk_slice); dev_free(d_dets); dev_free(d_flag); dev_free(d_scan_tmp);
         dev_free(hp.off); dev_free(hp.n); dev_free(hp.dist); dev_free(hp.att); dev_free(hp.lnatt);
         dev_free(hp.act); dev_free(hp.cosabs); dev_free(hp.status);
         dev_free(smp.rec); dev_free(smp.len); dev_free(smp.a64);
-        dev_free(d_out); dev_free(d_partial); dev_free(d_items);
+        dev_free(d_out); dev_free(d_part_sum); dev_free(d_part_cnt); dev_free(d_items);
     }
 };
 
@@ -410,36 +413,32 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
                 for (int tj = ti; tj < nt; ++tj)
                     tiles.push_back(make_int4(r1, r2, ti * kTileRows, tj * kTileCols));
     s->n_tiles = (int)tiles.size();
-    // slice chunks of about equal scatter-point counts.  By default one chunk
-    // per slice: the launch walks chunk-major, so the waves resident at one
-    // time share one slice's half paths in cache; the per-chunk partial
-    // outputs are capped at 4 GiB.
-    int chunks = d->n_chunks;
-    if (chunks <= 0) {
-        const int64_t out_bytes = std::max<int64_t>(
-            1, s->out_elems * (d->precision == SSS_PRECISION_FP64 ? 8 : 4));
-        chunks = (int)std::max<int64_t>(1, std::min<int64_t>(d->n_slices, (int64_t(4) << 30) / out_bytes));
-    }
-    chunks = std::max(1, std::min(chunks, std::max(1, d->n_slices)));
-    std::vector<int32_t> chunk_slices(1, 0);
-    if (chunks >= d->n_slices) {
-        for (int sl = 1; sl < d->n_slices; ++sl) chunk_slices.push_back(sl);
-    } else {
-        for (int c = 1; c < chunks; ++c) {
-            const int64_t target = (int64_t)d->n_sp * c / chunks;
-            int sl = chunk_slices.back();
-            while (sl < d->n_slices && d->slice_ptr[sl] < target) ++sl;
-            if (sl > chunk_slices.back() && sl < d->n_slices) chunk_slices.push_back(sl);
+    // scatter-point chunks: groups of at most G consecutive points of one
+    // slice.  The launch walks chunk-major, so small groups keep the half
+    // paths that the resident waves read inside each XCD's L2; every chunk
+    // costs one partial-sum and one count array (at most 16 GiB in all).
+    const size_t elem = s->precision == SSS_PRECISION_FP64 ? sizeof(double) : sizeof(float);
+    s->n_cells = (int64_t)d->nru * d->nru * s->n_cells_plane;
+    const int64_t chunk_bytes = std::max<int64_t>(1, s->out_elems * (int64_t)elem + s->n_cells * 4);
+    const int64_t max_chunks = std::max<int64_t>(1, (int64_t(16) << 30) / chunk_bytes);
+    int64_t group = d->n_chunks > 0 ? (d->n_sp + d->n_chunks - 1) / d->n_chunks : kAutoChunkPoints;
+    group = std::max<int64_t>({group, 1, (d->n_sp + max_chunks - 1) / max_chunks});
+    std::vector<int32_t> chunk_sp, chunk_slice;
+    for (int sl = 0; sl < d->n_slices; ++sl) {
+        const int64_t b = d->slice_ptr[sl], m = d->slice_ptr[sl + 1] - b;
+        const int64_t k = (m + group - 1) / group;
+        for (int64_t j = 0; j < k; ++j) {
+            chunk_sp.push_back((int32_t)(b + j * m / k));
+            chunk_slice.push_back(sl);
         }
     }
-    chunk_slices.push_back(d->n_slices);
-    s->n_chunks = (int)chunk_slices.size() - 1;
+    chunk_sp.push_back(d->n_sp);
+    s->n_chunks = (int)chunk_slice.size();
     const std::vector<int2> items = xcd_work_items(tiles, d->nru, s->n_chunks, kScatterWaves);
     s->n_items = (int)items.size();
     std::vector<float> ratio32(d->ratio, d->ratio + d->n_energy);
 
     int64_t& t = s->device_bytes;
-    const size_t elem = s->precision == SSS_PRECISION_FP64 ? sizeof(double) : sizeof(float);
     if ((rc = dev_alloc(&s->d_mu, (size_t)s->n_vox, &t)) || (rc = dev_alloc(&s->d_lam, (size_t)s->n_vox, &t)) ||
         (rc = dev_alloc(&s->d_sp, (size_t)std::max(1, d->n_sp) * 3, &t)) ||
         (rc = dev_alloc(&s->d_detpos, detpos.size(), &t)) || (rc = dev_alloc(&s->d_detnorm, detnorm.size(), &t)) ||
@@ -447,7 +446,8 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
         (rc = dev_alloc(&s->d_tiles, tiles.size() + 1, &t)) ||
         (rc = dev_alloc(&s->d_items, items.size() + 1, &t)) ||
         (rc = dev_alloc(&s->d_slice_ptr, (size_t)d->n_slices + 1, &t)) ||
-        (rc = dev_alloc(&s->d_chunk_slices, chunk_slices.size(), &t)) ||
+        (rc = dev_alloc(&s->d_chunk_sp, chunk_sp.size(), &t)) ||
+        (rc = dev_alloc(&s->d_chunk_slice, chunk_slice.size() + 1, &t)) ||
         (rc = dev_alloc(&s->d_dets, (size_t)d->ndu, &t)) || (rc = dev_alloc(&s->d_flag, 1, &t)) ||
         (rc = dev_alloc(&s->hp.off, (size_t)s->n_hp + 1, &t)) || (rc = dev_alloc(&s->hp.n, (size_t)s->n_hp + 1, &t)) ||
         (rc = dev_alloc(&s->hp.dist, (size_t)s->n_hp + 1, &t)) || (rc = dev_alloc(&s->hp.att, (size_t)s->n_hp + 1, &t)) ||
@@ -455,17 +455,19 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
         (rc = dev_alloc(&s->hp.cosabs, (size_t)s->n_hp + 1, &t)) ||
         (rc = dev_alloc(&s->hp.status, (size_t)s->n_hp + 1, &t)) ||
         (rc = dev_alloc(reinterpret_cast<unsigned char**>(&s->d_out), (size_t)s->out_elems * elem + 1, &t)) ||
-        (s->n_chunks > 1 &&
-         (rc = dev_alloc(reinterpret_cast<unsigned char**>(&s->d_partial),
-                         (size_t)s->out_elems * elem * s->n_chunks, &t)))) {
+        (rc = dev_alloc(reinterpret_cast<unsigned char**>(&s->d_part_sum),
+                        (size_t)s->out_elems * elem * s->n_chunks + 1, &t)) ||
+        (rc = dev_alloc(&s->d_part_cnt, (size_t)s->n_cells * s->n_chunks + 1, &t))) {
         delete s;
         return rc;
     }
     hipStream_t st = ctx->stream;
     // cells no lane owns (i1 == i2 on same-ring planes) are never written: zero once
     hipError_t e = hipMemsetAsync(s->d_out, 0, (size_t)s->out_elems * elem, st);
-    if (e == hipSuccess && s->d_partial)
-        e = hipMemsetAsync(s->d_partial, 0, (size_t)s->out_elems * elem * s->n_chunks, st);
+    if (e == hipSuccess)
+        e = hipMemsetAsync(s->d_part_sum, 0, (size_t)s->out_elems * elem * s->n_chunks + 1, st);
+    if (e == hipSuccess)
+        e = hipMemsetAsync(s->d_part_cnt, 0, ((size_t)s->n_cells * s->n_chunks + 1) * sizeof(float), st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(s->d_mu, d->mu, (size_t)s->n_vox * sizeof(double), hipMemcpyHostToDevice, st);
     if (e == hipSuccess && d->n_sp > 0)
@@ -477,7 +479,8 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
     if (e == hipSuccess && !tiles.empty()) e = hipMemcpyAsync(s->d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, st);
     if (e == hipSuccess && !items.empty()) e = hipMemcpyAsync(s->d_items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(s->d_slice_ptr, d->slice_ptr, ((size_t)d->n_slices + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(s->d_chunk_slices, chunk_slices.data(), chunk_slices.size() * sizeof(int32_t), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->d_chunk_sp, chunk_sp.data(), chunk_sp.size() * sizeof(int32_t), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && !chunk_slice.empty()) e = hipMemcpyAsync(s->d_chunk_slice, chunk_slice.data(), chunk_slice.size() * sizeof(int32_t), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(s->d_dets, d->dets, (size_t)d->ndu * sizeof(int32_t), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipEventCreate(&s->ev_scatter[0]);
     if (e == hipSuccess) e = hipEventCreate(&s->ev_scatter[1]);
@@ -591,7 +594,7 @@ static ScatterArgs scatter_args(const sss_subject* s)
     a.s_rec = s->smp.rec; a.s_a64 = s->smp.a64;
     a.detpos = s->d_detpos; a.eff = s->d_eff; a.ratio = s->d_ratio; a.tiles = s->d_tiles;
     a.items = s->d_items; a.n_items = s->n_items;
-    a.slice_ptr = s->d_slice_ptr; a.chunk_slices = s->d_chunk_slices;
+    a.slice_ptr = s->d_slice_ptr; a.chunk_sp = s->d_chunk_sp;
     const double log2e = 1.4426950408889634;
     a.inv_w_bins = 1.0 / s->bin_w;
     a.mid = (s->n_bins - 1) / 2.0;
@@ -612,7 +615,8 @@ static ScatterArgs scatter_args(const sss_subject* s)
     a.f_c = (float)a.c_beta;
     a.g_const = s->g_const;
     a.kn_gamma = (511.0e3 * 1.6e-19) / (9.10938356e-31 * (299792458.0 * 299792458.0));
-    a.out = s->n_chunks > 1 ? s->d_partial : s->d_out;
+    a.part_sum = s->d_part_sum;
+    a.part_cnt = s->d_part_cnt;
     return a;
 }
 
@@ -629,8 +633,8 @@ int sss_subject_scatter(sss_subject* s)
     SSS_HIP(launch_scatter(a, fp64, s->trivial, st));
     SSS_HIP(hipEventRecord(s->ev_scatter[1], st));
     s->ev_recorded = true;
-    if (s->n_chunks > 1)
-        SSS_HIP(launch_chunk_reduce_typed(s->d_partial, s->n_chunks, s->out_elems, s->d_out, fp64, st));
+    SSS_HIP(launch_chunk_reduce_typed(s->d_part_sum, s->d_part_cnt, s->d_chunk_slice, s->n_chunks,
+                                      s->out_elems, s->n_cells, s->d_out, fp64, st));
     return SSS_OK;
 }
 

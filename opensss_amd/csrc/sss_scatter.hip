@@ -323,64 +323,46 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
 
     const int plane = r2 + r1 * nru;
     const int64_t cell = (int64_t)i1 * ndu - (int64_t)i1 * (i1 - 1) / 2 + (i2 - i1);
-    const int64_t planes = (int64_t)nru * nru;
-    R* dst = reinterpret_cast<R*>(a.out) + ((int64_t)chunk * B * planes + plane) * a.n_cells_plane + cell;
-    const int64_t bin_stride = planes * a.n_cells_plane;
+    const int64_t n_cells = (int64_t)nru * nru * a.n_cells_plane;   // cells of all planes
+    const int64_t pc = (int64_t)plane * a.n_cells_plane + cell;
 
-    const int s_begin = a.chunk_slices[chunk];
-    const int s_end = a.chunk_slices[chunk + 1];
-    for (int sl = s_begin; sl < s_end; ++sl) {
-        R num = (R)0;   // one-bin path
-        float cnt = 0.0f;
-        BinWindow<R, HM> win;
-        win.reset((R)0);
-        const int p_end = a.slice_ptr[sl + 1];
-        int cursor = valid ? a.slice_ptr[sl] : p_end;
-        while (true) {
-            // next scatter point of this lane's cell that the reference keeps
-            Pair<R> p;
-            bool found = false;
-            while (cursor < p_end) {
-                found = pair_eval<R>(a, cursor, d1, d2, lor2, eff_ref, p);
-                ++cursor;
-                if (found) break;
-            }
-            if (!__any(found)) break;
-            if (found) {
-                cnt += p.counts;
-                if constexpr (TRIVIAL) {
-                    num += p.coef1 * p.act1 + p.coef2 * p.act2;
-                } else if (a.t_hi >= a.t_lo) {   // else every TOF weight is zero
-                    fold_path<R, HM, SINGLE_EXP, RANGE, -1>(a, tc, col, win, p.b1, p.n1, p.shift, p.coef1);
-                    fold_path<R, HM, SINGLE_EXP, RANGE, +1>(a, tc, col, win, p.b2, p.n2, p.shift, p.coef2);
-                }
-            }
+    R num = (R)0;   // one-bin path
+    float cnt = 0.0f;
+    BinWindow<R, HM> win;
+    win.reset((R)0);
+    const int p_end = a.chunk_sp[chunk + 1];
+    int cursor = valid ? a.chunk_sp[chunk] : p_end;
+    while (true) {
+        // next scatter point of this lane's cell that the reference keeps
+        Pair<R> p;
+        bool found = false;
+        while (cursor < p_end) {
+            found = pair_eval<R>(a, cursor, d1, d2, lor2, eff_ref, p);
+            ++cursor;
+            if (found) break;
         }
-        // slice sum / slice count into the chunk's output (0/0 -> 0, :201)
-        if constexpr (TRIVIAL) {
-            if (valid) {
-                const R v = cnt > 0.0f ? num / (R)cnt : (R)0;
-                dst[0] = sl == s_begin ? v : dst[0] + v;
-            }
-        } else {
-            win.spill(col);
-            const R inv = cnt > 0.0f ? (R)1 / (R)cnt : (R)0;
-            for (int b = 0; b < B; ++b) {
-                R* q = col + (b + HM) * kWave;
-                const R v = *q * inv;
-                *q = (R)0;
-                if (valid) {
-                    R* o = dst + b * bin_stride;
-                    *o = sl == s_begin ? v : *o + v;
-                }
-            }
-            // window reach rows (bins beyond the ends) are dropped
-            for (int r = 0; r < HM; ++r) {
-                col[r * kWave] = (R)0;
-                col[(B + HM + r) * kWave] = (R)0;
+        if (!__any(found)) break;
+        if (found) {
+            cnt += p.counts;
+            if constexpr (TRIVIAL) {
+                num += p.coef1 * p.act1 + p.coef2 * p.act2;
+            } else if (a.t_hi >= a.t_lo) {   // else every TOF weight is zero
+                fold_path<R, HM, SINGLE_EXP, RANGE, -1>(a, tc, col, win, p.b1, p.n1, p.shift, p.coef1);
+                fold_path<R, HM, SINGLE_EXP, RANGE, +1>(a, tc, col, win, p.b2, p.n2, p.shift, p.coef2);
             }
         }
     }
+    // the chunk's sums and pair count; window reach rows (bins beyond the
+    // ends) are dropped
+    if constexpr (!TRIVIAL) win.spill(col);
+    if (!valid) return;
+    R* dst = reinterpret_cast<R*>(a.part_sum) + (int64_t)chunk * B * n_cells + pc;
+    if constexpr (TRIVIAL) {
+        dst[0] = num;
+    } else {
+        for (int b = 0; b < B; ++b) dst[b * n_cells] = col[(b + HM) * kWave];
+    }
+    a.part_cnt[(int64_t)chunk * n_cells + pc] = cnt;
 }
 
 // Counts the (scatter point, LOR) pairs that pass the reference's filters and
@@ -410,8 +392,8 @@ __global__ __launch_bounds__(256) void k_pair_stats(ScatterArgs a, unsigned long
     }
     const double eff_ref = a.eff[a.eff_ref_row];
     unsigned long long pairs = 0, samples = 0;
-    const int p_end = a.slice_ptr[a.chunk_slices[a.n_chunks]];
-    for (int sp = a.slice_ptr[a.chunk_slices[0]]; sp < p_end; ++sp) {
+    const int p_end = a.chunk_sp[a.n_chunks];
+    for (int sp = a.chunk_sp[0]; sp < p_end; ++sp) {
         Pair<float> p;
         if (!pair_eval<float>(a, sp, d1, d2, lor2, eff_ref, p)) continue;
         ++pairs;
@@ -421,14 +403,32 @@ __global__ __launch_bounds__(256) void k_pair_stats(ScatterArgs a, unsigned long
     atomicAdd(out + 1, samples);
 }
 
+// Per output element: sum the chunks of each slice, divide by the slice's
+// pair count (0/0 -> 0, SingleScatterSimulationTOF.py:201,555-557) and add
+// the slices in order.  Chunks of one slice are consecutive.
 template <typename R>
-__global__ void k_chunk_reduce(const R* __restrict__ partial, int n_chunks, int64_t n, R* out)
+__global__ void k_chunk_reduce(const R* __restrict__ part_sum, const float* __restrict__ part_cnt,
+                               const int32_t* __restrict__ chunk_slice, int n_chunks, int64_t n,
+                               int64_t n_cells, R* out)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    R s = partial[i];
-    for (int c = 1; c < n_chunks; ++c) s += partial[(int64_t)c * n + i];
-    out[i] = s;
+    const int64_t pc = i % n_cells;
+    R total = (R)0, num = (R)0;
+    float cnt = 0.0f;
+    int slice = n_chunks > 0 ? chunk_slice[0] : 0;
+    for (int c = 0; c < n_chunks; ++c) {
+        if (chunk_slice[c] != slice) {
+            if (cnt > 0.0f) total += num / (R)cnt;
+            num = (R)0;
+            cnt = 0.0f;
+            slice = chunk_slice[c];
+        }
+        num += part_sum[(int64_t)c * n + i];
+        cnt += part_cnt[(int64_t)c * n_cells + pc];
+    }
+    if (cnt > 0.0f) total += num / (R)cnt;
+    out[i] = total;
 }
 
 template <typename R>
@@ -513,17 +513,20 @@ hipError_t launch_pair_stats(const ScatterArgs& a, unsigned long long* out, hipS
     return hipGetLastError();
 }
 
-hipError_t launch_chunk_reduce_typed(const void* partial, int n_chunks, int64_t n, void* out,
-                                     bool fp64, hipStream_t st)
+hipError_t launch_chunk_reduce_typed(const void* part_sum, const float* part_cnt,
+                                     const int32_t* chunk_slice, int n_chunks, int64_t n,
+                                     int64_t n_cells, void* out, bool fp64, hipStream_t st)
 {
     if (n == 0) return hipSuccess;
     const unsigned blocks = (unsigned)((n + 255) / 256);
     if (fp64)
         hipLaunchKernelGGL(k_chunk_reduce<double>, dim3(blocks), dim3(256), 0, st,
-                           (const double*)partial, n_chunks, n, (double*)out);
+                           (const double*)part_sum, part_cnt, chunk_slice, n_chunks, n, n_cells,
+                           (double*)out);
     else
         hipLaunchKernelGGL(k_chunk_reduce<float>, dim3(blocks), dim3(256), 0, st,
-                           (const float*)partial, n_chunks, n, (float*)out);
+                           (const float*)part_sum, part_cnt, chunk_slice, n_chunks, n, n_cells,
+                           (float*)out);
     return hipGetLastError();
 }
 
