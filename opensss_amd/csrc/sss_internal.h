@@ -13,8 +13,15 @@ namespace sss {
 constexpr int kTofHalf = 4000;            // TOF table half width, ps
 constexpr double kLightCmPerPs = 299792458.0 * 1e-10;
 constexpr int kWave = 64;
-constexpr int kTileRows = 8;              // detector tile of one wave: 8 x 8 cells
+// A wave owns 8 x 8 cells: 8 rows (ring r1, detector i1) x 8 columns (ring
+// r2, detector i2), each a block of 2^k rings x 8/2^k detectors.  Path
+// lengths vary far less across neighbouring rings than across neighbouring
+// detectors, so ring blocks keep the lanes' trip counts close (simulated on
+// config 2: 83% of lane slots busy with 4 rings x 2 detectors, 58% with
+// 1 ring x 8 detectors).
+constexpr int kTileRows = 8;
 constexpr int kTileCols = 8;
+constexpr int kTileMaxRings = 4;
 constexpr int kScatterWaves = 4;          // waves per workgroup of the scatter launch
 constexpr int kAutoChunkPoints = 8;       // scatter points per chunk by default
 constexpr int kMaxBins = 32;
@@ -89,7 +96,8 @@ struct ScatterArgs {
     const double* detpos;        // (NRU, NDU, 3)
     const double* eff;           // n_energy
     const float* ratio;          // n_energy
-    const int4* tiles;           // (r1, r2, i1_0, i2_0)
+    const int4* tiles;           // (r1_0, r2_0, i1_0, i2_0) block origins
+    int tile_db_log2;            // log2 of the detectors per row/column block
     const int2* items;           // per wave of the scatter launch: (tile or -1, chunk)
     int n_items;
     const int32_t* slice_ptr;    // n_slices + 1

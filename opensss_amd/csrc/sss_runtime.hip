@@ -149,6 +149,7 @@ struct sss_subject {
     int64_t n_vox = 0, n_hp = 0, n_samples = 0, samples_cap = 0, n_cells_plane = 0, out_elems = 0;
     int64_t n_cells = 0;                  // cells of all planes
     int n_tiles = 0, n_chunks = 1;
+    int tile_db_log2 = 3;                 // log2 detectors per tile row/column block
     int64_t device_bytes = 0;
     bool traced = false, has_activity = false;
 
@@ -404,14 +405,21 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
     // inside a table whose columns are all non-zero
     const double tof_bound = tof_offset_bound(g, d->sp_xyz, d->n_sp, detpos.data(), d->nru * d->ndu);
     s->range_check = !(s->t_lo == -kTofHalf && s->t_hi == kTofHalf && tof_bound < kTofHalf - 2);
-    // wave tiles of 8 x 8 cells over the upper triangle of every ring-pair plane
+    // wave tiles: rows and columns are blocks of rb rings x db detectors
+    // (rb * db = 8, rb the largest power of two <= min(4, NRU)); detector
+    // blocks cover the upper triangle i1 <= i2
+    int rb = 1;
+    while (rb * 2 <= std::min(kTileMaxRings, d->nru)) rb *= 2;
+    const int db = kTileRows / rb;
+    s->tile_db_log2 = 0;
+    while ((1 << s->tile_db_log2) < db) ++s->tile_db_log2;
     std::vector<int4> tiles;
-    const int nt = (d->ndu + kTileRows - 1) / kTileRows;
-    for (int r1 = 0; r1 < d->nru; ++r1)
-        for (int r2 = 0; r2 < d->nru; ++r2)
-            for (int ti = 0; ti < nt; ++ti)
-                for (int tj = ti; tj < nt; ++tj)
-                    tiles.push_back(make_int4(r1, r2, ti * kTileRows, tj * kTileCols));
+    const int ndb = (d->ndu + db - 1) / db;
+    for (int r1 = 0; r1 < d->nru; r1 += rb)
+        for (int r2 = 0; r2 < d->nru; r2 += rb)
+            for (int ti = 0; ti < ndb; ++ti)
+                for (int tj = ti; tj < ndb; ++tj)
+                    tiles.push_back(make_int4(r1, r2, ti * db, tj * db));
     s->n_tiles = (int)tiles.size();
     // scatter-point chunks: groups of at most G consecutive points of one
     // slice.  The launch walks chunk-major, so small groups keep the half
@@ -593,7 +601,7 @@ static ScatterArgs scatter_args(const sss_subject* s)
     a.hp_lnatt = s->hp.lnatt; a.hp_act = s->hp.act; a.hp_cos = s->hp.cosabs;
     a.s_rec = s->smp.rec; a.s_a64 = s->smp.a64;
     a.detpos = s->d_detpos; a.eff = s->d_eff; a.ratio = s->d_ratio; a.tiles = s->d_tiles;
-    a.items = s->d_items; a.n_items = s->n_items;
+    a.items = s->d_items; a.n_items = s->n_items; a.tile_db_log2 = s->tile_db_log2;
     a.slice_ptr = s->d_slice_ptr; a.chunk_sp = s->d_chunk_sp;
     const double log2e = 1.4426950408889634;
     a.inv_w_bins = 1.0 / s->bin_w;

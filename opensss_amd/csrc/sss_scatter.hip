@@ -1,8 +1,9 @@
 // Fused single-scatter sum (the bodySSS LOR loop, SingleScatterSimulationTOF.py:433-557).
 //
-// Work decomposition: one wavefront owns an 8 x 8 tile of cells (i1, i2) of
-// one ring-pair plane (r1, r2) for a contiguous range of normalisation
-// slices (a chunk); the (tile, chunk) items are listed per wave in an
+// Work decomposition: one wavefront owns an 8 x 8 tile of cells, rows
+// (r1, i1) x columns (r2, i2), each a block of up to 4 rings x 2 detectors,
+// for one chunk of scatter points (a few consecutive points of one
+// normalisation slice); the (tile, chunk) items are listed per wave in an
 // XCD-aware order built on the host.  Each lane accumulates its own cell over the scatter
 // points of a slice, then adds sum/count of the slice (the per-slice
 // normalisation of :555-557) into the chunk's output.  No global atomics and
@@ -61,6 +62,32 @@ template <typename R>
 __device__ __forceinline__ void lds_add(R* p, R v)
 {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Cell of one lane of a tile: row lane/8 is (ring r1_0 + row >> k, detector
+// i1_0 + row & (2^k - 1)), column lane%8 likewise for (r2, i2).  valid: the
+// cell exists and is one the reference computes (D1 < D2, or D1 == D2 on
+// different rings, SingleScatterSimulationTOF.py:466-469).  d1, d2: sampled
+// detector index ring * NDU + detector, clamped into range for invalid lanes.
+struct LaneCell {
+    int r1, i1, r2, i2, d1, d2;
+    bool valid;
+};
+
+__device__ __forceinline__ LaneCell lane_cell(const ScatterArgs& a, int4 tl, int lane)
+{
+    const int row = lane / kTileCols, cl = lane % kTileCols;
+    const int k = a.tile_db_log2, m = (1 << k) - 1;
+    LaneCell c;
+    c.r1 = tl.x + (row >> k);
+    c.i1 = tl.z + (row & m);
+    c.r2 = tl.y + (cl >> k);
+    c.i2 = tl.w + (cl & m);
+    c.valid = c.r1 < a.nru && c.r2 < a.nru && c.i1 < a.ndu && c.i2 < a.ndu &&
+              (c.i1 < c.i2 || (c.i1 == c.i2 && c.r1 != c.r2));
+    c.d1 = min(c.r1, a.nru - 1) * a.ndu + min(c.i1, a.ndu - 1);
+    c.d2 = min(c.r2, a.nru - 1) * a.ndu + min(c.i2, a.ndu - 1);
+    return c;
 }
 
 // Geometry and coefficients of one (scatter point, cell) pair.
@@ -303,14 +330,10 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
         for (int r = 0; r < rows; ++r) col[r * kWave] = (R)0;
     }
 
-    const int4 tl = a.tiles[tile];
-    const int r1 = tl.x, r2 = tl.y;
+    const LaneCell lc = lane_cell(a, a.tiles[tile], lane);
+    const int r1 = lc.r1, r2 = lc.r2, i1 = lc.i1, i2 = lc.i2, d1 = lc.d1, d2 = lc.d2;
+    const bool valid = lc.valid;
     const int ndu = a.ndu, nru = a.nru;
-    const int i1 = tl.z + lane / kTileCols;
-    const int i2 = tl.w + lane % kTileCols;
-    const bool valid = (i1 < ndu) && (i2 < ndu) && (i1 < i2 || (i1 == i2 && r1 != r2));
-    const int d1 = r1 * ndu + (i1 < ndu ? i1 : ndu - 1);
-    const int d2 = r2 * ndu + (i2 < ndu ? i2 : ndu - 1);
     double lor2;
     {
         const double dx = a.detpos[3 * d1 + 0] - a.detpos[3 * d2 + 0];
@@ -374,15 +397,9 @@ __global__ __launch_bounds__(256) void k_pair_stats(ScatterArgs a, unsigned long
     const int lane = threadIdx.x % kWave;
     const int tile = blockIdx.x * (blockDim.x / kWave) + wave;
     if (tile >= a.n_tiles) return;
-    const int4 tl = a.tiles[tile];
-    const int r1 = tl.x, r2 = tl.y;
-    const int ndu = a.ndu;
-    const int i1 = tl.z + lane / kTileCols;
-    const int i2 = tl.w + lane % kTileCols;
-    const bool valid = (i1 < ndu) && (i2 < ndu) && (i1 < i2 || (i1 == i2 && r1 != r2));
-    if (!valid) return;
-    const int d1 = r1 * ndu + i1;
-    const int d2 = r2 * ndu + i2;
+    const LaneCell lc = lane_cell(a, a.tiles[tile], lane);
+    if (!lc.valid) return;
+    const int d1 = lc.d1, d2 = lc.d2;
     double lor2;
     {
         const double dx = a.detpos[3 * d1 + 0] - a.detpos[3 * d2 + 0];
