@@ -420,32 +420,65 @@ __global__ __launch_bounds__(256) void k_pair_stats(ScatterArgs a, unsigned long
     atomicAdd(out + 1, samples);
 }
 
-// Per output element: sum the chunks of each slice, divide by the slice's
-// pair count (0/0 -> 0, SingleScatterSimulationTOF.py:201,555-557) and add
-// the slices in order.  Chunks of one slice are consecutive.
-template <typename R>
+// Per cell (all bins in one thread, so the count is read once per chunk):
+// sum the chunks of each slice, divide by the slice's pair count (0/0 -> 0,
+// SingleScatterSimulationTOF.py:201,555-557) and add the slices in order.
+// Chunks of one slice are consecutive.
+template <typename R, int BMAX>
 __global__ void k_chunk_reduce(const R* __restrict__ part_sum, const float* __restrict__ part_cnt,
-                               const int32_t* __restrict__ chunk_slice, int n_chunks, int64_t n,
+                               const int32_t* __restrict__ chunk_slice, int n_chunks, int n_bins,
                                int64_t n_cells, R* out)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int64_t pc = i % n_cells;
-    R total = (R)0, num = (R)0;
+    const int64_t pc = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pc >= n_cells) return;
+    const int64_t n = (int64_t)n_bins * n_cells;
+    R total[BMAX], num[BMAX];
+#pragma unroll
+    for (int b = 0; b < BMAX; ++b) total[b] = num[b] = (R)0;
     float cnt = 0.0f;
     int slice = n_chunks > 0 ? chunk_slice[0] : 0;
     for (int c = 0; c < n_chunks; ++c) {
         if (chunk_slice[c] != slice) {
-            if (cnt > 0.0f) total += num / (R)cnt;
-            num = (R)0;
+            if (cnt > 0.0f) {
+#pragma unroll
+                for (int b = 0; b < BMAX; ++b) total[b] += num[b] / (R)cnt;
+            }
+#pragma unroll
+            for (int b = 0; b < BMAX; ++b) num[b] = (R)0;
             cnt = 0.0f;
             slice = chunk_slice[c];
         }
-        num += part_sum[(int64_t)c * n + i];
+        const R* src = part_sum + (int64_t)c * n + pc;
+#pragma unroll
+        for (int b = 0; b < BMAX; ++b)
+            if (b < n_bins) num[b] += src[b * n_cells];
         cnt += part_cnt[(int64_t)c * n_cells + pc];
     }
-    if (cnt > 0.0f) total += num / (R)cnt;
-    out[i] = total;
+#pragma unroll
+    for (int b = 0; b < BMAX; ++b) {
+        if (cnt > 0.0f) total[b] += num[b] / (R)cnt;
+        if (b < n_bins) out[b * n_cells + pc] = total[b];
+    }
+}
+
+template <typename R>
+hipError_t launch_reduce_typed(const R* part_sum, const float* part_cnt, const int32_t* chunk_slice,
+                               int n_chunks, int n_bins, int64_t n_cells, R* out, hipStream_t st)
+{
+    const unsigned blocks = (unsigned)((n_cells + 255) / 256);
+    if (n_bins <= 4)
+        hipLaunchKernelGGL((k_chunk_reduce<R, 4>), dim3(blocks), dim3(256), 0, st, part_sum, part_cnt,
+                           chunk_slice, n_chunks, n_bins, n_cells, out);
+    else if (n_bins <= 8)
+        hipLaunchKernelGGL((k_chunk_reduce<R, 8>), dim3(blocks), dim3(256), 0, st, part_sum, part_cnt,
+                           chunk_slice, n_chunks, n_bins, n_cells, out);
+    else if (n_bins <= 16)
+        hipLaunchKernelGGL((k_chunk_reduce<R, 16>), dim3(blocks), dim3(256), 0, st, part_sum, part_cnt,
+                           chunk_slice, n_chunks, n_bins, n_cells, out);
+    else
+        hipLaunchKernelGGL((k_chunk_reduce<R, 32>), dim3(blocks), dim3(256), 0, st, part_sum, part_cnt,
+                           chunk_slice, n_chunks, n_bins, n_cells, out);
+    return hipGetLastError();
 }
 
 template <typename R>
@@ -534,17 +567,14 @@ hipError_t launch_chunk_reduce_typed(const void* part_sum, const float* part_cnt
                                      const int32_t* chunk_slice, int n_chunks, int64_t n,
                                      int64_t n_cells, void* out, bool fp64, hipStream_t st)
 {
-    if (n == 0) return hipSuccess;
-    const unsigned blocks = (unsigned)((n + 255) / 256);
+    if (n == 0 || n_cells == 0) return hipSuccess;
+    const int n_bins = (int)(n / n_cells);
+    if (n_bins > kMaxBins) return hipErrorInvalidValue;
     if (fp64)
-        hipLaunchKernelGGL(k_chunk_reduce<double>, dim3(blocks), dim3(256), 0, st,
-                           (const double*)part_sum, part_cnt, chunk_slice, n_chunks, n, n_cells,
-                           (double*)out);
-    else
-        hipLaunchKernelGGL(k_chunk_reduce<float>, dim3(blocks), dim3(256), 0, st,
-                           (const float*)part_sum, part_cnt, chunk_slice, n_chunks, n, n_cells,
-                           (float*)out);
-    return hipGetLastError();
+        return launch_reduce_typed<double>((const double*)part_sum, part_cnt, chunk_slice, n_chunks,
+                                           n_bins, n_cells, (double*)out, st);
+    return launch_reduce_typed<float>((const float*)part_sum, part_cnt, chunk_slice, n_chunks, n_bins,
+                                      n_cells, (float*)out, st);
 }
 
 hipError_t launch_dense_scatter_typed(const void* compact, bool fp64, int n_bins, int nru, int ndu,

@@ -89,21 +89,31 @@ __global__ void k_halfpath_fill(Grid g, const double* __restrict__ sp_xyz, int n
     t.status[h] = status;
 }
 
+// Emission terms lam[v] L per sample and their float64 sum per half path
+// (ActivityRay and ActPath, SingleScatterSimulationTOF.py:417-418).  A
+// group of kRefreshLanes lanes walks one half path with consecutive samples
+// on consecutive lanes (coalesced), then sums its partial sums; the terms are
+// non-negative, so the exact-zero tests on the sum do not depend on the order.
+constexpr int kRefreshLanes = 16;
+
 __global__ void k_activity_refresh(HalfPathTable t, SampleTable s, int64_t n_hp,
                                    const double* __restrict__ lam)
 {
-    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= n_hp) return;
-    const int64_t base = t.off[h];
-    const int n = t.n[h];
+    const int64_t h = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kRefreshLanes;
+    const int j = threadIdx.x % kRefreshLanes;
     double act = 0;
-    for (int k = 0; k < n; ++k) {
-        const double ar = lam[s.rec[base + k].vox] * s.len[base + k];
-        s.rec[base + k].a = (float)ar;
-        if (s.a64) s.a64[base + k] = ar;
-        act += ar;   // sequential float64 sum, as the reference's np.sum under numba
+    if (h < n_hp) {
+        const int64_t base = t.off[h];
+        const int n = t.n[h];
+        for (int k = j; k < n; k += kRefreshLanes) {
+            const double ar = lam[s.rec[base + k].vox] * s.len[base + k];
+            s.rec[base + k].a = (float)ar;
+            if (s.a64) s.a64[base + k] = ar;
+            act += ar;
+        }
     }
-    t.act[h] = act;
+    for (int o = kRefreshLanes / 2; o > 0; o /= 2) act += __shfl_xor(act, o, kRefreshLanes);
+    if (h < n_hp && j == 0) t.act[h] = act;
 }
 
 __global__ void k_trace_segments(Grid g, const double* __restrict__ segs, int64_t n, int cap,
@@ -175,8 +185,8 @@ hipError_t launch_activity_refresh(const HalfPathTable t, SampleTable s, int64_t
                                    const double* lam, hipStream_t st)
 {
     if (n_hp == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_activity_refresh, dim3(blocks_for(n_hp, 128)), dim3(128), 0, st, t, s, n_hp,
-                       lam);
+    hipLaunchKernelGGL(k_activity_refresh, dim3(blocks_for(n_hp * kRefreshLanes, 256)), dim3(256), 0, st,
+                       t, s, n_hp, lam);
     return hipGetLastError();
 }
 
