@@ -70,6 +70,17 @@ def subject(w, precision):
                           precision=precision)
 
 
+def profile_traffic(workload):
+    """HBM bytes per k_scatter launch from the committed rocprofv3 summary of
+    this workload (calibrated FETCH_SIZE + WRITE_SIZE), or (None, None)."""
+    import re
+    path = os.path.join(ROOT, 'profiles', f'k_scatter_{workload}.md')
+    if not os.path.exists(path):
+        return None, None
+    m = re.search(r'HBM traffic per launch \(bytes\):\s*([0-9.eE+]+)', open(path).read())
+    return (float(m.group(1)), os.path.relpath(path, ROOT)) if m else (None, None)
+
+
 def rel_err_above(got, want, frac=0.01):
     got = np.asarray(got, dtype=np.float64)
     want = np.asarray(want, dtype=np.float64)
@@ -147,9 +158,14 @@ def main():
     pairs, samples = s.work_stats()
     flops = 2.0 * w.n_bins * samples + 64.0 * pairs
     achieved = flops / (k_ms * 1e-3) / 1e12
-    roofline = {'bound': 'valu_fp32', 'achieved': achieved, 'peak': FP32_VALU_PEAK_TFLOPS,
-                'unit': 'TFLOP/s', 'frac': achieved / FP32_VALU_PEAK_TFLOPS, 'traffic': None,
-                'kernel': 'k_scatter', 'kernel_ms': k_ms, 'pairs': pairs, 'samples': samples}
+    traffic, traffic_src = profile_traffic(w.name)
+    # FP32 compute bound: MI355X's dense FP32 peak (157.3 TFLOP/s) is the same
+    # for its matrix and vector pipes; this kernel issues on the vector pipe
+    roofline = {'bound': 'mfma', 'pipe': 'valu_fp32', 'achieved': achieved,
+                'peak': FP32_VALU_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                'frac': achieved / FP32_VALU_PEAK_TFLOPS, 'traffic': traffic,
+                'traffic_source': traffic_src, 'kernel': 'k_scatter', 'kernel_ms': k_ms,
+                'flops_per_launch': flops, 'pairs': pairs, 'samples': samples}
     out32 = s.output_compact()
     s.close()
 

@@ -9,6 +9,7 @@ Module map (reference file -> module):
 * ``Sinogram.py``                   -> :mod:`opensss_amd.sinogram`
 * ``DataReduction.py``              -> :mod:`opensss_amd.data_reduction`
 * ``Geometry.py``                   -> :mod:`opensss_amd.geometry`
+* ``Interpolation.py``              -> :mod:`opensss_amd.interpolation`
 """
 from .data_reduction import ComputeLORCoordinates, SinogramLUT, SumLORsCounts
 from .geometry import (ImportCASToRGeometry, PrepareGeometryAndSinograms, expandRings,
@@ -19,6 +20,7 @@ __all__ = [
     'ComputeLORCoordinates', 'SinogramLUT', 'SumLORsCounts', 'ImportCASToRGeometry',
     'PrepareGeometryAndSinograms', 'expandRings', 'mashDetectors', 'SinogramCoordinates',
     'SingleScatterSimulationTOF', 'ScatterSubject', 'RayTracing3DTOF', 'RayTracing3DTOFBatch',
+    'LineIntegrals', 'InterpolateScatters',
 ]
 
 
@@ -30,4 +32,7 @@ def __getattr__(name):
     if name in ('RayTracing3DTOF', 'RayTracing3DTOFBatch', 'LineIntegrals'):
         from . import raytracing
         return getattr(raytracing, name)
+    if name in ('InterpolateScatters', 'Interpolator'):
+        from . import interpolation
+        return getattr(interpolation, name)
     raise AttributeError(name)

@@ -113,10 +113,14 @@ struct ScatterArgs {
     float f_inv_w, f_kap, f_mid, f_jmax, f_wk, f_mid_wk, f_c;
     double g_const;              // (Dx Dy 1e-2)^2 / (4 pi)
     double kn_gamma;
-    // output: per chunk, the un-normalised sums and the pair counts; the
-    // chunk reduce divides per slice and sums the slices
-    void* part_sum;              // [chunk][B][NRU^2][cells], float or double
-    float* part_cnt;             // [chunk][NRU^2][cells]
+    // output: per chunk, the un-normalised sums and the pair counts of every
+    // lane slot (tile * 64 + lane, so a wave writes 64 contiguous values per
+    // bin); the chunk reduce divides per slice, sums the slices and scatters
+    // the slots into the compact output
+    void* part_sum;              // [chunk][B][n_tiles * 64], float or double
+    float* part_cnt;             // [chunk][n_tiles * 64]
+    const int32_t* chunk_slice;  // n_chunks: slice of each chunk
+    void* out;                   // compact output [B][NRU^2][cells]
 };
 
 // Largest |TOF offset| in ps the fold can produce for a subject: the
@@ -142,9 +146,7 @@ hipError_t launch_line_integrals(const Grid& g, const double* segs, int64_t n, c
                                  double* lam_int, hipStream_t st);
 hipError_t launch_scatter(const ScatterArgs& a, bool fp64, bool trivial, hipStream_t st);
 hipError_t launch_pair_stats(const ScatterArgs& a, unsigned long long* out, hipStream_t st);
-hipError_t launch_chunk_reduce_typed(const void* part_sum, const float* part_cnt,
-                                     const int32_t* chunk_slice, int n_chunks, int64_t n,
-                                     int64_t n_cells, void* out, bool fp64, hipStream_t st);
+hipError_t launch_chunk_reduce(const ScatterArgs& a, bool fp64, hipStream_t st);
 hipError_t launch_interp(const float* sampled, int n_bins, int n_sampled_planes, int64_t cells_s,
                          const int32_t* cell_map, int64_t n_cells_out, const int32_t* plane_map,
                          int n_out_planes, float* out, float* sum, hipStream_t st);

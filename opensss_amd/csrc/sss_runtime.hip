@@ -173,8 +173,9 @@ struct sss_subject {
     HalfPathTable hp{};
     SampleTable smp{};
     void* d_out = nullptr;
-    void* d_part_sum = nullptr;           // [chunk][B][planes][cells]
-    float* d_part_cnt = nullptr;          // [chunk][planes][cells]
+    void* d_part_sum = nullptr;           // [chunk][B][slot], slot = tile * 64 + lane
+    float* d_part_cnt = nullptr;          // [chunk][slot]
+    int64_t n_slots = 0;
     std::vector<int32_t> dets;
     hipEvent_t ev_scatter[2] = {nullptr, nullptr};   // around the last scatter kernel
     bool ev_recorded = false;
@@ -427,7 +428,9 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
     // costs one partial-sum and one count array (at most 16 GiB in all).
     const size_t elem = s->precision == SSS_PRECISION_FP64 ? sizeof(double) : sizeof(float);
     s->n_cells = (int64_t)d->nru * d->nru * s->n_cells_plane;
-    const int64_t chunk_bytes = std::max<int64_t>(1, s->out_elems * (int64_t)elem + s->n_cells * 4);
+    s->n_slots = (int64_t)s->n_tiles * kWave;
+    const int64_t chunk_bytes =
+        std::max<int64_t>(1, s->n_slots * (d->n_bins * (int64_t)elem + (int64_t)sizeof(float)));
     const int64_t max_chunks = std::max<int64_t>(1, (int64_t(16) << 30) / chunk_bytes);
     int64_t group = d->n_chunks > 0 ? (d->n_sp + d->n_chunks - 1) / d->n_chunks : kAutoChunkPoints;
     group = std::max<int64_t>({group, 1, (d->n_sp + max_chunks - 1) / max_chunks});
@@ -464,8 +467,8 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
         (rc = dev_alloc(&s->hp.status, (size_t)s->n_hp + 1, &t)) ||
         (rc = dev_alloc(reinterpret_cast<unsigned char**>(&s->d_out), (size_t)s->out_elems * elem + 1, &t)) ||
         (rc = dev_alloc(reinterpret_cast<unsigned char**>(&s->d_part_sum),
-                        (size_t)s->out_elems * elem * s->n_chunks + 1, &t)) ||
-        (rc = dev_alloc(&s->d_part_cnt, (size_t)s->n_cells * s->n_chunks + 1, &t))) {
+                        (size_t)s->n_slots * s->n_bins * elem * s->n_chunks + 1, &t)) ||
+        (rc = dev_alloc(&s->d_part_cnt, (size_t)s->n_slots * s->n_chunks + 1, &t))) {
         delete s;
         return rc;
     }
@@ -473,9 +476,9 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
     // cells no lane owns (i1 == i2 on same-ring planes) are never written: zero once
     hipError_t e = hipMemsetAsync(s->d_out, 0, (size_t)s->out_elems * elem, st);
     if (e == hipSuccess)
-        e = hipMemsetAsync(s->d_part_sum, 0, (size_t)s->out_elems * elem * s->n_chunks + 1, st);
+        e = hipMemsetAsync(s->d_part_sum, 0, (size_t)s->n_slots * s->n_bins * elem * s->n_chunks + 1, st);
     if (e == hipSuccess)
-        e = hipMemsetAsync(s->d_part_cnt, 0, ((size_t)s->n_cells * s->n_chunks + 1) * sizeof(float), st);
+        e = hipMemsetAsync(s->d_part_cnt, 0, ((size_t)s->n_slots * s->n_chunks + 1) * sizeof(float), st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(s->d_mu, d->mu, (size_t)s->n_vox * sizeof(double), hipMemcpyHostToDevice, st);
     if (e == hipSuccess && d->n_sp > 0)
@@ -625,6 +628,8 @@ static ScatterArgs scatter_args(const sss_subject* s)
     a.kn_gamma = (511.0e3 * 1.6e-19) / (9.10938356e-31 * (299792458.0 * 299792458.0));
     a.part_sum = s->d_part_sum;
     a.part_cnt = s->d_part_cnt;
+    a.chunk_slice = s->d_chunk_slice;
+    a.out = s->d_out;
     return a;
 }
 
@@ -641,8 +646,7 @@ int sss_subject_scatter(sss_subject* s)
     SSS_HIP(launch_scatter(a, fp64, s->trivial, st));
     SSS_HIP(hipEventRecord(s->ev_scatter[1], st));
     s->ev_recorded = true;
-    SSS_HIP(launch_chunk_reduce_typed(s->d_part_sum, s->d_part_cnt, s->d_chunk_slice, s->n_chunks,
-                                      s->out_elems, s->n_cells, s->d_out, fp64, st));
+    SSS_HIP(launch_chunk_reduce(a, fp64, st));
     return SSS_OK;
 }
 

@@ -4,10 +4,11 @@
 // (r1, i1) x columns (r2, i2), each a block of up to 4 rings x 2 detectors,
 // for one chunk of scatter points (a few consecutive points of one
 // normalisation slice); the (tile, chunk) items are listed per wave in an
-// XCD-aware order built on the host.  Each lane accumulates its own cell over the scatter
-// points of a slice, then adds sum/count of the slice (the per-slice
-// normalisation of :555-557) into the chunk's output.  No global atomics and
-// a fixed reduction order: results do not depend on what else runs.
+// XCD-aware order built on the host.  Each lane accumulates its own cell
+// over the chunk's scatter points and writes the chunk's sum and pair count;
+// k_chunk_reduce then applies the per-slice normalisation (:555-557) and
+// adds the slices.  No global atomics and a fixed reduction order: results
+// do not depend on what else runs.
 //
 // Every lane walks the slice's scatter points with its own cursor and skips
 // the pairs the reference filters out (attenuation 0, no activity on either
@@ -29,8 +30,8 @@
 // evaluated analytically).  Along a half path the nearest bin changes only
 // every few tens of samples, so the weights collect in a register window
 // that is added into the lane's LDS column (one row per bin, padded by h on
-// both sides) when the nearest bin changes.  The column holds the slice's
-// sums; at the slice end it is divided by the slice's pair count.
+// both sides) when the nearest bin changes.  The column holds the chunk's
+// sums.
 #include "sss_internal.h"
 
 namespace sss {
@@ -331,9 +332,8 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
     }
 
     const LaneCell lc = lane_cell(a, a.tiles[tile], lane);
-    const int r1 = lc.r1, r2 = lc.r2, i1 = lc.i1, i2 = lc.i2, d1 = lc.d1, d2 = lc.d2;
+    const int d1 = lc.d1, d2 = lc.d2;
     const bool valid = lc.valid;
-    const int ndu = a.ndu, nru = a.nru;
     double lor2;
     {
         const double dx = a.detpos[3 * d1 + 0] - a.detpos[3 * d2 + 0];
@@ -343,11 +343,6 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
     }
     const double eff_ref = a.eff[a.eff_ref_row];
     const TofConst<R> tc = tof_const<R>(a);
-
-    const int plane = r2 + r1 * nru;
-    const int64_t cell = (int64_t)i1 * ndu - (int64_t)i1 * (i1 - 1) / 2 + (i2 - i1);
-    const int64_t n_cells = (int64_t)nru * nru * a.n_cells_plane;   // cells of all planes
-    const int64_t pc = (int64_t)plane * a.n_cells_plane + cell;
 
     R num = (R)0;   // one-bin path
     float cnt = 0.0f;
@@ -375,17 +370,19 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
             }
         }
     }
-    // the chunk's sums and pair count; window reach rows (bins beyond the
-    // ends) are dropped
+    // the chunk's sums and pair count into the lane's slot (the reduce maps
+    // slots to cells); window reach rows (bins beyond the ends) are dropped
     if constexpr (!TRIVIAL) win.spill(col);
     if (!valid) return;
-    R* dst = reinterpret_cast<R*>(a.part_sum) + (int64_t)chunk * B * n_cells + pc;
+    const int64_t n_slots = (int64_t)a.n_tiles * kWave;
+    const int64_t slot = (int64_t)tile * kWave + lane;
+    R* dst = reinterpret_cast<R*>(a.part_sum) + (int64_t)chunk * B * n_slots + slot;
     if constexpr (TRIVIAL) {
         dst[0] = num;
     } else {
-        for (int b = 0; b < B; ++b) dst[b * n_cells] = col[(b + HM) * kWave];
+        for (int b = 0; b < B; ++b) dst[b * n_slots] = col[(b + HM) * kWave];
     }
-    a.part_cnt[(int64_t)chunk * n_cells + pc] = cnt;
+    a.part_cnt[(int64_t)chunk * n_slots + slot] = cnt;
 }
 
 // Counts the (scatter point, LOR) pairs that pass the reference's filters and
@@ -420,64 +417,80 @@ __global__ __launch_bounds__(256) void k_pair_stats(ScatterArgs a, unsigned long
     atomicAdd(out + 1, samples);
 }
 
-// Per cell (all bins in one thread, so the count is read once per chunk):
+// One thread per lane slot (all bins, so the count is read once per chunk):
 // sum the chunks of each slice, divide by the slice's pair count (0/0 -> 0,
-// SingleScatterSimulationTOF.py:201,555-557) and add the slices in order.
+// SingleScatterSimulationTOF.py:201,555-557), add the slices in order
+// (compensated in FP32) and write the slot's cell of the compact output.
 // Chunks of one slice are consecutive.
-template <typename R, int BMAX>
-__global__ void k_chunk_reduce(const R* __restrict__ part_sum, const float* __restrict__ part_cnt,
-                               const int32_t* __restrict__ chunk_slice, int n_chunks, int n_bins,
-                               int64_t n_cells, R* out)
+template <typename R>
+__device__ __forceinline__ void kahan_add(R& s, R& c, R v)
 {
-    const int64_t pc = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (pc >= n_cells) return;
-    const int64_t n = (int64_t)n_bins * n_cells;
-    R total[BMAX], num[BMAX];
+    if constexpr (sizeof(R) == 4) {
+        const R y = v - c;
+        const R t = s + y;
+        c = (t - s) - y;
+        s = t;
+    } else {
+        s += v;
+    }
+}
+
+template <typename R, int BMAX>
+__global__ void k_chunk_reduce(ScatterArgs a)
+{
+    const int64_t n_slots = (int64_t)a.n_tiles * kWave;
+    const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= n_slots) return;
+    const LaneCell lc = lane_cell(a, a.tiles[slot / kWave], (int)(slot % kWave));
+    if (!lc.valid) return;
+    const int B = a.n_bins;
+    const R* __restrict__ part_sum = reinterpret_cast<const R*>(a.part_sum);
+    R total[BMAX], comp[BMAX], num[BMAX];
 #pragma unroll
-    for (int b = 0; b < BMAX; ++b) total[b] = num[b] = (R)0;
+    for (int b = 0; b < BMAX; ++b) total[b] = comp[b] = num[b] = (R)0;
     float cnt = 0.0f;
-    int slice = n_chunks > 0 ? chunk_slice[0] : 0;
-    for (int c = 0; c < n_chunks; ++c) {
-        if (chunk_slice[c] != slice) {
+    int slice = a.n_chunks > 0 ? a.chunk_slice[0] : 0;
+    for (int c = 0; c < a.n_chunks; ++c) {
+        if (a.chunk_slice[c] != slice) {
             if (cnt > 0.0f) {
 #pragma unroll
-                for (int b = 0; b < BMAX; ++b) total[b] += num[b] / (R)cnt;
+                for (int b = 0; b < BMAX; ++b) kahan_add(total[b], comp[b], num[b] / (R)cnt);
             }
 #pragma unroll
             for (int b = 0; b < BMAX; ++b) num[b] = (R)0;
             cnt = 0.0f;
-            slice = chunk_slice[c];
+            slice = a.chunk_slice[c];
         }
-        const R* src = part_sum + (int64_t)c * n + pc;
+        const R* src = part_sum + (int64_t)c * B * n_slots + slot;
 #pragma unroll
         for (int b = 0; b < BMAX; ++b)
-            if (b < n_bins) num[b] += src[b * n_cells];
-        cnt += part_cnt[(int64_t)c * n_cells + pc];
+            if (b < B) num[b] += src[b * n_slots];
+        cnt += a.part_cnt[(int64_t)c * n_slots + slot];
     }
+    const int64_t n_cells = (int64_t)a.nru * a.nru * a.n_cells_plane;
+    const int64_t cell = (int64_t)lc.i1 * a.ndu - (int64_t)lc.i1 * (lc.i1 - 1) / 2 + (lc.i2 - lc.i1);
+    const int64_t pc = (int64_t)(lc.r2 + lc.r1 * a.nru) * a.n_cells_plane + cell;
+    R* out = reinterpret_cast<R*>(a.out);
 #pragma unroll
     for (int b = 0; b < BMAX; ++b) {
-        if (cnt > 0.0f) total[b] += num[b] / (R)cnt;
-        if (b < n_bins) out[b * n_cells + pc] = total[b];
+        if (cnt > 0.0f) kahan_add(total[b], comp[b], num[b] / (R)cnt);
+        if (b < B) out[b * n_cells + pc] = total[b];
     }
 }
 
 template <typename R>
-hipError_t launch_reduce_typed(const R* part_sum, const float* part_cnt, const int32_t* chunk_slice,
-                               int n_chunks, int n_bins, int64_t n_cells, R* out, hipStream_t st)
+hipError_t launch_reduce_typed(const ScatterArgs& a, hipStream_t st)
 {
-    const unsigned blocks = (unsigned)((n_cells + 255) / 256);
-    if (n_bins <= 4)
-        hipLaunchKernelGGL((k_chunk_reduce<R, 4>), dim3(blocks), dim3(256), 0, st, part_sum, part_cnt,
-                           chunk_slice, n_chunks, n_bins, n_cells, out);
-    else if (n_bins <= 8)
-        hipLaunchKernelGGL((k_chunk_reduce<R, 8>), dim3(blocks), dim3(256), 0, st, part_sum, part_cnt,
-                           chunk_slice, n_chunks, n_bins, n_cells, out);
-    else if (n_bins <= 16)
-        hipLaunchKernelGGL((k_chunk_reduce<R, 16>), dim3(blocks), dim3(256), 0, st, part_sum, part_cnt,
-                           chunk_slice, n_chunks, n_bins, n_cells, out);
+    const int64_t n_slots = (int64_t)a.n_tiles * kWave;
+    const unsigned blocks = (unsigned)((n_slots + 255) / 256);
+    if (a.n_bins <= 4)
+        hipLaunchKernelGGL((k_chunk_reduce<R, 4>), dim3(blocks), dim3(256), 0, st, a);
+    else if (a.n_bins <= 8)
+        hipLaunchKernelGGL((k_chunk_reduce<R, 8>), dim3(blocks), dim3(256), 0, st, a);
+    else if (a.n_bins <= 16)
+        hipLaunchKernelGGL((k_chunk_reduce<R, 16>), dim3(blocks), dim3(256), 0, st, a);
     else
-        hipLaunchKernelGGL((k_chunk_reduce<R, 32>), dim3(blocks), dim3(256), 0, st, part_sum, part_cnt,
-                           chunk_slice, n_chunks, n_bins, n_cells, out);
+        hipLaunchKernelGGL((k_chunk_reduce<R, 32>), dim3(blocks), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
@@ -563,18 +576,11 @@ hipError_t launch_pair_stats(const ScatterArgs& a, unsigned long long* out, hipS
     return hipGetLastError();
 }
 
-hipError_t launch_chunk_reduce_typed(const void* part_sum, const float* part_cnt,
-                                     const int32_t* chunk_slice, int n_chunks, int64_t n,
-                                     int64_t n_cells, void* out, bool fp64, hipStream_t st)
+hipError_t launch_chunk_reduce(const ScatterArgs& a, bool fp64, hipStream_t st)
 {
-    if (n == 0 || n_cells == 0) return hipSuccess;
-    const int n_bins = (int)(n / n_cells);
-    if (n_bins > kMaxBins) return hipErrorInvalidValue;
-    if (fp64)
-        return launch_reduce_typed<double>((const double*)part_sum, part_cnt, chunk_slice, n_chunks,
-                                           n_bins, n_cells, (double*)out, st);
-    return launch_reduce_typed<float>((const float*)part_sum, part_cnt, chunk_slice, n_chunks, n_bins,
-                                      n_cells, (float*)out, st);
+    if (a.n_tiles == 0) return hipSuccess;
+    if (a.n_bins > kMaxBins) return hipErrorInvalidValue;
+    return fp64 ? launch_reduce_typed<double>(a, st) : launch_reduce_typed<float>(a, st);
 }
 
 hipError_t launch_dense_scatter_typed(const void* compact, bool fp64, int n_bins, int nru, int ndu,
