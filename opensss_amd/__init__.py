@@ -10,6 +10,9 @@ Module map (reference file -> module):
 * ``DataReduction.py``              -> :mod:`opensss_amd.data_reduction`
 * ``Geometry.py``                   -> :mod:`opensss_amd.geometry`
 * ``Interpolation.py``              -> :mod:`opensss_amd.interpolation`
+* ``TailsMask.py``                  -> :mod:`opensss_amd.tails_mask`
+* ``BackProjection.py``             -> :mod:`opensss_amd.backprojection`
+* ``Scaling.py``                    -> :mod:`opensss_amd.scaling`
 """
 from .data_reduction import ComputeLORCoordinates, SinogramLUT, SumLORsCounts
 from .geometry import (ImportCASToRGeometry, PrepareGeometryAndSinograms, expandRings,
@@ -20,19 +23,25 @@ __all__ = [
     'ComputeLORCoordinates', 'SinogramLUT', 'SumLORsCounts', 'ImportCASToRGeometry',
     'PrepareGeometryAndSinograms', 'expandRings', 'mashDetectors', 'SinogramCoordinates',
     'SingleScatterSimulationTOF', 'ScatterSubject', 'RayTracing3DTOF', 'RayTracing3DTOFBatch',
-    'LineIntegrals', 'InterpolateScatters',
+    'LineIntegrals', 'InterpolateScatters', 'GenerateTailsMask', 'MaskGenerator',
+    'MaskGeneratorMashing', 'BackProjectionScatterRandFit', 'BackProjectionScatterRandFitMashing',
+    'ScaleScattersToPrompts',
 ]
+
+_LAZY = {
+    'SingleScatterSimulationTOF': 'sss', 'ScatterSubject': 'sss',
+    'RayTracing3DTOF': 'raytracing', 'RayTracing3DTOFBatch': 'raytracing', 'LineIntegrals': 'raytracing',
+    'InterpolateScatters': 'interpolation', 'Interpolator': 'interpolation',
+    'GenerateTailsMask': 'tails_mask', 'MaskGenerator': 'tails_mask', 'MaskGeneratorMashing': 'tails_mask',
+    'BackProjectionScatterRandFit': 'backprojection',
+    'BackProjectionScatterRandFitMashing': 'backprojection',
+    'ScaleScattersToPrompts': 'scaling',
+}
 
 
 def __getattr__(name):
     # GPU entry points load libsss_hip.so on first use (raises if it is missing)
-    if name in ('SingleScatterSimulationTOF', 'ScatterSubject'):
-        from . import sss
-        return getattr(sss, name)
-    if name in ('RayTracing3DTOF', 'RayTracing3DTOFBatch', 'LineIntegrals'):
-        from . import raytracing
-        return getattr(raytracing, name)
-    if name in ('InterpolateScatters', 'Interpolator'):
-        from . import interpolation
-        return getattr(interpolation, name)
+    if name in _LAZY:
+        import importlib
+        return getattr(importlib.import_module(f'.{_LAZY[name]}', __name__), name)
     raise AttributeError(name)

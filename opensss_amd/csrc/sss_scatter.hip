@@ -435,16 +435,23 @@ __device__ __forceinline__ void kahan_add(R& s, R& c, R v)
     }
 }
 
-template <typename R, int BMAX>
+// Bins per thread of the chunk reduce; blockIdx.y picks the group.  Several
+// groups per slot keep enough waves in flight for the chunk loop's loads.
+constexpr int kReduceBins = 4;
+
+template <typename R>
 __global__ void k_chunk_reduce(ScatterArgs a)
 {
+    constexpr int BMAX = kReduceBins;
     const int64_t n_slots = (int64_t)a.n_tiles * kWave;
     const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= n_slots) return;
     const LaneCell lc = lane_cell(a, a.tiles[slot / kWave], (int)(slot % kWave));
     if (!lc.valid) return;
+    const int b0 = blockIdx.y * kReduceBins;
     const int B = a.n_bins;
-    const R* __restrict__ part_sum = reinterpret_cast<const R*>(a.part_sum);
+    const int nb = min(kReduceBins, B - b0);
+    const R* __restrict__ part_sum = reinterpret_cast<const R*>(a.part_sum) + (int64_t)b0 * n_slots;
     R total[BMAX], comp[BMAX], num[BMAX];
 #pragma unroll
     for (int b = 0; b < BMAX; ++b) total[b] = comp[b] = num[b] = (R)0;
@@ -464,17 +471,17 @@ __global__ void k_chunk_reduce(ScatterArgs a)
         const R* src = part_sum + (int64_t)c * B * n_slots + slot;
 #pragma unroll
         for (int b = 0; b < BMAX; ++b)
-            if (b < B) num[b] += src[b * n_slots];
+            if (b < nb) num[b] += src[b * n_slots];
         cnt += a.part_cnt[(int64_t)c * n_slots + slot];
     }
     const int64_t n_cells = (int64_t)a.nru * a.nru * a.n_cells_plane;
     const int64_t cell = (int64_t)lc.i1 * a.ndu - (int64_t)lc.i1 * (lc.i1 - 1) / 2 + (lc.i2 - lc.i1);
     const int64_t pc = (int64_t)(lc.r2 + lc.r1 * a.nru) * a.n_cells_plane + cell;
-    R* out = reinterpret_cast<R*>(a.out);
+    R* out = reinterpret_cast<R*>(a.out) + (int64_t)b0 * n_cells;
 #pragma unroll
     for (int b = 0; b < BMAX; ++b) {
         if (cnt > 0.0f) kahan_add(total[b], comp[b], num[b] / (R)cnt);
-        if (b < B) out[b * n_cells + pc] = total[b];
+        if (b < nb) out[b * n_cells + pc] = total[b];
     }
 }
 
@@ -482,15 +489,8 @@ template <typename R>
 hipError_t launch_reduce_typed(const ScatterArgs& a, hipStream_t st)
 {
     const int64_t n_slots = (int64_t)a.n_tiles * kWave;
-    const unsigned blocks = (unsigned)((n_slots + 255) / 256);
-    if (a.n_bins <= 4)
-        hipLaunchKernelGGL((k_chunk_reduce<R, 4>), dim3(blocks), dim3(256), 0, st, a);
-    else if (a.n_bins <= 8)
-        hipLaunchKernelGGL((k_chunk_reduce<R, 8>), dim3(blocks), dim3(256), 0, st, a);
-    else if (a.n_bins <= 16)
-        hipLaunchKernelGGL((k_chunk_reduce<R, 16>), dim3(blocks), dim3(256), 0, st, a);
-    else
-        hipLaunchKernelGGL((k_chunk_reduce<R, 32>), dim3(blocks), dim3(256), 0, st, a);
+    const dim3 grid((unsigned)((n_slots + 255) / 256), (unsigned)((a.n_bins + kReduceBins - 1) / kReduceBins));
+    hipLaunchKernelGGL(k_chunk_reduce<R>, grid, dim3(256), 0, st, a);
     return hipGetLastError();
 }
 

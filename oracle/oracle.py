@@ -267,3 +267,169 @@ def sinogram_coordinates_loops(n_sectors_trans, n_sectors_axial, n_modules_axial
                 idx = (2 * d - 1) * NR - d * (d - 1) + (NR - d) + r1
             sino[r1, r2] = idx
     return lor.astype(np.int16), sino.astype(np.int16), shift
+
+
+# ---------------------------------------------------------------------------
+# Sinogram stages after the scatter estimate (small cases only: plain loops)
+# ---------------------------------------------------------------------------
+
+def _nearest_fill(F):
+    """fill_with_interp1d (Interpolation.py:133-164) for a nearest-neighbour
+    griddata result: no NaN can occur, so only the clamp at 0 applies."""
+    F = F.copy()
+    F[F < 0] = 0
+    return F
+
+
+def interpolate_bins(Scatters, rings, dets, SinogramCoordinates, SinogramIndex=None,
+                     MashedSinogramIndices=None):
+    """Restatement of InterpolateAllBins / InterpolateAllBinsSpan
+    (Interpolation.py:197-319, :321-399), one bin at a time:
+    griddata(nearest) of each sampled plane over the (radial, angular) grid,
+    clamp at 0, then RegularGridInterpolator(nearest) over the sampled rings
+    evaluated at every output ring pair (without span) or at the averaged
+    rings of every span plane.  Returns (B, N+1, N//2, P) float32."""
+    from scipy.interpolate import RegularGridInterpolator, griddata
+    S = np.asarray(Scatters, dtype=np.float32)
+    B, NR1, NA, NP = S.shape
+    nru = len(rings)
+    lor = np.asarray(SinogramCoordinates)
+    gx, gy = np.mgrid[0:NR1, 0:NA]
+    out = []
+    for b in range(B):
+        cur = S[b].copy()
+        for i in range(NP):
+            rad = lor[np.ix_(dets, dets, [1])][:, :, 0]
+            ang = lor[np.ix_(dets, dets, [0])][:, :, 0]
+            pts = np.array([(y, x) for y, x in zip(rad.flatten(), ang.flatten())])
+            vals = cur[rad.astype(int), ang.astype(int), i].flatten()
+            cur[:, :, i] = _nearest_fill(griddata(pts, vals, (gx, gy), method='nearest'))
+        cur = np.transpose(cur.reshape(NR1, NA, nru, nru), (0, 1, 3, 2))
+        rgi = RegularGridInterpolator((np.arange(NR1), np.arange(NA), np.asarray(rings, float),
+                                       np.asarray(rings, float)), cur, method='nearest',
+                                      bounds_error=False, fill_value=0)
+        if MashedSinogramIndices is None:
+            NR = np.asarray(SinogramIndex).shape[0]
+            xi = np.meshgrid(np.arange(NR1), np.arange(NA), np.arange(NR), np.arange(NR), indexing='ij')
+            full = rgi(np.stack([x.flatten() for x in xi], axis=-1)).reshape(NR1, NA, NR * NR)
+            order = np.argsort(np.asarray(SinogramIndex)[:NR, :NR].T.flatten())
+            out.append(full[:, :, order].astype(np.float32))
+        else:
+            msi = np.asarray(MashedSinogramIndices, dtype=np.float64)
+            planes = []
+            for i in range(msi.shape[0]):
+                xi = np.meshgrid(np.arange(NR1), np.arange(NA), [msi[i, 0]], [msi[i, 1]], indexing='ij')
+                planes.append(rgi(np.stack([x.flatten() for x in xi], axis=-1)).reshape(NR1, NA))
+            out.append(np.stack(planes, axis=-1).astype(np.float32))
+    return np.stack(out)
+
+
+def _lor_segment(geom, ra, d1, rb, d2):
+    """Segment of a trace loop: detector x, y of its ring, ring z from detector 0."""
+    return [geom[ra, d1, 0], geom[ra, d1, 1], geom[ra, 0, 2],
+            geom[rb, d2, 0], geom[rb, d2, 1], geom[rb, 0, 2]]
+
+
+def tail_mask(ActivityMap, AttenuationMap, ImageSize, Geometry, SinogramCoordinates, UseAttenuation,
+              AccelerationFactor, SinogramIndex=None, MashedSinogramIndices=None):
+    """Restatement of MaskGenerator (TailsMask.py:84-228) and
+    MaskGeneratorMashing (:230-376); traces through ora_line_integrals."""
+    geom = np.asarray(Geometry, dtype=np.float64)
+    NR, N = geom.shape[0], geom.shape[1]
+    lor = np.asarray(SinogramCoordinates)
+    grid = np.asarray(np.asarray(ActivityMap).shape)
+    mu = np.asarray(AttenuationMap, dtype=np.float64).flatten(order='F')
+    lam = np.asarray(ActivityMap, dtype=np.float64).flatten(order='F')
+    af = AccelerationFactor or 1
+
+    def masks(pairs):
+        """pairs: list of (Ring1, Ring2); returns uint8 (N+1, N//2, len(pairs))."""
+        segs, where = [], []
+        for p, (r1, r2) in enumerate(pairs):
+            for d1 in range(0, N, af):
+                for d2 in range(0, N, af):
+                    if (d1 == d2 and r2 == r1) or d1 > d2:
+                        continue
+                    segs.append(_lor_segment(geom, r1, d1, r2, d2))
+                    where.append((lor[d1, d2, 1], lor[d1, d2, 0], p))
+        out = np.zeros((N + 1, N // 2, len(pairs)), dtype=np.uint8)
+        if not segs:
+            return out
+        n, m, l = line_integrals(grid, ImageSize, mu, lam, np.array(segs))
+        for k, (rad, ang, p) in enumerate(where):
+            if n[k] != 0:
+                act = l[k]
+                att = 1 / np.exp(-m[k])
+            else:
+                act, att = 0, 1
+            att = 0 if att != 1 else 1
+            act = 0 if act > 0 else 1
+            out[rad, ang, p] += np.uint8(att if UseAttenuation else act)
+        return out
+
+    if MashedSinogramIndices is None:
+        # Sinograms[:, :, Ring2, Ring1], reshaped and reordered (:209-219)
+        sino = np.zeros((N + 1, N // 2, NR, NR), dtype=np.uint8)
+        for r1 in range(NR):
+            sino[:, :, :, r1] = masks([(r1, r2) for r2 in range(NR)])
+        order = np.argsort(np.asarray(SinogramIndex)[:NR, :NR].T.flatten())
+        return sino.reshape(N + 1, N // 2, NR * NR)[:, :, order]
+    msi = np.asarray(MashedSinogramIndices)
+    return masks([(int(msi[i][0] * 2), int(msi[i][1] * 2)) for i in range(msi.shape[0])])
+
+
+def tail_fit(PromptSinogram, ScatterSinogram, RandomSinogram, MaskSinogram, GridSize, ImageSize, Geometry,
+             SinogramCoordinates, AccelerationFactor, SinogramOrder=None, MashedSinogramIndices=None):
+    """Restatement of BackProjectionScatterRandFit (BackProjection.py:31-214)
+    and ...Mashing (:217-392): float32 images per worker (per Ring1, or per
+    span plane) summed in float32, float32 ratios, float32 lstsq scaled by
+    1e10.  Returns (scale float32, images dict)."""
+    geom = np.asarray(Geometry, dtype=np.float64)
+    NR, N = geom.shape[0], geom.shape[1]
+    lor = np.asarray(SinogramCoordinates)
+    gs = np.asarray(GridSize, dtype=np.int32)
+    af = AccelerationFactor or 1
+    S = np.asarray(MaskSinogram, np.float32) * np.asarray(ScatterSinogram, np.float32)
+    R = np.asarray(MaskSinogram, np.float32) * np.asarray(RandomSinogram, np.float32)
+    P = np.asarray(MaskSinogram, np.float32) * np.float32(PromptSinogram)
+    nvox = int(np.prod(gs))
+    acc = [np.zeros(nvox, np.float32) for _ in range(4)]
+
+    def worker(r1, r2_list, plane_of, mode):
+        segs, vp, vs, vr = [], [], [], []
+        for d1 in range(0, N, af):
+            for r2 in r2_list:
+                for d2 in range(0, N, af):
+                    if (d1 == d2 and r2 == r1) or d1 > d2:
+                        continue
+                    rad, ang = lor[d1, d2, 1], lor[d1, d2, 0]
+                    pl = plane_of(r2)
+                    segs.append(_lor_segment(geom, r1, d1, r2, d2))
+                    vp.append(P[rad, ang, pl]); vs.append(S[rad, ang, pl]); vr.append(R[rad, ang, pl])
+        if segs:
+            imgs = backproject(gs, ImageSize, np.array(segs), np.array(vp), np.array(vs), np.array(vr), mode)
+            for a_, b_ in zip(acc, imgs):
+                a_ += b_
+
+    if MashedSinogramIndices is None:
+        order = np.asarray(SinogramOrder).reshape(NR, NR)
+        for r1 in range(NR):
+            worker(r1, range(NR), lambda r2, r1=r1: order[r1, r2], 0)
+    else:
+        msi = np.asarray(MashedSinogramIndices)
+        for i in range(msi.shape[0]):
+            worker(int(msi[i, 0] * 2), [int(msi[i, 1] * 2)], lambda r2, i=i: i, 1)
+    prompt_img, scatter_img, random_img, sens = acc
+    eps = 0 if MashedSinogramIndices is None else 1e-15
+    with np.errstate(divide='ignore', invalid='ignore'):
+        sbp = scatter_img / (sens + np.float32(eps))
+        abp = prompt_img / (sens + np.float32(eps))
+        rbp = random_img / (sens + np.float32(eps))
+    for x in (sbp, abp, rbp):
+        x[np.isnan(x)] = 0
+        x[np.isinf(x)] = 0
+    tails = abp - rbp
+    tails[tails < 0] = 0
+    a, _, _, _ = np.linalg.lstsq(sbp[:, np.newaxis] * np.float32(1e10), tails, rcond=None)
+    scale = a * np.float32(1e10)
+    return scale, {'prompt': prompt_img, 'scatter': scatter_img, 'random': random_img, 'sens': sens}
