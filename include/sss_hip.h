@@ -88,6 +88,10 @@ typedef struct {
     int64_t device_bytes;        /* resident device memory of the subject */
     int32_t n_tiles, n_chunks;
     int32_t n_truncated_walks;   /* half paths whose walk stopped early (voxel-face runs) */
+    int32_t tof_fold;            /* TOF weights of the scatter launch: 0 one exp2 per window bin,
+                                    1 one exp2 (h = 1, end bins masked), 2 one exp2 with the
+                                    window centre inside the bins, 3 as 2 with a window that
+                                    slides along each half path (FP32) */
 } sss_subject_info;
 
 const char* sss_last_error(void);

@@ -59,7 +59,7 @@ class SubjectInfo(ctypes.Structure):
         ('n_cells', ctypes.c_int64), ('n_pairs_enumerated', ctypes.c_int64),
         ('evals', ctypes.c_int64), ('device_bytes', ctypes.c_int64),
         ('n_tiles', ctypes.c_int32), ('n_chunks', ctypes.c_int32),
-        ('n_truncated_walks', ctypes.c_int32),
+        ('n_truncated_walks', ctypes.c_int32), ('tof_fold', ctypes.c_int32),
     ]
 
 

@@ -25,6 +25,15 @@ constexpr int kTileMaxRings = 4;
 constexpr int kScatterWaves = 4;          // waves per workgroup of the scatter launch
 constexpr int kAutoChunkPoints = 8;       // scatter points per chunk by default
 constexpr int kMaxBins = 32;
+
+// TOF weight evaluation of the fold (see tof_weights in sss_scatter.hip)
+constexpr int kTofGeneral = 0;            // one exp2 per window bin
+constexpr int kTofOneExp = 1;             // h = 1: one exp2, end bins masked
+constexpr int kTofOneExpInner = 2;        // h = 1: one exp2, window centre kept inside the bins
+// float only: kTofOneExpInner with a window that slides along a half path
+// (the centre bin moves one bin at a time, in one direction)
+constexpr int kTofSlide = 3;              // odd bin count (bin centres at integer multiples of W)
+constexpr int kTofSlideHalf = 4;          // even bin count (centres at half-integer multiples)
 constexpr int kMaxHalfWindow = 8;
 
 void set_error(const std::string& msg);
@@ -61,7 +70,7 @@ struct HalfPathTable {
 // sample in a thousand (the fractions of (d1 - d2)/c and 2 s_k/c are often
 // nearly complementary), so the fraction needs far more than 20 bits.
 constexpr double kFixOne = 4294967296.0;   // 2^32
-constexpr int kSamplePad = 4;              // records after the last one (fold prefetch reads 3 ahead)
+constexpr int kSamplePad = 8;              // records after the last one (fold prefetch reads up to 7 ahead)
 
 // One voxel sample of a half path, read by the fold with a single 16-byte load.
 struct alignas(16) SampleRec {
@@ -107,10 +116,14 @@ struct ScatterArgs {
     double inv_w_bins, mid, bin_w, kappa2, beta2;  // log2 scaled kappa, beta
     double c_beta;                                 // 2^-beta2
     int half_window, t_lo, t_hi;
-    int single_exp;                                // h = 1 and beta2 small: one exp2 per sample
+    int tof_mode;                                  // kTofGeneral / kTofOneExp / kTofOneExpInner
     int range_check;                               // offsets may leave [t_lo, t_hi]: test each sample
     // the same constants rounded to float once on the host (uniform registers)
     float f_inv_w, f_kap, f_mid, f_jmax, f_wk, f_mid_wk, f_c;
+    // sliding window: centred bin jr = bin - floor(mid) kept in [f_jlo, f_jhi];
+    // LDS row of bin jr - 1 is jr + slide_row0
+    float f_jlo, f_jhi;
+    int slide_row0;
     double g_const;              // (Dx Dy 1e-2)^2 / (4 pi)
     double kn_gamma;
     // output: per chunk, the un-normalised sums and the pair counts of every

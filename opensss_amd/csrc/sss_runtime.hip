@@ -145,6 +145,7 @@ struct sss_subject {
     bool trivial = false;
     int half_window = 0, t_lo = 0, t_hi = 0;
     bool range_check = true;          // TOF offsets may leave the non-zero table columns
+    double tof_bound = 0;             // largest |TOF offset| any sample can reach, ps
     double bin_w = 0, kappa = 0, beta = 0, g_const = 0;
     int64_t n_vox = 0, n_hp = 0, n_samples = 0, samples_cap = 0, n_cells_plane = 0, out_elems = 0;
     int64_t n_cells = 0;                  // cells of all planes
@@ -405,6 +406,7 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
     // the per-sample window test can go when every offset provably stays
     // inside a table whose columns are all non-zero
     const double tof_bound = tof_offset_bound(g, d->sp_xyz, d->n_sp, detpos.data(), d->nru * d->ndu);
+    s->tof_bound = tof_bound;
     s->range_check = !(s->t_lo == -kTofHalf && s->t_hi == kTofHalf && tof_bound < kTofHalf - 2);
     // wave tiles: rows and columns are blocks of rb rings x db detectors
     // (rb * db = 8, rb the largest power of two <= min(4, NRU)); detector
@@ -614,8 +616,19 @@ static ScatterArgs scatter_args(const sss_subject* s)
     a.beta2 = s->beta * log2e;
     a.c_beta = std::exp2(-a.beta2);
     a.half_window = s->half_window; a.t_lo = s->t_lo; a.t_hi = s->t_hi;
-    // one exp2 per sample needs E^2 = 2^(2 ku) finite inside the bins: ku <= beta2
-    a.single_exp = (s->half_window == 1 && a.beta2 <= 60.0) ? 1 : 0;
+    // one exp2 per sample needs c E^2 = 2^(2 ku - beta2) finite; inside the
+    // bins ku <= beta2.  With the window centre kept in [1, B-2], ku reaches
+    // kappa2 (|t| - s_(B-2)) at the largest offset |t| of the subject.
+    a.tof_mode = kTofGeneral;
+    if (s->half_window == 1 && a.beta2 <= 60.0) {
+        a.tof_mode = kTofOneExp;
+        if (s->n_bins >= 3 && !s->range_check) {
+            const double s_inner = (s->n_bins - 2 - a.mid) * s->bin_w;
+            const double umax = std::max(0.5 * s->bin_w, s->tof_bound - s_inner);
+            if (2.0 * a.kappa2 * umax - a.beta2 < 120.0 && a.kappa2 * umax < 120.0)
+                a.tof_mode = kTofOneExpInner;
+        }
+    }
     a.range_check = s->range_check ? 1 : 0;
     a.f_inv_w = (float)a.inv_w_bins;
     a.f_kap = (float)a.kappa2;
@@ -624,6 +637,13 @@ static ScatterArgs scatter_args(const sss_subject* s)
     a.f_wk = (float)(a.bin_w * a.kappa2);
     a.f_mid_wk = (float)(a.mid * a.bin_w * a.kappa2);
     a.f_c = (float)a.c_beta;
+    {
+        // centred bins jr = bin - floor(mid); the window centre stays in bins [1, B-2]
+        const int m = (s->n_bins - 1) / 2;
+        a.f_jlo = (float)(1 - m);
+        a.f_jhi = (float)(s->n_bins - 2 - m);
+        a.slide_row0 = m + s->half_window - 1;
+    }
     a.g_const = s->g_const;
     a.kn_gamma = (511.0e3 * 1.6e-19) / (9.10938356e-31 * (299792458.0 * 299792458.0));
     a.part_sum = s->d_part_sum;
@@ -709,6 +729,9 @@ int sss_subject_info_get(sss_subject* s, sss_subject_info* out)
         SSS_HIP(hipStreamSynchronize(s->ctx->stream));
     }
     out->n_truncated_walks = trunc;
+    const ScatterArgs a = scatter_args(s);
+    out->tof_fold = (s->trivial || a.half_window == 0) ? kTofGeneral : a.tof_mode;
+    if (out->tof_fold == kTofOneExpInner && s->precision != SSS_PRECISION_FP64) out->tof_fold = kTofSlide;
     return SSS_OK;
 }
 

@@ -173,15 +173,35 @@ __device__ __forceinline__ TofConst<R> tof_const(const ScatterArgs& a)
 
 // TOF weights of one voxel sample at offset tf ps:
 // w[H + j] = wsample * T[j0 + j, t] for |j| <= h, zero outside the bins or
-// the window; returns the nearest bin j0 (as R).
-template <typename R, int H, bool SINGLE_EXP>
+// the window; returns the window's centre bin j0 (as R).
+//
+// TM (TOF mode): kTofGeneral -- exp2 per window bin; kTofOneExp -- h = 1,
+// one exp2, centre = nearest bin, edge bins masked; kTofOneExpInner -- h = 1,
+// one exp2, centre clamped to [1, B-2] so all three window bins exist (no
+// masks): when the nearest bin is an end bin the window still holds it and
+// its neighbour, and a dropped bin is >= 2 bins from the nearest one, so the
+// truncation stays within the window tolerance; the host enables it only
+// when 2^(2 ku - beta2) stays finite for every offset of the subject.
+template <typename R, int H, int TM>
 __device__ __forceinline__ R tof_weights(const ScatterArgs& a, const TofConst<R>& d, R hf, R wsample,
                                          R (&w)[2 * H + 1])
 {
+    if constexpr (TM == kTofOneExpInner) {
+        const R jf = clampr(rint(hf * d.inv_w + d.mid), (R)1, d.jmax - (R)1);
+        const R ku = hf * d.kap + (d.mid_wk - jf * d.wk);
+        const R E = fast_exp2(ku);
+        const R cE = d.c * E;
+        const R D = cE * E + (E + d.c);
+        const R g = wsample * fast_rcp(D);
+        w[1] = g * E;
+        w[2] = w[1] * cE;
+        w[0] = g * d.c;
+        return jf;
+    }
     const R jf = clampr(rint(hf * d.inv_w + d.mid), (R)0, d.jmax);
     // ku = kappa2 (t - (j0 - mid) W)
     R ku = hf * d.kap + (d.mid_wk - jf * d.wk);
-    if constexpr (SINGLE_EXP) {
+    if constexpr (TM == kTofOneExp) {
         // h = 1: e+ = c E, e- = c / E with E = 2^ku, c = 2^-beta2; with
         // D = E (1 + e+ + e-) the weights are E/D, c E^2/D and c/D.
         ku = clampr(ku, (R)-100, (R)100);
@@ -249,7 +269,7 @@ struct BinWindow {
 };
 
 // One voxel sample of a half path (see fold_path).
-template <typename R, int H, bool SINGLE_EXP, bool RANGE, int SIGN>
+template <typename R, int H, int TM, bool RANGE, int SIGN>
 __device__ __forceinline__ void fold_sample(const ScatterArgs& a, const TofConst<R>& d, R* __restrict__ col,
                                             BinWindow<R, H>& win, int64_t x, int64_t nq, R av, R coef)
 {
@@ -267,7 +287,7 @@ __device__ __forceinline__ void fold_sample(const ScatterArgs& a, const TofConst
         tf = SIGN < 0 ? (R)hi : -(R)hi;
     }
     R w[2 * H + 1];
-    const R j0 = tof_weights<R, H, SINGLE_EXP>(a, d, tf, ws, w);
+    const R j0 = tof_weights<R, H, TM>(a, d, tf, ws, w);
     win.add(col, j0, w);
 }
 
@@ -278,7 +298,7 @@ __device__ __forceinline__ void fold_sample(const ScatterArgs& a, const TofConst
 // the loads run up to three records past the path (the table has spare
 // records after its end).  With RANGE, offsets outside the non-zero TOF
 // columns [t_lo, t_hi] weigh 0.
-template <typename R, int H, bool SINGLE_EXP, bool RANGE, int SIGN>
+template <typename R, int H, int TM, bool RANGE, int SIGN>
 __device__ __forceinline__ void fold_path(const ScatterArgs& a, const TofConst<R>& d, R* __restrict__ col,
                                           BinWindow<R, H>& win, int64_t base, int n, int64_t shift,
                                           R coef)
@@ -292,28 +312,126 @@ __device__ __forceinline__ void fold_path(const ScatterArgs& a, const TofConst<R
         const double* __restrict__ ap = a.s_a64 + base;
         double aa = ap[0], ab = ap[1];
         for (; p < end; p += 2, ap += 2) {
-            fold_sample<R, H, SINGLE_EXP, RANGE, SIGN>(a, d, col, win, x, ra.nq, aa, coef);
+            fold_sample<R, H, TM, RANGE, SIGN>(a, d, col, win, x, ra.nq, aa, coef);
             ra = p[2];
             aa = ap[2];
-            if (p + 1 < end) fold_sample<R, H, SINGLE_EXP, RANGE, SIGN>(a, d, col, win, x, rb.nq, ab, coef);
+            if (p + 1 < end) fold_sample<R, H, TM, RANGE, SIGN>(a, d, col, win, x, rb.nq, ab, coef);
             rb = p[3];
             ab = ap[3];
         }
     } else {
         // both loads unconditional, so the wait before a sample covers only its own record
         for (; p < end; p += 2) {
-            fold_sample<R, H, SINGLE_EXP, RANGE, SIGN>(a, d, col, win, x, ra.nq, ra.a, coef);
+            fold_sample<R, H, TM, RANGE, SIGN>(a, d, col, win, x, ra.nq, ra.a, coef);
             ra = p[2];
-            if (p + 1 < end) fold_sample<R, H, SINGLE_EXP, RANGE, SIGN>(a, d, col, win, x, rb.nq, rb.a, coef);
+            if (p + 1 < end) fold_sample<R, H, TM, RANGE, SIGN>(a, d, col, win, x, rb.nq, rb.a, coef);
             rb = p[3];
         }
     }
 }
 
+// Sliding window of the float h = 1 fold (kTofSlide, kTofSlideHalf): the
+// sums of bins cur - 1, cur, cur + 1 of the current half path, without the
+// path's coefficient (applied when a bin leaves the window).  wp: LDS row of
+// bin cur - 1 in the lane's column.
+struct SlideWindow {
+    float lo, mid, hi;
+    float cur;     // centred centre bin jr (bin jr + floor((B-1)/2))
+    float* wp;
+};
+
+// Adds the window, times coef, into its three LDS rows and clears it.
+__device__ __forceinline__ void slide_flush(SlideWindow& win, float coef)
+{
+    lds_add(win.wp, win.lo * coef);
+    lds_add(win.wp + kWave, win.mid * coef);
+    lds_add(win.wp + 2 * kWave, win.hi * coef);
+    win.lo = win.mid = win.hi = 0.0f;
+}
+
+// One sample: x = t / W + (mid - floor(mid)), centre jr = rint(x) clamped to
+// [1, B-2] (centred), ku = kappa2 (t - s_j0) = W kappa2 (x - jr); weights as
+// tof_weights' kTofOneExpInner.  t is monotone along a half path (s_k never
+// decreases) and a voxel step moves it by far less than a bin, so a new
+// centre is nearly always cur - 1 on the path to D1 and cur + 1 on the path
+// to D2: the bin leaving the window is added into LDS and the other two
+// shift.  Any other move flushes the whole window.
+template <int SIGN, bool HALF>
+__device__ __forceinline__ void slide_sample(const ScatterArgs& a, const TofConst<float>& d, float jlo,
+                                             float jhi, float* col, SlideWindow& win, int64_t x, int64_t nq,
+                                             float av, float coef, float sw)
+{
+    const float tf = (float)(int)((x + nq) >> 32);   // |t| (sign in sw)
+    const float xf = HALF ? __builtin_fmaf(tf, sw, 0.5f) : tf * sw;
+    const float jr = clampr(rintf(xf), jlo, jhi);
+    if (jr != win.cur) {
+        if (jr == win.cur + (float)SIGN) {
+            if constexpr (SIGN < 0) {
+                lds_add(win.wp + 2 * kWave, win.hi * coef);
+                win.hi = win.mid;
+                win.mid = win.lo;
+                win.lo = 0.0f;
+                win.wp -= kWave;
+            } else {
+                lds_add(win.wp, win.lo * coef);
+                win.lo = win.mid;
+                win.mid = win.hi;
+                win.hi = 0.0f;
+                win.wp += kWave;
+            }
+        } else {
+            slide_flush(win, coef);
+            win.wp = col + ((int)jr + a.slide_row0) * kWave;
+        }
+        win.cur = jr;
+    }
+    const float ku = (xf - jr) * d.wk;
+    const float E = fast_exp2(ku);
+    const float cE = d.c * E;
+    const float D = __builtin_fmaf(cE, E, E + d.c);
+    const float g = av * fast_rcp(D);
+    const float gE = g * E;
+    win.lo = __builtin_fmaf(g, d.c, win.lo);
+    win.mid += gE;
+    win.hi = __builtin_fmaf(gE, cE, win.hi);
+}
+
+// fold_path for the sliding window: the window is placed at the path's first
+// sample and added into LDS, times the path's coefficient, at its end.
+template <int SIGN, bool HALF>
+__device__ __forceinline__ void slide_path(const ScatterArgs& a, const TofConst<float>& d, float* __restrict__ col,
+                                           SlideWindow& win, int64_t base, int n, int64_t shift, float coef)
+{
+    const SampleRec* __restrict__ p = a.s_rec + base;
+    int64_t x = SIGN < 0 ? shift : ((int64_t)0xffffffffLL - shift);
+    asm volatile("" : "+v"(x));   // keep x + nq as one 64-bit add per sample
+    const float sw = SIGN < 0 ? d.inv_w : -d.inv_w;
+    const float jlo = a.f_jlo, jhi = a.f_jhi;
+    // four records in flight: each is loaded four samples before its use
+    SampleRec r0 = p[0], r1 = p[1], r2 = p[2], r3 = p[3];
+    {
+        const float tf = (float)(int)((x + r0.nq) >> 32);
+        const float xf = HALF ? __builtin_fmaf(tf, sw, 0.5f) : tf * sw;
+        win.cur = clampr(rintf(xf), jlo, jhi);
+        win.wp = col + ((int)win.cur + a.slide_row0) * kWave;
+    }
+    for (int k = n; k > 0; k -= 4, p += 4) {
+        slide_sample<SIGN, HALF>(a, d, jlo, jhi, col, win, x, r0.nq, r0.a, coef, sw);
+        r0 = p[4];
+        if (k > 1) slide_sample<SIGN, HALF>(a, d, jlo, jhi, col, win, x, r1.nq, r1.a, coef, sw);
+        r1 = p[5];
+        if (k > 2) slide_sample<SIGN, HALF>(a, d, jlo, jhi, col, win, x, r2.nq, r2.a, coef, sw);
+        r2 = p[6];
+        if (k > 3) slide_sample<SIGN, HALF>(a, d, jlo, jhi, col, win, x, r3.nq, r3.a, coef, sw);
+        r3 = p[7];
+    }
+    slide_flush(win, coef);
+}
+
 // At least 6 waves per SIMD (<= 80 VGPRs): the values that do not fit are
 // spilled outside the sample loops (per slice and per pair only).  Measured
 // on config 2: 4 waves 240 ms, 5 waves 203 ms, 6 waves 189 ms.
-template <typename R, int BMAX, int HM, bool SINGLE_EXP, bool TRIVIAL, bool RANGE>
+template <typename R, int HM, int TM, bool TRIVIAL, bool RANGE>
 __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArgs a)
 {
     extern __shared__ unsigned char smem_raw[];
@@ -346,8 +464,12 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
 
     R num = (R)0;   // one-bin path
     float cnt = 0.0f;
+    constexpr bool SLIDE = TM == kTofSlide || TM == kTofSlideHalf;
+    static_assert(!SLIDE || (sizeof(R) == 4 && HM == 1 && !RANGE), "sliding window: float, h = 1, no range test");
     BinWindow<R, HM> win;
     win.reset((R)0);
+    SlideWindow sl;
+    sl.lo = sl.mid = sl.hi = 0.0f;
     const int p_end = a.chunk_sp[chunk + 1];
     int cursor = valid ? a.chunk_sp[chunk] : p_end;
     while (true) {
@@ -364,15 +486,19 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
             cnt += p.counts;
             if constexpr (TRIVIAL) {
                 num += p.coef1 * p.act1 + p.coef2 * p.act2;
+            } else if constexpr (SLIDE) {
+                constexpr bool HALF = TM == kTofSlideHalf;
+                if (p.n1 > 0) slide_path<-1, HALF>(a, tc, col, sl, p.b1, p.n1, p.shift, p.coef1);
+                if (p.n2 > 0) slide_path<+1, HALF>(a, tc, col, sl, p.b2, p.n2, p.shift, p.coef2);
             } else if (a.t_hi >= a.t_lo) {   // else every TOF weight is zero
-                fold_path<R, HM, SINGLE_EXP, RANGE, -1>(a, tc, col, win, p.b1, p.n1, p.shift, p.coef1);
-                fold_path<R, HM, SINGLE_EXP, RANGE, +1>(a, tc, col, win, p.b2, p.n2, p.shift, p.coef2);
+                fold_path<R, HM, TM, RANGE, -1>(a, tc, col, win, p.b1, p.n1, p.shift, p.coef1);
+                fold_path<R, HM, TM, RANGE, +1>(a, tc, col, win, p.b2, p.n2, p.shift, p.coef2);
             }
         }
     }
     // the chunk's sums and pair count into the lane's slot (the reduce maps
     // slots to cells); window reach rows (bins beyond the ends) are dropped
-    if constexpr (!TRIVIAL) win.spill(col);
+    if constexpr (!TRIVIAL && !SLIDE) win.spill(col);
     if (!valid) return;
     const int64_t n_slots = (int64_t)a.n_tiles * kWave;
     const int64_t slot = (int64_t)tile * kWave + lane;
@@ -520,45 +646,45 @@ __global__ void k_dense_scatter(const R* __restrict__ compact, int n_bins, int n
     dense[((b * (n_det + 1) + rad) * na + ang) * planes + plane] = (float)compact[i];
 }
 
-template <typename R, int BMAX, int HM, bool SINGLE_EXP, bool TRIVIAL, bool RANGE>
+template <typename R, int HM, int TM, bool TRIVIAL, bool RANGE>
 hipError_t launch_variant(const ScatterArgs& a, hipStream_t st)
 {
     // one workgroup per kScatterWaves work items (the item list is padded to that)
     dim3 grid((unsigned)(a.n_items / kScatterWaves));
     size_t lds = TRIVIAL ? 0 : (size_t)kScatterWaves * (a.n_bins + 2 * HM) * kWave * sizeof(R);
-    hipLaunchKernelGGL((k_scatter<R, BMAX, HM, SINGLE_EXP, TRIVIAL, RANGE>), grid,
+    hipLaunchKernelGGL((k_scatter<R, HM, TM, TRIVIAL, RANGE>), grid,
                        dim3(kScatterWaves * kWave), lds, st, a);
     return hipGetLastError();
 }
 
-template <typename R, int BMAX, bool RANGE>
+template <typename R, bool RANGE>
 hipError_t launch_hm(const ScatterArgs& a, hipStream_t st)
 {
     const int h = a.half_window;
-    if (h == 0) return launch_variant<R, BMAX, 0, false, false, RANGE>(a, st);
+    if (h == 0) return launch_variant<R, 0, kTofGeneral, false, RANGE>(a, st);
     if (h == 1) {
-        if (a.single_exp) return launch_variant<R, BMAX, 1, true, false, RANGE>(a, st);
-        return launch_variant<R, BMAX, 1, false, false, RANGE>(a, st);
+        if constexpr (!RANGE) {
+            if (a.tof_mode == kTofOneExpInner) {
+                if constexpr (sizeof(R) == 4)
+                    return (a.n_bins % 2) ? launch_variant<R, 1, kTofSlide, false, false>(a, st)
+                                          : launch_variant<R, 1, kTofSlideHalf, false, false>(a, st);
+                else
+                    return launch_variant<R, 1, kTofOneExpInner, false, false>(a, st);
+            }
+        }
+        if (a.tof_mode == kTofOneExp) return launch_variant<R, 1, kTofOneExp, false, RANGE>(a, st);
+        return launch_variant<R, 1, kTofGeneral, false, RANGE>(a, st);
     }
-    if (h <= 2) return launch_variant<R, BMAX, 2, false, false, RANGE>(a, st);
-    if (h <= 4) return launch_variant<R, BMAX, 4, false, false, RANGE>(a, st);
-    return launch_variant<R, BMAX, 8, false, false, RANGE>(a, st);
-}
-
-template <typename R, bool RANGE>
-hipError_t launch_range(const ScatterArgs& a, hipStream_t st)
-{
-    if (a.n_bins <= 4) return launch_hm<R, 4, RANGE>(a, st);
-    if (a.n_bins <= 8) return launch_hm<R, 8, RANGE>(a, st);
-    if (a.n_bins <= 16) return launch_hm<R, 16, RANGE>(a, st);
-    return launch_hm<R, 32, RANGE>(a, st);
+    if (h <= 2) return launch_variant<R, 2, kTofGeneral, false, RANGE>(a, st);
+    if (h <= 4) return launch_variant<R, 4, kTofGeneral, false, RANGE>(a, st);
+    return launch_variant<R, 8, kTofGeneral, false, RANGE>(a, st);
 }
 
 template <typename R>
 hipError_t launch_typed(const ScatterArgs& a, bool trivial, hipStream_t st)
 {
-    if (trivial) return launch_variant<R, 1, 1, false, true, false>(a, st);
-    return a.range_check ? launch_range<R, true>(a, st) : launch_range<R, false>(a, st);
+    if (trivial) return launch_variant<R, 1, kTofGeneral, true, false>(a, st);
+    return a.range_check ? launch_hm<R, true>(a, st) : launch_hm<R, false>(a, st);
 }
 
 }  // namespace

@@ -106,20 +106,45 @@ def test_small_tof_workload_vs_oracle(oracle, precision, tol):
     assert rel <= tol and absn <= 1e-6
 
 
-@pytest.mark.parametrize('n_bins', [15, 21, 1, 3])
-def test_tof_kernel_variants_vs_oracle(oracle, n_bins):
-    """15 bins: one-exp window (h = 1); 21 bins: h = 2; one bin of a 214 ps
-    kernel: zero TOF columns beyond +-3508 ps; 3 bins: narrow kernel (two-exp)."""
+# (bins, TOF resolution ps, TOF range ps) -> TOF fold of the FP32 and FP64 launches
+# (0 one exp2 per window bin, 1 one exp2 with end-bin masks, 2 one exp2 with the
+# window centre inside the bins, 3 as 2 with the sliding window).  The FP64
+# path drops bins below 1e-16 of the nearest one (FP32: 1e-9), so its window
+# is wider at 214 ps.
+TOF_VARIANTS = {
+    'h1-15bins': ((15, 214.0, 3335.64), 3, 0),
+    'h1-14bins-even': ((14, 214.0, 3335.64), 3, 0),
+    'h2-21bins': ((21, 214.0, 3335.64), 0, 0),
+    'h0-1bin-zero-columns': ((1, 214.0, 3335.64), 0, 0),
+    'h1-3bins-narrow': ((3, 214.0, 3335.64), 0, 0),
+    'h1-3bins-masked': ((3, 600.0, 2417.0), 1, 1),
+}
+
+
+@pytest.mark.parametrize('case', list(TOF_VARIANTS))
+def test_tof_kernel_variants_vs_oracle(oracle, case):
+    """15 and 14 bins at 214 ps: one exp2 per sample with a sliding window (odd
+    and even bin counts, bin centres at integer / half-integer multiples of W);
+    21 bins: h = 2; one bin: zero TOF columns beyond +-3508 ps; 3 bins at
+    214 ps: narrow kernel (two exp2); 3 bins of 1611 ps at 600 ps: offsets far
+    beyond the outer bins, so the window keeps masked end bins."""
+    import dataclasses
+    (n_bins, res, rng), fold32, fold64 = TOF_VARIANTS[case]
     w = phantoms.small_tof(seed=12, n_bins=n_bins, n_sp=30)
-    got = gpu_compact(w, 'fp32')
+    w = dataclasses.replace(w, tof_resolution=res, tof_range=rng)
     want = oracle_compact(oracle, w)
-    rel, absn = rel_err_above(got, want)
-    print(f'{w.name}: max rel err {rel:.3e}, max abs/peak {absn:.3e}')
-    assert rel <= FP32_TOL and absn <= 1e-6
-    got64 = gpu_compact(w, 'fp64')
-    rel, absn = rel_err_above(got64, want)
-    # the oracle stores its per-slice sums in float32 as the reference does
-    assert rel <= FP64_TOL and absn <= 1e-6
+    for precision, fold, tol in (('fp32', fold32, FP32_TOL), ('fp64', fold64, FP64_TOL)):
+        s = gpu_subject(w, precision)
+        try:
+            s.run(w.activity)
+            got = s.output_compact()
+            assert s.info['tof_fold'] == fold, (precision, s.info['tof_fold'])
+        finally:
+            s.close()
+        rel, absn = rel_err_above(got, want)
+        print(f'{w.name} {case} {precision}: max rel err {rel:.3e}, max abs/peak {absn:.3e}')
+        # the oracle stores its per-slice sums in float32 as the reference does
+        assert rel <= tol and absn <= 1e-6
 
 
 def test_config1_nontof_full_vs_oracle(oracle):
