@@ -70,6 +70,9 @@ struct HalfPathTable {
 // sample in a thousand (the fractions of (d1 - d2)/c and 2 s_k/c are often
 // nearly complementary), so the fraction needs far more than 20 bits.
 constexpr double kFixOne = 4294967296.0;   // 2^32
+// Each half path's records are padded to a multiple of kSampleGroup with
+// zero-weight copies of its last record (the FP32 fold runs whole groups).
+constexpr int kSampleGroup = 4;
 constexpr int kSamplePad = 8;              // records after the last one (fold prefetch reads up to 7 ahead)
 
 // One voxel sample of a half path, read by the fold with a single 16-byte load.
@@ -79,8 +82,9 @@ struct alignas(16) SampleRec {
     int32_t vox;       // 0-based voxel (reference linear index - 1)
 };
 
-// Per-sample arrays (compacted, offsets in HalfPathTable::off; kSamplePad
-// spare records at the end of rec and a64).
+// Per-sample arrays (offsets in HalfPathTable::off, each half path padded to
+// a multiple of kSampleGroup records; kSamplePad spare records at the end of
+// rec and a64).
 struct SampleTable {
     SampleRec* rec;
     double* len;       // intersection length, cm
@@ -119,7 +123,7 @@ struct ScatterArgs {
     int tof_mode;                                  // kTofGeneral / kTofOneExp / kTofOneExpInner
     int range_check;                               // offsets may leave [t_lo, t_hi]: test each sample
     // the same constants rounded to float once on the host (uniform registers)
-    float f_inv_w, f_kap, f_mid, f_jmax, f_wk, f_mid_wk, f_c;
+    float f_inv_w, f_kap, f_mid, f_jmax, f_wk, f_mid_wk, f_c, f_inv_c;
     // sliding window: centred bin jr = bin - floor(mid) kept in [f_jlo, f_jhi];
     // LDS row of bin jr - 1 is jr + slide_row0
     float f_jlo, f_jhi;

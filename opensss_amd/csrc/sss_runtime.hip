@@ -47,6 +47,14 @@ __global__ void k_count_status(const int32_t* __restrict__ st, int64_t n, int32_
     if (m) atomicAdd(out, m);
 }
 
+// Records a half path occupies in the sample table.
+struct PaddedRecords {
+    __host__ __device__ int64_t operator()(int32_t n) const
+    {
+        return (int64_t)((n + kSampleGroup - 1) / kSampleGroup) * kSampleGroup;
+    }
+};
+
 template <typename T>
 int dev_alloc(T** p, size_t count, int64_t* tally)
 {
@@ -528,9 +536,11 @@ int sss_subject_trace(sss_subject* s)
     }
     SSS_HIP(launch_halfpath_count(s->grid, s->d_sp, s->n_sp, s->d_detpos, s->d_detnorm, s->nru, s->ndu,
                                   s->hp, st));
-    // exclusive scan of the per-half-path voxel counts (int64 offsets)
+    // exclusive scan of the per-half-path record counts (int64 offsets): each
+    // half path's samples padded to a multiple of kSampleGroup records
+    const auto padded = rocprim::make_transform_iterator(s->hp.n, PaddedRecords{});
     size_t need = 0;
-    SSS_HIP(rocprim::exclusive_scan(nullptr, need, s->hp.n, s->hp.off, (int64_t)0, (size_t)s->n_hp + 1,
+    SSS_HIP(rocprim::exclusive_scan(nullptr, need, padded, s->hp.off, (int64_t)0, (size_t)s->n_hp + 1,
                                     rocprim::plus<int64_t>(), st));
     if (need > s->scan_tmp_bytes) {
         dev_free(s->d_scan_tmp);
@@ -542,7 +552,7 @@ int sss_subject_trace(sss_subject* s)
     }
     // slot n_hp holds a zero count so off[n_hp] is the total
     SSS_HIP(hipMemsetAsync(s->hp.n + s->n_hp, 0, sizeof(int32_t), st));
-    SSS_HIP(rocprim::exclusive_scan(s->d_scan_tmp, need, s->hp.n, s->hp.off, (int64_t)0, (size_t)s->n_hp + 1,
+    SSS_HIP(rocprim::exclusive_scan(s->d_scan_tmp, need, padded, s->hp.off, (int64_t)0, (size_t)s->n_hp + 1,
                                     rocprim::plus<int64_t>(), st));
     int64_t total = 0;
     SSS_HIP(hipMemcpyAsync(&total, s->hp.off + s->n_hp, sizeof(int64_t), hipMemcpyDeviceToHost, st));
@@ -637,6 +647,7 @@ static ScatterArgs scatter_args(const sss_subject* s)
     a.f_wk = (float)(a.bin_w * a.kappa2);
     a.f_mid_wk = (float)(a.mid * a.bin_w * a.kappa2);
     a.f_c = (float)a.c_beta;
+    a.f_inv_c = (float)std::exp2(a.beta2);
     {
         // centred bins jr = bin - floor(mid); the window centre stays in bins [1, B-2]
         const int m = (s->n_bins - 1) / 2;

@@ -155,6 +155,7 @@ struct TofConst {
     R mid, jmax;  // (B-1)/2, B-1
     R wk, mid_wk; // W kappa2, mid W kappa2
     R c;          // 2^-beta2
+    R inv_c;      // 2^beta2
 };
 
 template <typename R>
@@ -163,10 +164,11 @@ __device__ __forceinline__ TofConst<R> tof_const(const ScatterArgs& a)
     TofConst<R> d;
     if constexpr (sizeof(R) == 4) {
         d.inv_w = a.f_inv_w; d.kap = a.f_kap; d.mid = a.f_mid; d.jmax = a.f_jmax;
-        d.wk = a.f_wk; d.mid_wk = a.f_mid_wk; d.c = a.f_c;
+        d.wk = a.f_wk; d.mid_wk = a.f_mid_wk; d.c = a.f_c; d.inv_c = a.f_inv_c;
     } else {
         d.inv_w = a.inv_w_bins; d.kap = a.kappa2; d.mid = a.mid; d.jmax = (R)(a.n_bins - 1);
         d.wk = a.bin_w * a.kappa2; d.mid_wk = a.mid * a.bin_w * a.kappa2; d.c = a.c_beta;
+        d.inv_c = 1.0 / a.c_beta;
     }
     return d;
 }
@@ -330,36 +332,40 @@ __device__ __forceinline__ void fold_path(const ScatterArgs& a, const TofConst<R
     }
 }
 
-// Sliding window of the float h = 1 fold (kTofSlide, kTofSlideHalf): the
-// sums of bins cur - 1, cur, cur + 1 of the current half path, without the
-// path's coefficient (applied when a bin leaves the window).  wp: LDS row of
-// bin cur - 1 in the lane's column.
+// Sliding window of the float h = 1 fold (kTofSlide, kTofSlideHalf) over
+// the current half path, without the path's coefficient (applied when a bin
+// leaves the window): bin cur - 1 holds c A, bin cur holds M and bin cur + 1
+// holds c P, c = 2^-beta2 (the side bins' common factor is applied when they
+// are added into LDS or become the centre).  wp: LDS row of bin cur - 1 in
+// the lane's column.
 struct SlideWindow {
-    float lo, mid, hi;
+    float A, M, P;
     float cur;     // centred centre bin jr (bin jr + floor((B-1)/2))
     float* wp;
 };
 
-// Adds the window, times coef, into its three LDS rows and clears it.
-__device__ __forceinline__ void slide_flush(SlideWindow& win, float coef)
+// Adds the window, times the path coefficient (ccoef = c coef), into its
+// three LDS rows and clears it.
+__device__ __forceinline__ void slide_flush(SlideWindow& win, float coef, float ccoef)
 {
-    lds_add(win.wp, win.lo * coef);
-    lds_add(win.wp + kWave, win.mid * coef);
-    lds_add(win.wp + 2 * kWave, win.hi * coef);
-    win.lo = win.mid = win.hi = 0.0f;
+    lds_add(win.wp, win.A * ccoef);
+    lds_add(win.wp + kWave, win.M * coef);
+    lds_add(win.wp + 2 * kWave, win.P * ccoef);
+    win.A = win.M = win.P = 0.0f;
 }
 
 // One sample: x = t / W + (mid - floor(mid)), centre jr = rint(x) clamped to
-// [1, B-2] (centred), ku = kappa2 (t - s_j0) = W kappa2 (x - jr); weights as
-// tof_weights' kTofOneExpInner.  t is monotone along a half path (s_k never
-// decreases) and a voxel step moves it by far less than a bin, so a new
-// centre is nearly always cur - 1 on the path to D1 and cur + 1 on the path
-// to D2: the bin leaving the window is added into LDS and the other two
-// shift.  Any other move flushes the whole window.
+// [1, B-2] (centred), ku = kappa2 (t - s_j0) = W kappa2 (x - jr), E = 2^ku,
+// D = c E^2 + E + c; the weights of bins jr - 1, jr, jr + 1 are a c/D,
+// a E/D and a c E^2/D (tof_weights' kTofOneExpInner).  t is monotone along a
+// half path (s_k never decreases) and a voxel step moves it by far less than
+// a bin, so a new centre is nearly always cur - 1 on the path to D1 and
+// cur + 1 on the path to D2: the bin leaving the window is added into LDS and
+// the other two shift.  Any other move flushes the whole window.
 template <int SIGN, bool HALF>
 __device__ __forceinline__ void slide_sample(const ScatterArgs& a, const TofConst<float>& d, float jlo,
                                              float jhi, float* col, SlideWindow& win, int64_t x, int64_t nq,
-                                             float av, float coef, float sw)
+                                             float av, float coef, float ccoef, float sw)
 {
     const float tf = (float)(int)((x + nq) >> 32);   // |t| (sign in sw)
     const float xf = HALF ? __builtin_fmaf(tf, sw, 0.5f) : tf * sw;
@@ -367,37 +373,39 @@ __device__ __forceinline__ void slide_sample(const ScatterArgs& a, const TofCons
     if (jr != win.cur) {
         if (jr == win.cur + (float)SIGN) {
             if constexpr (SIGN < 0) {
-                lds_add(win.wp + 2 * kWave, win.hi * coef);
-                win.hi = win.mid;
-                win.mid = win.lo;
-                win.lo = 0.0f;
+                lds_add(win.wp + 2 * kWave, win.P * ccoef);
+                win.P = win.M * d.inv_c;
+                win.M = win.A * d.c;
+                win.A = 0.0f;
                 win.wp -= kWave;
             } else {
-                lds_add(win.wp, win.lo * coef);
-                win.lo = win.mid;
-                win.mid = win.hi;
-                win.hi = 0.0f;
+                lds_add(win.wp, win.A * ccoef);
+                win.A = win.M * d.inv_c;
+                win.M = win.P * d.c;
+                win.P = 0.0f;
                 win.wp += kWave;
             }
         } else {
-            slide_flush(win, coef);
+            slide_flush(win, coef, ccoef);
             win.wp = col + ((int)jr + a.slide_row0) * kWave;
         }
         win.cur = jr;
     }
     const float ku = (xf - jr) * d.wk;
     const float E = fast_exp2(ku);
-    const float cE = d.c * E;
-    const float D = __builtin_fmaf(cE, E, E + d.c);
+    const float D = __builtin_fmaf(E, __builtin_fmaf(d.c, E, 1.0f), d.c);
     const float g = av * fast_rcp(D);
     const float gE = g * E;
-    win.lo = __builtin_fmaf(g, d.c, win.lo);
-    win.mid += gE;
-    win.hi = __builtin_fmaf(gE, cE, win.hi);
+    win.A += g;
+    win.M += gE;
+    win.P = __builtin_fmaf(gE, E, win.P);
 }
 
-// fold_path for the sliding window: the window is placed at the path's first
-// sample and added into LDS, times the path's coefficient, at its end.
+// fold_path for the sliding window.  The path's records are padded to a
+// multiple of 4 with zero-weight copies of its last record, so the loop runs
+// whole groups of 4 without tests; each record is loaded 4 samples before its
+// use.  The window is placed at the path's first sample and added into LDS at
+// its end.
 template <int SIGN, bool HALF>
 __device__ __forceinline__ void slide_path(const ScatterArgs& a, const TofConst<float>& d, float* __restrict__ col,
                                            SlideWindow& win, int64_t base, int n, int64_t shift, float coef)
@@ -406,8 +414,10 @@ __device__ __forceinline__ void slide_path(const ScatterArgs& a, const TofConst<
     int64_t x = SIGN < 0 ? shift : ((int64_t)0xffffffffLL - shift);
     asm volatile("" : "+v"(x));   // keep x + nq as one 64-bit add per sample
     const float sw = SIGN < 0 ? d.inv_w : -d.inv_w;
-    const float jlo = a.f_jlo, jhi = a.f_jhi;
-    // four records in flight: each is loaded four samples before its use
+    const float jlo = a.f_jlo;
+    float jhi = a.f_jhi;
+    asm volatile("" : "+v"(jhi));   // med3 takes one scalar operand
+    const float ccoef = coef * d.c;
     SampleRec r0 = p[0], r1 = p[1], r2 = p[2], r3 = p[3];
     {
         const float tf = (float)(int)((x + r0.nq) >> 32);
@@ -415,17 +425,24 @@ __device__ __forceinline__ void slide_path(const ScatterArgs& a, const TofConst<
         win.cur = clampr(rintf(xf), jlo, jhi);
         win.wp = col + ((int)win.cur + a.slide_row0) * kWave;
     }
+    // Each empty asm statement comes after the previous load (memory clobber)
+    // and takes the next sample's offset, so that sample waits for its record
+    // only after the load is issued: four loads stay in flight.
     for (int k = n; k > 0; k -= 4, p += 4) {
-        slide_sample<SIGN, HALF>(a, d, jlo, jhi, col, win, x, r0.nq, r0.a, coef, sw);
+        slide_sample<SIGN, HALF>(a, d, jlo, jhi, col, win, x, r0.nq, r0.a, coef, ccoef, sw);
         r0 = p[4];
-        if (k > 1) slide_sample<SIGN, HALF>(a, d, jlo, jhi, col, win, x, r1.nq, r1.a, coef, sw);
+        asm volatile("" : "+v"(r1.nq) : : "memory");
+        slide_sample<SIGN, HALF>(a, d, jlo, jhi, col, win, x, r1.nq, r1.a, coef, ccoef, sw);
         r1 = p[5];
-        if (k > 2) slide_sample<SIGN, HALF>(a, d, jlo, jhi, col, win, x, r2.nq, r2.a, coef, sw);
+        asm volatile("" : "+v"(r2.nq) : : "memory");
+        slide_sample<SIGN, HALF>(a, d, jlo, jhi, col, win, x, r2.nq, r2.a, coef, ccoef, sw);
         r2 = p[6];
-        if (k > 3) slide_sample<SIGN, HALF>(a, d, jlo, jhi, col, win, x, r3.nq, r3.a, coef, sw);
+        asm volatile("" : "+v"(r3.nq) : : "memory");
+        slide_sample<SIGN, HALF>(a, d, jlo, jhi, col, win, x, r3.nq, r3.a, coef, ccoef, sw);
         r3 = p[7];
+        asm volatile("" : "+v"(r0.nq) : : "memory");
     }
-    slide_flush(win, coef);
+    slide_flush(win, coef, ccoef);
 }
 
 // At least 6 waves per SIMD (<= 80 VGPRs): the values that do not fit are
@@ -469,7 +486,7 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
     BinWindow<R, HM> win;
     win.reset((R)0);
     SlideWindow sl;
-    sl.lo = sl.mid = sl.hi = 0.0f;
+    sl.A = sl.M = sl.P = 0.0f;
     const int p_end = a.chunk_sp[chunk + 1];
     int cursor = valid ? a.chunk_sp[chunk] : p_end;
     while (true) {

@@ -71,8 +71,8 @@ __global__ void k_halfpath_fill(Grid g, const double* __restrict__ sp_xyz, int n
         const int64_t base = t.off[h];
         const double two_over_c = 2.0 / kLightCmPerPs;
         bool truncated = false;
+        SampleRec r;
         const int got = walk_run(g, w, [&](int k, int vox, double L, double smp) {
-            SampleRec r;
             // q * 2^32 is exact in float64; rint drops bits below 2^-32 ps
             r.nq = -(int64_t)rint(smp * two_over_c * kFixOne);
             r.a = 0.0f;
@@ -83,6 +83,13 @@ __global__ void k_halfpath_fill(Grid g, const double* __restrict__ sp_xyz, int n
         }, &truncated);
         if (truncated) status = 1;
         t.n[h] = got;   // <= the count the offsets were built from
+        // zero-weight copies of the last record (same offset, length 0, so the
+        // activity refresh keeps their weight 0) up to a whole group
+        for (int k = got; k % kSampleGroup != 0; ++k) {
+            s.rec[base + k] = r;
+            s.len[base + k] = 0.0;
+            if (s.a64) s.a64[base + k] = 0.0;
+        }
     }
     t.lnatt[h] = mu_int;
     t.att[h] = exp(-mu_int);
