@@ -56,6 +56,8 @@ struct Grid {
 struct HalfPathTable {
     int64_t* off;      // first sample
     int32_t* n;        // samples
+    int64_t* off_fold; // first record in the fold table
+    int32_t* n_fold;   // samples with lam L != 0 (the ones the fold visits)
     double* dist;      // |D - S|, cm
     double* att;       // exp(-sum mu L)
     double* lnatt;     // sum mu L
@@ -70,10 +72,11 @@ struct HalfPathTable {
 // sample in a thousand (the fractions of (d1 - d2)/c and 2 s_k/c are often
 // nearly complementary), so the fraction needs far more than 20 bits.
 constexpr double kFixOne = 4294967296.0;   // 2^32
-// Each half path's records are padded to a multiple of kSampleGroup with
-// zero-weight copies of its last record (the FP32 fold runs whole groups).
+// In the fold table each half path's records are padded to a multiple of
+// kSampleGroup with zero-weight copies of its last record (the FP32 fold runs
+// whole groups).
 constexpr int kSampleGroup = 4;
-constexpr int kSamplePad = 8;              // records after the last one (fold prefetch reads up to 7 ahead)
+constexpr int kSamplePad = 8;              // fold records after the last one (prefetch reads up to 7 ahead)
 
 // One voxel sample of a half path, read by the fold with a single 16-byte load.
 struct alignas(16) SampleRec {
@@ -82,13 +85,18 @@ struct alignas(16) SampleRec {
     int32_t vox;       // 0-based voxel (reference linear index - 1)
 };
 
-// Per-sample arrays (offsets in HalfPathTable::off, each half path padded to
-// a multiple of kSampleGroup records; kSamplePad spare records at the end of
-// rec and a64).
+// Per-sample arrays.  rec/len/a64: every voxel sample of every half path
+// (offsets HalfPathTable::off), built by the tracer; a and a64 rewritten per
+// activity image.  fold/fold_a64: per activity image, the samples with
+// lam L != 0 only (their terms are exactly zero otherwise), offsets
+// HalfPathTable::off_fold, padded per half path to whole groups, kSamplePad
+// spare records at the end.
 struct SampleTable {
     SampleRec* rec;
     double* len;       // intersection length, cm
     double* a64;       // lam * len in float64 (FP64 validation path only, else null)
+    SampleRec* fold;
+    double* fold_a64;  // FP64 path only
 };
 
 struct ScatterArgs {
@@ -97,15 +105,15 @@ struct ScatterArgs {
     int n_tiles, n_chunks;
     int64_t n_cells_plane;       // NDU (NDU+1)/2
     // tables
-    const int64_t* hp_off;
-    const int32_t* hp_n;
+    const int64_t* hp_off;       // first fold record of each half path
+    const int32_t* hp_n;         // fold records (samples with lam L != 0)
     const double* hp_dist;
     const double* hp_att;
     const double* hp_lnatt;
     const double* hp_act;
     const float* hp_cos;
-    const SampleRec* s_rec;      // fixed-point TOF offsets, FP32 emission terms
-    const double* s_a64;         // FP64 emission terms (FP64 path)
+    const SampleRec* s_rec;      // fold table: fixed-point TOF offsets, FP32 emission terms
+    const double* s_a64;         // fold table FP64 emission terms (FP64 path)
     const double* detpos;        // (NRU, NDU, 3)
     const double* eff;           // n_energy
     const float* ratio;          // n_energy
@@ -153,8 +161,12 @@ hipError_t launch_halfpath_count(const Grid& g, const double* sp_xyz, int n_sp, 
 hipError_t launch_halfpath_fill(const Grid& g, const double* sp_xyz, int n_sp, const double* detpos,
                                 int nru, int ndu, const double* mu, HalfPathTable t, SampleTable s,
                                 hipStream_t st);
+// Emission terms lam L of every sample, their sums and the fold-table counts
+// (then the caller scans the padded counts into off_fold), and the copy of
+// the non-zero terms into the fold table.
 hipError_t launch_activity_refresh(const HalfPathTable t, SampleTable s, int64_t n_hp,
                                    const double* lam, hipStream_t st);
+hipError_t launch_fold_compact(const HalfPathTable t, SampleTable s, int64_t n_hp, hipStream_t st);
 hipError_t launch_trace_segments(const Grid& g, const double* segs, int64_t n, int cap,
                                  int32_t* counts, double* L, int32_t* idx, double* s,
                                  hipStream_t st);

@@ -196,9 +196,10 @@ struct sss_subject {
         dev_free(d_mu); dev_free(d_lam); dev_free(d_sp); dev_free(d_detpos); dev_free(d_detnorm);
         dev_free(d_eff); dev_free(d_ratio); dev_free(d_tiles); dev_free(d_slice_ptr);
         dev_free(d_chunk_sp); dev_free(d_chunk_slice); dev_free(d_dets); dev_free(d_flag); dev_free(d_scan_tmp);
-        dev_free(hp.off); dev_free(hp.n); dev_free(hp.dist); dev_free(hp.att); dev_free(hp.lnatt);
+        dev_free(hp.off); dev_free(hp.n); dev_free(hp.off_fold); dev_free(hp.n_fold);
+        dev_free(hp.dist); dev_free(hp.att); dev_free(hp.lnatt);
         dev_free(hp.act); dev_free(hp.cosabs); dev_free(hp.status);
-        dev_free(smp.rec); dev_free(smp.len); dev_free(smp.a64);
+        dev_free(smp.rec); dev_free(smp.len); dev_free(smp.a64); dev_free(smp.fold); dev_free(smp.fold_a64);
         dev_free(d_out); dev_free(d_part_sum); dev_free(d_part_cnt); dev_free(d_items);
     }
 };
@@ -471,6 +472,8 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
         (rc = dev_alloc(&s->d_chunk_slice, chunk_slice.size() + 1, &t)) ||
         (rc = dev_alloc(&s->d_dets, (size_t)d->ndu, &t)) || (rc = dev_alloc(&s->d_flag, 1, &t)) ||
         (rc = dev_alloc(&s->hp.off, (size_t)s->n_hp + 1, &t)) || (rc = dev_alloc(&s->hp.n, (size_t)s->n_hp + 1, &t)) ||
+        (rc = dev_alloc(&s->hp.off_fold, (size_t)s->n_hp + 1, &t)) ||
+        (rc = dev_alloc(&s->hp.n_fold, (size_t)s->n_hp + 1, &t)) ||
         (rc = dev_alloc(&s->hp.dist, (size_t)s->n_hp + 1, &t)) || (rc = dev_alloc(&s->hp.att, (size_t)s->n_hp + 1, &t)) ||
         (rc = dev_alloc(&s->hp.lnatt, (size_t)s->n_hp + 1, &t)) || (rc = dev_alloc(&s->hp.act, (size_t)s->n_hp + 1, &t)) ||
         (rc = dev_alloc(&s->hp.cosabs, (size_t)s->n_hp + 1, &t)) ||
@@ -523,6 +526,39 @@ int sss_subject_destroy(sss_subject* s)
     return SSS_OK;
 }
 
+// Exclusive scan of n_hp + 1 per-half-path record counts (slot n_hp is 0,
+// so out[n_hp] is the total) into int64 offsets.
+extern "C++" template <typename It>
+static int scan_counts(sss_subject* s, It counts, int64_t* out, hipStream_t st)
+{
+    size_t need = 0;
+    SSS_HIP(rocprim::exclusive_scan(nullptr, need, counts, out, (int64_t)0, (size_t)s->n_hp + 1,
+                                    rocprim::plus<int64_t>(), st));
+    if (need > s->scan_tmp_bytes) {
+        dev_free(s->d_scan_tmp);
+        unsigned char* tmp = nullptr;
+        int rc = dev_alloc(&tmp, need, &s->device_bytes);
+        if (rc) return rc;
+        s->d_scan_tmp = tmp;
+        s->scan_tmp_bytes = need;
+    }
+    SSS_HIP(rocprim::exclusive_scan(s->d_scan_tmp, need, counts, out, (int64_t)0, (size_t)s->n_hp + 1,
+                                    rocprim::plus<int64_t>(), st));
+    return SSS_OK;
+}
+
+// Per activity image: emission terms, their sums and the fold table of the
+// non-zero terms.
+static int refresh_fold(sss_subject* s)
+{
+    hipStream_t st = s->ctx->stream;
+    SSS_HIP(launch_activity_refresh(s->hp, s->smp, s->n_hp, s->d_lam, st));
+    const auto padded = rocprim::make_transform_iterator(s->hp.n_fold, PaddedRecords{});
+    if (int rc = scan_counts(s, padded, s->hp.off_fold, st)) return rc;
+    SSS_HIP(launch_fold_compact(s->hp, s->smp, s->n_hp, st));
+    return SSS_OK;
+}
+
 int sss_subject_trace(sss_subject* s)
 {
     if (!s) return fail(SSS_ERR_INVALID, "sss_subject_trace: NULL subject");
@@ -536,36 +572,27 @@ int sss_subject_trace(sss_subject* s)
     }
     SSS_HIP(launch_halfpath_count(s->grid, s->d_sp, s->n_sp, s->d_detpos, s->d_detnorm, s->nru, s->ndu,
                                   s->hp, st));
-    // exclusive scan of the per-half-path record counts (int64 offsets): each
-    // half path's samples padded to a multiple of kSampleGroup records
-    const auto padded = rocprim::make_transform_iterator(s->hp.n, PaddedRecords{});
-    size_t need = 0;
-    SSS_HIP(rocprim::exclusive_scan(nullptr, need, padded, s->hp.off, (int64_t)0, (size_t)s->n_hp + 1,
-                                    rocprim::plus<int64_t>(), st));
-    if (need > s->scan_tmp_bytes) {
-        dev_free(s->d_scan_tmp);
-        unsigned char* tmp = nullptr;
-        int rc = dev_alloc(&tmp, need, &s->device_bytes);
-        if (rc) return rc;
-        s->d_scan_tmp = tmp;
-        s->scan_tmp_bytes = need;
-    }
     // slot n_hp holds a zero count so off[n_hp] is the total
     SSS_HIP(hipMemsetAsync(s->hp.n + s->n_hp, 0, sizeof(int32_t), st));
-    SSS_HIP(rocprim::exclusive_scan(s->d_scan_tmp, need, padded, s->hp.off, (int64_t)0, (size_t)s->n_hp + 1,
-                                    rocprim::plus<int64_t>(), st));
+    SSS_HIP(hipMemsetAsync(s->hp.n_fold + s->n_hp, 0, sizeof(int32_t), st));
+    if (int rc = scan_counts(s, s->hp.n, s->hp.off, st)) return rc;
     int64_t total = 0;
     SSS_HIP(hipMemcpyAsync(&total, s->hp.off + s->n_hp, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     SSS_HIP(hipStreamSynchronize(st));
     if (total > s->samples_cap || !s->smp.rec) {
         dev_free(s->smp.rec); dev_free(s->smp.len); dev_free(s->smp.a64);
+        dev_free(s->smp.fold); dev_free(s->smp.fold_a64);
         s->samples_cap = 0;
+        // the fold table holds at most every sample plus the group padding,
+        // and spare records after the last one (the fold prefetches ahead)
+        const size_t fold_cap = (size_t)total + (size_t)(kSampleGroup - 1) * s->n_hp + kSamplePad;
+        const bool fp64 = s->precision == SSS_PRECISION_FP64;
         int rc;
-        // spare records after the last sample: the fold prefetches ahead
-        if ((rc = dev_alloc(&s->smp.rec, (size_t)total + kSamplePad, &s->device_bytes)) ||
+        if ((rc = dev_alloc(&s->smp.rec, (size_t)total, &s->device_bytes)) ||
             (rc = dev_alloc(&s->smp.len, (size_t)total, &s->device_bytes)) ||
-            (s->precision == SSS_PRECISION_FP64 &&
-             (rc = dev_alloc(&s->smp.a64, (size_t)total + kSamplePad, &s->device_bytes))))
+            (rc = dev_alloc(&s->smp.fold, fold_cap, &s->device_bytes)) ||
+            (fp64 && ((rc = dev_alloc(&s->smp.a64, (size_t)total, &s->device_bytes)) ||
+                      (rc = dev_alloc(&s->smp.fold_a64, fold_cap, &s->device_bytes)))))
             return rc;
         s->samples_cap = total;
     }
@@ -577,14 +604,14 @@ int sss_subject_trace(sss_subject* s)
     hipLaunchKernelGGL(k_count_status, dim3(256), dim3(256), 0, st, s->hp.status, s->n_hp, s->d_flag);
     SSS_HIP(hipGetLastError());
     s->traced = true;
-    if (s->has_activity) SSS_HIP(launch_activity_refresh(s->hp, s->smp, s->n_hp, s->d_lam, st));
+    if (s->has_activity) return refresh_fold(s);
     return SSS_OK;
 }
 
 static int refresh_activity(sss_subject* s)
 {
     s->has_activity = true;
-    if (s->traced) SSS_HIP(launch_activity_refresh(s->hp, s->smp, s->n_hp, s->d_lam, s->ctx->stream));
+    if (s->traced) return refresh_fold(s);
     return SSS_OK;
 }
 
@@ -612,9 +639,9 @@ static ScatterArgs scatter_args(const sss_subject* s)
     a.n_sp = s->n_sp; a.nru = s->nru; a.ndu = s->ndu; a.n_bins = s->n_bins;
     a.n_energy = s->n_energy; a.eff_ref_row = s->eff_ref_row;
     a.n_tiles = s->n_tiles; a.n_chunks = s->n_chunks; a.n_cells_plane = s->n_cells_plane;
-    a.hp_off = s->hp.off; a.hp_n = s->hp.n; a.hp_dist = s->hp.dist; a.hp_att = s->hp.att;
+    a.hp_off = s->hp.off_fold; a.hp_n = s->hp.n_fold; a.hp_dist = s->hp.dist; a.hp_att = s->hp.att;
     a.hp_lnatt = s->hp.lnatt; a.hp_act = s->hp.act; a.hp_cos = s->hp.cosabs;
-    a.s_rec = s->smp.rec; a.s_a64 = s->smp.a64;
+    a.s_rec = s->smp.fold; a.s_a64 = s->smp.fold_a64;
     a.detpos = s->d_detpos; a.eff = s->d_eff; a.ratio = s->d_ratio; a.tiles = s->d_tiles;
     a.items = s->d_items; a.n_items = s->n_items; a.tile_db_log2 = s->tile_db_log2;
     a.slice_ptr = s->d_slice_ptr; a.chunk_sp = s->d_chunk_sp;

@@ -71,8 +71,8 @@ __global__ void k_halfpath_fill(Grid g, const double* __restrict__ sp_xyz, int n
         const int64_t base = t.off[h];
         const double two_over_c = 2.0 / kLightCmPerPs;
         bool truncated = false;
-        SampleRec r;
         const int got = walk_run(g, w, [&](int k, int vox, double L, double smp) {
+            SampleRec r;
             // q * 2^32 is exact in float64; rint drops bits below 2^-32 ps
             r.nq = -(int64_t)rint(smp * two_over_c * kFixOne);
             r.a = 0.0f;
@@ -83,23 +83,17 @@ __global__ void k_halfpath_fill(Grid g, const double* __restrict__ sp_xyz, int n
         }, &truncated);
         if (truncated) status = 1;
         t.n[h] = got;   // <= the count the offsets were built from
-        // zero-weight copies of the last record (same offset, length 0, so the
-        // activity refresh keeps their weight 0) up to a whole group
-        for (int k = got; k % kSampleGroup != 0; ++k) {
-            s.rec[base + k] = r;
-            s.len[base + k] = 0.0;
-            if (s.a64) s.a64[base + k] = 0.0;
-        }
     }
     t.lnatt[h] = mu_int;
     t.att[h] = exp(-mu_int);
     t.status[h] = status;
 }
 
-// Emission terms lam[v] L per sample and their float64 sum per half path
-// (ActivityRay and ActPath, SingleScatterSimulationTOF.py:417-418).  A
-// group of kRefreshLanes lanes walks one half path with consecutive samples
-// on consecutive lanes (coalesced), then sums its partial sums; the terms are
+// Emission terms lam[v] L per sample, their float64 sum per half path
+// (ActivityRay and ActPath, SingleScatterSimulationTOF.py:417-418) and the
+// number of non-zero terms (the samples the fold visits).  A group of
+// kRefreshLanes lanes walks one half path with consecutive samples on
+// consecutive lanes (coalesced), then sums its partial sums; the terms are
 // non-negative, so the exact-zero tests on the sum do not depend on the order.
 constexpr int kRefreshLanes = 16;
 
@@ -109,6 +103,7 @@ __global__ void k_activity_refresh(HalfPathTable t, SampleTable s, int64_t n_hp,
     const int64_t h = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kRefreshLanes;
     const int j = threadIdx.x % kRefreshLanes;
     double act = 0;
+    int nz = 0;
     if (h < n_hp) {
         const int64_t base = t.off[h];
         const int n = t.n[h];
@@ -117,10 +112,66 @@ __global__ void k_activity_refresh(HalfPathTable t, SampleTable s, int64_t n_hp,
             s.rec[base + k].a = (float)ar;
             if (s.a64) s.a64[base + k] = ar;
             act += ar;
+            // the fold's own precision decides which terms are zero
+            nz += s.a64 ? (ar != 0.0) : ((float)ar != 0.0f);
         }
     }
-    for (int o = kRefreshLanes / 2; o > 0; o /= 2) act += __shfl_xor(act, o, kRefreshLanes);
-    if (h < n_hp && j == 0) t.act[h] = act;
+    for (int o = kRefreshLanes / 2; o > 0; o /= 2) {
+        act += __shfl_xor(act, o, kRefreshLanes);
+        nz += __shfl_xor(nz, o, kRefreshLanes);
+    }
+    if (h < n_hp && j == 0) {
+        t.act[h] = act;
+        t.n_fold[h] = nz;
+    }
+}
+
+// Copies the samples with lam L != 0 of each half path, in order, to the
+// fold table at off_fold (scanned from the padded n_fold), then pads the
+// half path to a whole group with copies of its last kept record of weight
+// 0.  One group of kRefreshLanes lanes per half path; a lane's place among
+// the kept samples of one pass comes from the wave's ballot.
+__global__ void k_fold_compact(HalfPathTable t, SampleTable s, int64_t n_hp)
+{
+    const int64_t h = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kRefreshLanes;
+    const int j = threadIdx.x % kRefreshLanes;
+    const int g0 = (threadIdx.x % kWave) - j;   // the group's first lane in the wave
+    if (h >= n_hp) return;
+    const int64_t base = t.off[h];
+    const int n = t.n[h];
+    const int64_t dst = t.off_fold[h];
+    int kept = 0;
+    SampleRec last{};
+    for (int k0 = 0; k0 < n; k0 += kRefreshLanes) {
+        const int k = k0 + j;
+        SampleRec r{};
+        double a64 = 0.0;
+        if (k < n) {
+            r = s.rec[base + k];
+            if (s.fold_a64) a64 = s.a64[base + k];
+        }
+        // the same test as the count in k_activity_refresh
+        const bool keep = k < n && (s.fold_a64 ? a64 != 0.0 : r.a != 0.0f);
+        const uint64_t wave_mask = __ballot(keep);
+        const uint32_t gm = (uint32_t)(wave_mask >> g0) & ((1u << kRefreshLanes) - 1);
+        if (keep) {
+            const int at = kept + __popc(gm & ((1u << j) - 1));
+            s.fold[dst + at] = r;
+            if (s.fold_a64) s.fold_a64[dst + at] = a64;
+        }
+        if (gm) {
+            const int src = g0 + 31 - __clz(gm);
+            last.nq = __shfl(r.nq, src, kWave);
+            last.vox = __shfl(r.vox, src, kWave);
+        }
+        kept += __popc(gm);
+    }
+    const int padded = (kept + kSampleGroup - 1) / kSampleGroup * kSampleGroup;
+    if (kept + j < padded) {
+        last.a = 0.0f;
+        s.fold[dst + kept + j] = last;
+        if (s.fold_a64) s.fold_a64[dst + kept + j] = 0.0;
+    }
 }
 
 __global__ void k_trace_segments(Grid g, const double* __restrict__ segs, int64_t n, int cap,
@@ -194,6 +245,14 @@ hipError_t launch_activity_refresh(const HalfPathTable t, SampleTable s, int64_t
     if (n_hp == 0) return hipSuccess;
     hipLaunchKernelGGL(k_activity_refresh, dim3(blocks_for(n_hp * kRefreshLanes, 256)), dim3(256), 0, st,
                        t, s, n_hp, lam);
+    return hipGetLastError();
+}
+
+hipError_t launch_fold_compact(const HalfPathTable t, SampleTable s, int64_t n_hp, hipStream_t st)
+{
+    if (n_hp == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fold_compact, dim3(blocks_for(n_hp * kRefreshLanes, 256)), dim3(256), 0, st,
+                       t, s, n_hp);
     return hipGetLastError();
 }
 
