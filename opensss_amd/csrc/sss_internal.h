@@ -23,7 +23,7 @@ constexpr int kTileRows = 8;
 constexpr int kTileCols = 8;
 constexpr int kTileMaxRings = 4;
 constexpr int kScatterWaves = 4;          // waves per workgroup of the scatter launch
-constexpr int kAutoChunkPoints = 8;       // scatter points per chunk by default
+constexpr int kAutoChunkPoints = 16;      // scatter points per chunk by default
 constexpr int kMaxBins = 32;
 
 // TOF weight evaluation of the fold (see tof_weights in sss_scatter.hip)
@@ -32,8 +32,7 @@ constexpr int kTofOneExp = 1;             // h = 1: one exp2, end bins masked
 constexpr int kTofOneExpInner = 2;        // h = 1: one exp2, window centre kept inside the bins
 // float only: kTofOneExpInner with a window that slides along a half path
 // (the centre bin moves one bin at a time, in one direction)
-constexpr int kTofSlide = 3;              // odd bin count (bin centres at integer multiples of W)
-constexpr int kTofSlideHalf = 4;          // even bin count (centres at half-integer multiples)
+constexpr int kTofSlide = 3;
 constexpr int kMaxHalfWindow = 8;
 
 void set_error(const std::string& msg);
@@ -90,14 +89,17 @@ struct alignas(16) SampleRec {
 // activity image.  fold/fold_a64: per activity image, the samples with
 // lam L != 0 only (their terms are exactly zero otherwise), offsets
 // HalfPathTable::off_fold, padded per half path to whole groups, kSamplePad
-// spare records at the end.
+// spare records at the end.  The fold table packs each group of 4 records
+// into 12 words (three 16-byte loads): nq of records 0..3 (low, high word),
+// then a of records 0..3.
 struct SampleTable {
     SampleRec* rec;
     double* len;       // intersection length, cm
     double* a64;       // lam * len in float64 (FP64 validation path only, else null)
-    SampleRec* fold;
+    uint32_t* fold;    // kFoldGroupWords per group of kSampleGroup records
     double* fold_a64;  // FP64 path only
 };
+constexpr int kFoldGroupWords = 3 * kSampleGroup;
 
 struct ScatterArgs {
     // sizes
@@ -112,7 +114,7 @@ struct ScatterArgs {
     const double* hp_lnatt;
     const double* hp_act;
     const float* hp_cos;
-    const SampleRec* s_rec;      // fold table: fixed-point TOF offsets, FP32 emission terms
+    const uint32_t* s_fold;      // fold table (packed groups): fixed-point TOF offsets, FP32 emission terms
     const double* s_a64;         // fold table FP64 emission terms (FP64 path)
     const double* detpos;        // (NRU, NDU, 3)
     const double* eff;           // n_energy
@@ -132,10 +134,14 @@ struct ScatterArgs {
     int range_check;                               // offsets may leave [t_lo, t_hi]: test each sample
     // the same constants rounded to float once on the host (uniform registers)
     float f_inv_w, f_kap, f_mid, f_jmax, f_wk, f_mid_wk, f_c, f_inv_c;
-    // sliding window: centred bin jr = bin - floor(mid) kept in [f_jlo, f_jhi];
-    // LDS row of bin jr - 1 is jr + slide_row0
-    float f_jlo, f_jhi;
+    // sliding window (kTofSlide): LDS row of bin b is b + slide_row0; per
+    // centre bin b in [1, B-2] with L(b) = ceil((b - mid - 1/2) W): the
+    // threshold of the high word below which the centre moves on the path to
+    // D1 (L(b), INT_MIN for b = 1) and to D2 (1 - L(b+1), INT_MIN for
+    // b = B-2), and kappa2 (mid - b) W
     int slide_row0;
+    int slide_thr_d1[kMaxBins], slide_thr_d2[kMaxBins];
+    float slide_kc[kMaxBins];
     double g_const;              // (Dx Dy 1e-2)^2 / (4 pi)
     double kn_gamma;
     // output: per chunk, the un-normalised sums and the pair counts of every

@@ -10,6 +10,7 @@
 #include <rocprim/rocprim.hpp>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <new>
 #include <string>
@@ -585,12 +586,13 @@ int sss_subject_trace(sss_subject* s)
         s->samples_cap = 0;
         // the fold table holds at most every sample plus the group padding,
         // and spare records after the last one (the fold prefetches ahead)
-        const size_t fold_cap = (size_t)total + (size_t)(kSampleGroup - 1) * s->n_hp + kSamplePad;
+        size_t fold_cap = (size_t)total + (size_t)(kSampleGroup - 1) * s->n_hp + kSamplePad;
+        fold_cap = (fold_cap + kSampleGroup - 1) / kSampleGroup * kSampleGroup;
         const bool fp64 = s->precision == SSS_PRECISION_FP64;
         int rc;
         if ((rc = dev_alloc(&s->smp.rec, (size_t)total, &s->device_bytes)) ||
             (rc = dev_alloc(&s->smp.len, (size_t)total, &s->device_bytes)) ||
-            (rc = dev_alloc(&s->smp.fold, fold_cap, &s->device_bytes)) ||
+            (rc = dev_alloc(&s->smp.fold, fold_cap / kSampleGroup * kFoldGroupWords, &s->device_bytes)) ||
             (fp64 && ((rc = dev_alloc(&s->smp.a64, (size_t)total, &s->device_bytes)) ||
                       (rc = dev_alloc(&s->smp.fold_a64, fold_cap, &s->device_bytes)))))
             return rc;
@@ -641,7 +643,7 @@ static ScatterArgs scatter_args(const sss_subject* s)
     a.n_tiles = s->n_tiles; a.n_chunks = s->n_chunks; a.n_cells_plane = s->n_cells_plane;
     a.hp_off = s->hp.off_fold; a.hp_n = s->hp.n_fold; a.hp_dist = s->hp.dist; a.hp_att = s->hp.att;
     a.hp_lnatt = s->hp.lnatt; a.hp_act = s->hp.act; a.hp_cos = s->hp.cosabs;
-    a.s_rec = s->smp.fold; a.s_a64 = s->smp.fold_a64;
+    a.s_fold = s->smp.fold; a.s_a64 = s->smp.fold_a64;
     a.detpos = s->d_detpos; a.eff = s->d_eff; a.ratio = s->d_ratio; a.tiles = s->d_tiles;
     a.items = s->d_items; a.n_items = s->n_items; a.tile_db_log2 = s->tile_db_log2;
     a.slice_ptr = s->d_slice_ptr; a.chunk_sp = s->d_chunk_sp;
@@ -675,12 +677,12 @@ static ScatterArgs scatter_args(const sss_subject* s)
     a.f_mid_wk = (float)(a.mid * a.bin_w * a.kappa2);
     a.f_c = (float)a.c_beta;
     a.f_inv_c = (float)std::exp2(a.beta2);
-    {
-        // centred bins jr = bin - floor(mid); the window centre stays in bins [1, B-2]
-        const int m = (s->n_bins - 1) / 2;
-        a.f_jlo = (float)(1 - m);
-        a.f_jhi = (float)(s->n_bins - 2 - m);
-        a.slide_row0 = m + s->half_window - 1;
+    a.slide_row0 = s->half_window;
+    for (int b = 0; b < kMaxBins; ++b) {
+        auto L = [&](int j) { return (int)std::ceil((j - a.mid - 0.5) * s->bin_w); };
+        a.slide_thr_d1[b] = (b > 1 && b < s->n_bins) ? L(b) : INT_MIN;
+        a.slide_thr_d2[b] = (b + 1 < s->n_bins - 1) ? 1 - L(b + 1) : INT_MIN;
+        a.slide_kc[b] = (float)((a.mid - b) * s->bin_w * a.kappa2);
     }
     a.g_const = s->g_const;
     a.kn_gamma = (511.0e3 * 1.6e-19) / (9.10938356e-31 * (299792458.0 * 299792458.0));

@@ -293,11 +293,21 @@ __device__ __forceinline__ void fold_sample(const ScatterArgs& a, const TofConst
     win.add(col, j0, w);
 }
 
+// Fold-table access: a half path's records start at a group boundary; group
+// j of the path is three 16-byte words {nq0, nq1}, {nq2, nq3}, {a0..a3}.
+__device__ __forceinline__ const uint4* fold_groups(const ScatterArgs& a, int64_t base)
+{
+    return reinterpret_cast<const uint4*>(a.s_fold) + (base / kSampleGroup) * 3;
+}
+
+__device__ __forceinline__ int64_t nq_lo(const uint4& q) { return (int64_t)(((uint64_t)q.y << 32) | q.x); }
+__device__ __forceinline__ int64_t nq_hi(const uint4& q) { return (int64_t)(((uint64_t)q.w << 32) | q.z); }
+
 // Fold the voxel samples of one half path.  SIGN = -1 (path to D1): offset
 // t = floor(X - q) = hi32(shift + nq); SIGN = +1 (path to D2): t = floor(X + q)
-// = -ceil(-X - q) = -hi32((2^32 - 1 - shift) + nq).  Two samples per trip,
-// each record loaded right after its predecessor in the same slot is used;
-// the loads run up to three records past the path (the table has spare
+// = -ceil(-X - q) = -hi32((2^32 - 1 - shift) + nq).  Whole groups of 4 (the
+// padding records weigh 0); the next group is loaded before the current one
+// is folded, and the loads run one group past the path (the table has spare
 // records after its end).  With RANGE, offsets outside the non-zero TOF
 // columns [t_lo, t_hi] weigh 0.
 template <typename R, int H, int TM, bool RANGE, int SIGN>
@@ -305,43 +315,55 @@ __device__ __forceinline__ void fold_path(const ScatterArgs& a, const TofConst<R
                                           BinWindow<R, H>& win, int64_t base, int n, int64_t shift,
                                           R coef)
 {
-    const SampleRec* __restrict__ p = a.s_rec + base;
-    const SampleRec* const end = p + n;
+    const uint4* __restrict__ g = fold_groups(a, base);
+    const double* __restrict__ ap = a.s_a64 + base;
     int64_t x = SIGN < 0 ? shift : ((int64_t)0xffffffffLL - shift);
     asm volatile("" : "+v"(x));   // keep x + nq as one 64-bit add per sample
-    SampleRec ra = p[0], rb = p[1];
-    if constexpr (sizeof(R) == 8) {
-        const double* __restrict__ ap = a.s_a64 + base;
-        double aa = ap[0], ab = ap[1];
-        for (; p < end; p += 2, ap += 2) {
-            fold_sample<R, H, TM, RANGE, SIGN>(a, d, col, win, x, ra.nq, aa, coef);
-            ra = p[2];
-            aa = ap[2];
-            if (p + 1 < end) fold_sample<R, H, TM, RANGE, SIGN>(a, d, col, win, x, rb.nq, ab, coef);
-            rb = p[3];
-            ab = ap[3];
+    uint4 c0 = g[0], c1 = g[1], c2 = g[2];
+    for (int k = n; k > 0; k -= kSampleGroup, g += 3, ap += kSampleGroup) {
+        const uint4 q01 = c0, q23 = c1, av = c2;
+        c0 = g[3];
+        c1 = g[4];
+        c2 = g[5];
+        R a0, a1, a2, a3;
+        if constexpr (sizeof(R) == 8) {
+            a0 = ap[0]; a1 = ap[1]; a2 = ap[2]; a3 = ap[3];
+        } else {
+            a0 = __uint_as_float(av.x); a1 = __uint_as_float(av.y);
+            a2 = __uint_as_float(av.z); a3 = __uint_as_float(av.w);
         }
-    } else {
-        // both loads unconditional, so the wait before a sample covers only its own record
-        for (; p < end; p += 2) {
-            fold_sample<R, H, TM, RANGE, SIGN>(a, d, col, win, x, ra.nq, ra.a, coef);
-            ra = p[2];
-            if (p + 1 < end) fold_sample<R, H, TM, RANGE, SIGN>(a, d, col, win, x, rb.nq, rb.a, coef);
-            rb = p[3];
-        }
+        fold_sample<R, H, TM, RANGE, SIGN>(a, d, col, win, x, nq_lo(q01), a0, coef);
+        fold_sample<R, H, TM, RANGE, SIGN>(a, d, col, win, x, nq_hi(q01), a1, coef);
+        fold_sample<R, H, TM, RANGE, SIGN>(a, d, col, win, x, nq_lo(q23), a2, coef);
+        fold_sample<R, H, TM, RANGE, SIGN>(a, d, col, win, x, nq_hi(q23), a3, coef);
     }
 }
 
-// Sliding window of the float h = 1 fold (kTofSlide, kTofSlideHalf) over
-// the current half path, without the path's coefficient (applied when a bin
-// leaves the window): bin cur - 1 holds c A, bin cur holds M and bin cur + 1
-// holds c P, c = 2^-beta2 (the side bins' common factor is applied when they
-// are added into LDS or become the centre).  wp: LDS row of bin cur - 1 in
-// the lane's column.
+// Sliding window of the float h = 1 fold (kTofSlide) over the current half
+// path, without the path's coefficient (applied when a bin leaves the
+// window): bin cur - 1 holds c A, bin cur holds M and bin cur + 1 holds c P,
+// c = 2^-beta2 (the side bins' common factor is applied when they are added
+// into LDS or become the centre).
+//
+// The centre is the bin nearest to t, kept in [1, B-2]: bin b owns the
+// integer offsets [L(b), L(b+1)), L(b) = ceil((b - mid - 1/2) W) (a tie may
+// go either way: both nearest bins are in the window).  Along a half path t
+// only moves one way (s_k never decreases), so the centre only changes once
+// the high word of the fixed-point offset drops below one integer
+// threshold: one integer compare per sample.  The thresholds and ku offsets
+// of every centre are tables in LDS (SlideTables, filled from ScatterArgs).
 struct SlideWindow {
     float A, M, P;
-    float cur;     // centred centre bin jr (bin jr + floor((B-1)/2))
-    float* wp;
+    int cur;       // centre bin
+    int thr;       // the centre changes once hi < thr (INT_MIN: it cannot)
+    float kc;      // kappa2 (mid - cur) W: ku = kappa2 t + kc
+    float* wp;     // LDS row of bin cur - 1 in the lane's column
+};
+
+struct SlideTables {
+    int thr_d1[kMaxBins];   // path to D1 (t = hi falls): L(b) for b > 1, else INT_MIN
+    int thr_d2[kMaxBins];   // path to D2 (t = -hi rises): 1 - L(b+1) for b < B-2, else INT_MIN
+    float kc[kMaxBins];     // kappa2 (mid - b) W
 };
 
 // Adds the window, times the path coefficient (ccoef = c coef), into its
@@ -354,24 +376,28 @@ __device__ __forceinline__ void slide_flush(SlideWindow& win, float coef, float 
     win.A = win.M = win.P = 0.0f;
 }
 
-// One sample: x = t / W + (mid - floor(mid)), centre jr = rint(x) clamped to
-// [1, B-2] (centred), ku = kappa2 (t - s_j0) = W kappa2 (x - jr), E = 2^ku,
-// D = c E^2 + E + c; the weights of bins jr - 1, jr, jr + 1 are a c/D,
-// a E/D and a c E^2/D (tof_weights' kTofOneExpInner).  t is monotone along a
-// half path (s_k never decreases) and a voxel step moves it by far less than
-// a bin, so a new centre is nearly always cur - 1 on the path to D1 and
-// cur + 1 on the path to D2: the bin leaving the window is added into LDS and
-// the other two shift.  Any other move flushes the whole window.
-template <int SIGN, bool HALF>
-__device__ __forceinline__ void slide_sample(const ScatterArgs& a, const TofConst<float>& d, float jlo,
-                                             float jhi, float* col, SlideWindow& win, int64_t x, int64_t nq,
-                                             float av, float coef, float ccoef, float sw)
+template <int SIGN>
+__device__ __forceinline__ int slide_thr(const SlideTables* tb, int b)
 {
-    const float tf = (float)(int)((x + nq) >> 32);   // |t| (sign in sw)
-    const float xf = HALF ? __builtin_fmaf(tf, sw, 0.5f) : tf * sw;
-    const float jr = clampr(rintf(xf), jlo, jhi);
-    if (jr != win.cur) {
-        if (jr == win.cur + (float)SIGN) {
+    return SIGN < 0 ? tb->thr_d1[b] : tb->thr_d2[b];
+}
+
+// One sample.  ku = kappa2 (t - s_cur), E = 2^ku, D = c E^2 + E + c; the
+// weights of bins cur - 1, cur, cur + 1 are a c/D, a E/D and a c E^2/D
+// (tof_weights' kTofOneExpInner).  A centre move by one bin (nearly every
+// move: a voxel step moves t by far less than a bin) adds the bin leaving the
+// window into LDS and shifts the other two; a longer move (after a run of
+// zero terms) flushes the window.
+template <int SIGN>
+__device__ __forceinline__ void slide_sample(const ScatterArgs& a, const TofConst<float>& d,
+                                             const SlideTables* tb, float* col, SlideWindow& win, int64_t x,
+                                             int64_t nq, float av, float coef, float ccoef, float skap)
+{
+    const int hi = (int)((x + nq) >> 32);
+    if (hi < win.thr) {
+        int b = win.cur + SIGN;
+        int thr = slide_thr<SIGN>(tb, b);
+        if (hi >= thr) {
             if constexpr (SIGN < 0) {
                 lds_add(win.wp + 2 * kWave, win.P * ccoef);
                 win.P = win.M * d.inv_c;
@@ -386,12 +412,18 @@ __device__ __forceinline__ void slide_sample(const ScatterArgs& a, const TofCons
                 win.wp += kWave;
             }
         } else {
+            do {
+                b += SIGN;
+                thr = slide_thr<SIGN>(tb, b);
+            } while (hi < thr);
             slide_flush(win, coef, ccoef);
-            win.wp = col + ((int)jr + a.slide_row0) * kWave;
+            win.wp = col + (b - 1 + a.slide_row0) * kWave;
         }
-        win.cur = jr;
+        win.cur = b;
+        win.thr = thr;
+        win.kc = tb->kc[b];
     }
-    const float ku = (xf - jr) * d.wk;
+    const float ku = __builtin_fmaf((float)hi, skap, win.kc);
     const float E = fast_exp2(ku);
     const float D = __builtin_fmaf(E, __builtin_fmaf(d.c, E, 1.0f), d.c);
     const float g = av * fast_rcp(D);
@@ -401,46 +433,68 @@ __device__ __forceinline__ void slide_sample(const ScatterArgs& a, const TofCons
     win.P = __builtin_fmaf(gE, E, win.P);
 }
 
-// fold_path for the sliding window.  The path's records are padded to a
-// multiple of 4 with zero-weight copies of its last record, so the loop runs
-// whole groups of 4 without tests; each record is loaded 4 samples before its
-// use.  The window is placed at the path's first sample and added into LDS at
-// its end.
-template <int SIGN, bool HALF>
-__device__ __forceinline__ void slide_path(const ScatterArgs& a, const TofConst<float>& d, float* __restrict__ col,
-                                           SlideWindow& win, int64_t base, int n, int64_t shift, float coef)
+// Centre bin of the path's first sample (offset t, high word hi): the float
+// estimate, then the thresholds.
+template <int SIGN>
+__device__ __forceinline__ int slide_first(const ScatterArgs& a, const SlideTables* tb, int hi)
 {
-    const SampleRec* __restrict__ p = a.s_rec + base;
+    const int t = SIGN < 0 ? hi : -hi;
+    int b = (int)rintf((float)t * a.f_inv_w + a.f_mid);
+    b = min(max(b, 1), a.n_bins - 2);
+    // move the centre towards the path's direction of travel while t is past
+    // its threshold, and back while the previous centre's threshold holds
+    while (hi < slide_thr<SIGN>(tb, b)) b += SIGN;
+    while (b - SIGN >= 1 && b - SIGN <= a.n_bins - 2 && hi >= slide_thr<SIGN>(tb, b - SIGN)) b -= SIGN;
+    return b;
+}
+
+// fold_path for the sliding window.  The path's records come in whole
+// groups of 4 (the padding records weigh 0), so the loop has no tests; two
+// groups are in registers, the one being folded and the next, and the group
+// after is loaded once one is done (loads run two groups past the path).
+// The window is placed at the path's first sample and added into LDS at its
+// end.
+template <int SIGN>
+__device__ __forceinline__ void slide_group(const ScatterArgs& a, const TofConst<float>& d, const SlideTables* tb,
+                                            float* col, SlideWindow& win, int64_t x, const uint4& q01,
+                                            const uint4& q23, const uint4& av, float coef, float ccoef, float skap)
+{
+    slide_sample<SIGN>(a, d, tb, col, win, x, nq_lo(q01), __uint_as_float(av.x), coef, ccoef, skap);
+    slide_sample<SIGN>(a, d, tb, col, win, x, nq_hi(q01), __uint_as_float(av.y), coef, ccoef, skap);
+    slide_sample<SIGN>(a, d, tb, col, win, x, nq_lo(q23), __uint_as_float(av.z), coef, ccoef, skap);
+    slide_sample<SIGN>(a, d, tb, col, win, x, nq_hi(q23), __uint_as_float(av.w), coef, ccoef, skap);
+}
+
+template <int SIGN>
+__device__ __forceinline__ void slide_path(const ScatterArgs& a, const TofConst<float>& d, const SlideTables* tb,
+                                           float* __restrict__ col, SlideWindow& win, int64_t base, int n,
+                                           int64_t shift, float coef)
+{
+    const uint4* __restrict__ g = fold_groups(a, base);
     int64_t x = SIGN < 0 ? shift : ((int64_t)0xffffffffLL - shift);
     asm volatile("" : "+v"(x));   // keep x + nq as one 64-bit add per sample
-    const float sw = SIGN < 0 ? d.inv_w : -d.inv_w;
-    const float jlo = a.f_jlo;
-    float jhi = a.f_jhi;
-    asm volatile("" : "+v"(jhi));   // med3 takes one scalar operand
+    const float skap = SIGN < 0 ? d.kap : -d.kap;
     const float ccoef = coef * d.c;
-    SampleRec r0 = p[0], r1 = p[1], r2 = p[2], r3 = p[3];
+    uint4 c0 = g[0], c1 = g[1], c2 = g[2];
+    uint4 n0 = g[3], n1 = g[4], n2 = g[5];
     {
-        const float tf = (float)(int)((x + r0.nq) >> 32);
-        const float xf = HALF ? __builtin_fmaf(tf, sw, 0.5f) : tf * sw;
-        win.cur = clampr(rintf(xf), jlo, jhi);
-        win.wp = col + ((int)win.cur + a.slide_row0) * kWave;
+        const int hi = (int)((x + nq_lo(c0)) >> 32);
+        win.cur = slide_first<SIGN>(a, tb, hi);
+        win.thr = slide_thr<SIGN>(tb, win.cur);
+        win.kc = tb->kc[win.cur];
+        win.wp = col + (win.cur - 1 + a.slide_row0) * kWave;
     }
-    // Each empty asm statement comes after the previous load (memory clobber)
-    // and takes the next sample's offset, so that sample waits for its record
-    // only after the load is issued: four loads stay in flight.
-    for (int k = n; k > 0; k -= 4, p += 4) {
-        slide_sample<SIGN, HALF>(a, d, jlo, jhi, col, win, x, r0.nq, r0.a, coef, ccoef, sw);
-        r0 = p[4];
-        asm volatile("" : "+v"(r1.nq) : : "memory");
-        slide_sample<SIGN, HALF>(a, d, jlo, jhi, col, win, x, r1.nq, r1.a, coef, ccoef, sw);
-        r1 = p[5];
-        asm volatile("" : "+v"(r2.nq) : : "memory");
-        slide_sample<SIGN, HALF>(a, d, jlo, jhi, col, win, x, r2.nq, r2.a, coef, ccoef, sw);
-        r2 = p[6];
-        asm volatile("" : "+v"(r3.nq) : : "memory");
-        slide_sample<SIGN, HALF>(a, d, jlo, jhi, col, win, x, r3.nq, r3.a, coef, ccoef, sw);
-        r3 = p[7];
-        asm volatile("" : "+v"(r0.nq) : : "memory");
+    for (int k = n;; ) {
+        slide_group<SIGN>(a, d, tb, col, win, x, c0, c1, c2, coef, ccoef, skap);
+        c0 = g[6]; c1 = g[7]; c2 = g[8];
+        k -= kSampleGroup;
+        g += 3;
+        if (k <= 0) break;
+        slide_group<SIGN>(a, d, tb, col, win, x, n0, n1, n2, coef, ccoef, skap);
+        n0 = g[6]; n1 = g[7]; n2 = g[8];
+        k -= kSampleGroup;
+        g += 3;
+        if (k <= 0) break;
     }
     slide_flush(win, coef, ccoef);
 }
@@ -452,8 +506,21 @@ template <typename R, int HM, int TM, bool TRIVIAL, bool RANGE>
 __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArgs a)
 {
     extern __shared__ unsigned char smem_raw[];
+    constexpr bool SLIDE = TM == kTofSlide;
+    static_assert(!SLIDE || (sizeof(R) == 4 && HM == 1 && !RANGE), "sliding window: float, h = 1, no range test");
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
+    // LDS: the sliding window's tables (shared), then one column per lane
+    SlideTables* tb = reinterpret_cast<SlideTables*>(smem_raw);
+    if constexpr (SLIDE) {
+        const int i = threadIdx.x;
+        if (i < kMaxBins) {
+            tb->thr_d1[i] = a.slide_thr_d1[i];
+            tb->thr_d2[i] = a.slide_thr_d2[i];
+            tb->kc[i] = a.slide_kc[i];
+        }
+        __syncthreads();   // every wave of the workgroup, before any exits
+    }
     // (tile, chunk) of this wave; padding items (tile -1) exit at once
     const int2 item = a.items[blockIdx.x * kScatterWaves + wave];
     if (item.x < 0) return;
@@ -461,7 +528,8 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
     const int chunk = item.y;
     const int B = a.n_bins;
     const int rows = B + 2 * HM;   // bins -HM .. B-1+HM (the window's reach)
-    R* col = reinterpret_cast<R*>(smem_raw) + (size_t)wave * rows * kWave + lane;
+    R* col = reinterpret_cast<R*>(smem_raw + (SLIDE ? sizeof(SlideTables) : 0)) +
+             (size_t)wave * rows * kWave + lane;
     if constexpr (!TRIVIAL) {
         for (int r = 0; r < rows; ++r) col[r * kWave] = (R)0;
     }
@@ -481,8 +549,6 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
 
     R num = (R)0;   // one-bin path
     float cnt = 0.0f;
-    constexpr bool SLIDE = TM == kTofSlide || TM == kTofSlideHalf;
-    static_assert(!SLIDE || (sizeof(R) == 4 && HM == 1 && !RANGE), "sliding window: float, h = 1, no range test");
     BinWindow<R, HM> win;
     win.reset((R)0);
     SlideWindow sl;
@@ -504,9 +570,8 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
             if constexpr (TRIVIAL) {
                 num += p.coef1 * p.act1 + p.coef2 * p.act2;
             } else if constexpr (SLIDE) {
-                constexpr bool HALF = TM == kTofSlideHalf;
-                if (p.n1 > 0) slide_path<-1, HALF>(a, tc, col, sl, p.b1, p.n1, p.shift, p.coef1);
-                if (p.n2 > 0) slide_path<+1, HALF>(a, tc, col, sl, p.b2, p.n2, p.shift, p.coef2);
+                if (p.n1 > 0) slide_path<-1>(a, tc, tb, col, sl, p.b1, p.n1, p.shift, p.coef1);
+                if (p.n2 > 0) slide_path<+1>(a, tc, tb, col, sl, p.b2, p.n2, p.shift, p.coef2);
             } else if (a.t_hi >= a.t_lo) {   // else every TOF weight is zero
                 fold_path<R, HM, TM, RANGE, -1>(a, tc, col, win, p.b1, p.n1, p.shift, p.coef1);
                 fold_path<R, HM, TM, RANGE, +1>(a, tc, col, win, p.b2, p.n2, p.shift, p.coef2);
@@ -669,6 +734,7 @@ hipError_t launch_variant(const ScatterArgs& a, hipStream_t st)
     // one workgroup per kScatterWaves work items (the item list is padded to that)
     dim3 grid((unsigned)(a.n_items / kScatterWaves));
     size_t lds = TRIVIAL ? 0 : (size_t)kScatterWaves * (a.n_bins + 2 * HM) * kWave * sizeof(R);
+    if (TM == kTofSlide) lds += sizeof(SlideTables);
     hipLaunchKernelGGL((k_scatter<R, HM, TM, TRIVIAL, RANGE>), grid,
                        dim3(kScatterWaves * kWave), lds, st, a);
     return hipGetLastError();
@@ -683,8 +749,7 @@ hipError_t launch_hm(const ScatterArgs& a, hipStream_t st)
         if constexpr (!RANGE) {
             if (a.tof_mode == kTofOneExpInner) {
                 if constexpr (sizeof(R) == 4)
-                    return (a.n_bins % 2) ? launch_variant<R, 1, kTofSlide, false, false>(a, st)
-                                          : launch_variant<R, 1, kTofSlideHalf, false, false>(a, st);
+                    return launch_variant<R, 1, kTofSlide, false, false>(a, st);
                 else
                     return launch_variant<R, 1, kTofOneExpInner, false, false>(a, st);
             }

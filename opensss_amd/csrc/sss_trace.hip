@@ -126,6 +126,16 @@ __global__ void k_activity_refresh(HalfPathTable t, SampleTable s, int64_t n_hp,
     }
 }
 
+// Record r of the fold table (packed groups of kSampleGroup records).
+__device__ __forceinline__ void fold_store(SampleTable s, int64_t r, int64_t nq, float a)
+{
+    uint32_t* g = s.fold + (r / kSampleGroup) * kFoldGroupWords;
+    const int i = (int)(r % kSampleGroup);
+    g[2 * i] = (uint32_t)nq;
+    g[2 * i + 1] = (uint32_t)((uint64_t)nq >> 32);
+    g[2 * kSampleGroup + i] = __float_as_uint(a);
+}
+
 // Copies the samples with lam L != 0 of each half path, in order, to the
 // fold table at off_fold (scanned from the padded n_fold), then pads the
 // half path to a whole group with copies of its last kept record of weight
@@ -156,7 +166,7 @@ __global__ void k_fold_compact(HalfPathTable t, SampleTable s, int64_t n_hp)
         const uint32_t gm = (uint32_t)(wave_mask >> g0) & ((1u << kRefreshLanes) - 1);
         if (keep) {
             const int at = kept + __popc(gm & ((1u << j) - 1));
-            s.fold[dst + at] = r;
+            fold_store(s, dst + at, r.nq, r.a);
             if (s.fold_a64) s.fold_a64[dst + at] = a64;
         }
         if (gm) {
@@ -168,8 +178,7 @@ __global__ void k_fold_compact(HalfPathTable t, SampleTable s, int64_t n_hp)
     }
     const int padded = (kept + kSampleGroup - 1) / kSampleGroup * kSampleGroup;
     if (kept + j < padded) {
-        last.a = 0.0f;
-        s.fold[dst + kept + j] = last;
+        fold_store(s, dst + kept + j, last.nq, 0.0f);
         if (s.fold_a64) s.fold_a64[dst + kept + j] = 0.0;
     }
 }
