@@ -63,7 +63,23 @@ struct HalfPathTable {
     double* act;       // sum lam L (f64, exact-zero tests)
     float* cosabs;     // |cos| of the angle to the detector normal
     int32_t* status;   // 0 ok, 1 walk overflow
+    struct PairHalf* rec;   // per activity image: what the FP32 pair evaluation reads
 };
+
+// One half path as the FP32 pair evaluation reads it: two 16-byte loads.
+// The zero tests of the float64 attenuation and activity are kept as flags,
+// so they stay exact.
+struct alignas(16) PairHalf {
+    double dist;       // |D - S|, cm
+    int64_t off;       // first fold record
+    int32_t n_flags;   // fold records | kHalfAttZero | kHalfActive
+    float cosabs;
+    float lnatt;       // sum mu L
+    float att;         // exp(-sum mu L)
+};
+constexpr int32_t kHalfAttZero = 1 << 30;   // exp(-sum mu L) == 0
+constexpr int32_t kHalfActive = 1 << 29;    // sum lam L != 0
+constexpr int32_t kHalfCountMask = kHalfActive - 1;
 
 // TOF time offsets are carried in 32.32 fixed point: one 64-bit add and the
 // high word give floor(X - q) exactly for X, q quantised to 2^-32 ps.  The
@@ -107,6 +123,7 @@ struct ScatterArgs {
     int n_tiles, n_chunks;
     int64_t n_cells_plane;       // NDU (NDU+1)/2
     // tables
+    const PairHalf* hp_rec;      // per half path, FP32 pair evaluation
     const int64_t* hp_off;       // first fold record of each half path
     const int32_t* hp_n;         // fold records (samples with lam L != 0)
     const double* hp_dist;
@@ -173,6 +190,7 @@ hipError_t launch_halfpath_fill(const Grid& g, const double* sp_xyz, int n_sp, c
 hipError_t launch_activity_refresh(const HalfPathTable t, SampleTable s, int64_t n_hp,
                                    const double* lam, hipStream_t st);
 hipError_t launch_fold_compact(const HalfPathTable t, SampleTable s, int64_t n_hp, hipStream_t st);
+hipError_t launch_pair_half(const HalfPathTable t, int64_t n_hp, hipStream_t st);
 hipError_t launch_trace_segments(const Grid& g, const double* segs, int64_t n, int cap,
                                  int32_t* counts, double* L, int32_t* idx, double* s,
                                  hipStream_t st);

@@ -197,7 +197,7 @@ struct sss_subject {
         dev_free(d_mu); dev_free(d_lam); dev_free(d_sp); dev_free(d_detpos); dev_free(d_detnorm);
         dev_free(d_eff); dev_free(d_ratio); dev_free(d_tiles); dev_free(d_slice_ptr);
         dev_free(d_chunk_sp); dev_free(d_chunk_slice); dev_free(d_dets); dev_free(d_flag); dev_free(d_scan_tmp);
-        dev_free(hp.off); dev_free(hp.n); dev_free(hp.off_fold); dev_free(hp.n_fold);
+        dev_free(hp.off); dev_free(hp.n); dev_free(hp.off_fold); dev_free(hp.n_fold); dev_free(hp.rec);
         dev_free(hp.dist); dev_free(hp.att); dev_free(hp.lnatt);
         dev_free(hp.act); dev_free(hp.cosabs); dev_free(hp.status);
         dev_free(smp.rec); dev_free(smp.len); dev_free(smp.a64); dev_free(smp.fold); dev_free(smp.fold_a64);
@@ -475,6 +475,7 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
         (rc = dev_alloc(&s->hp.off, (size_t)s->n_hp + 1, &t)) || (rc = dev_alloc(&s->hp.n, (size_t)s->n_hp + 1, &t)) ||
         (rc = dev_alloc(&s->hp.off_fold, (size_t)s->n_hp + 1, &t)) ||
         (rc = dev_alloc(&s->hp.n_fold, (size_t)s->n_hp + 1, &t)) ||
+        (rc = dev_alloc(&s->hp.rec, (size_t)s->n_hp + 1, &t)) ||
         (rc = dev_alloc(&s->hp.dist, (size_t)s->n_hp + 1, &t)) || (rc = dev_alloc(&s->hp.att, (size_t)s->n_hp + 1, &t)) ||
         (rc = dev_alloc(&s->hp.lnatt, (size_t)s->n_hp + 1, &t)) || (rc = dev_alloc(&s->hp.act, (size_t)s->n_hp + 1, &t)) ||
         (rc = dev_alloc(&s->hp.cosabs, (size_t)s->n_hp + 1, &t)) ||
@@ -557,6 +558,7 @@ static int refresh_fold(sss_subject* s)
     const auto padded = rocprim::make_transform_iterator(s->hp.n_fold, PaddedRecords{});
     if (int rc = scan_counts(s, padded, s->hp.off_fold, st)) return rc;
     SSS_HIP(launch_fold_compact(s->hp, s->smp, s->n_hp, st));
+    SSS_HIP(launch_pair_half(s->hp, s->n_hp, st));
     return SSS_OK;
 }
 
@@ -641,6 +643,7 @@ static ScatterArgs scatter_args(const sss_subject* s)
     a.n_sp = s->n_sp; a.nru = s->nru; a.ndu = s->ndu; a.n_bins = s->n_bins;
     a.n_energy = s->n_energy; a.eff_ref_row = s->eff_ref_row;
     a.n_tiles = s->n_tiles; a.n_chunks = s->n_chunks; a.n_cells_plane = s->n_cells_plane;
+    a.hp_rec = s->hp.rec;
     a.hp_off = s->hp.off_fold; a.hp_n = s->hp.n_fold; a.hp_dist = s->hp.dist; a.hp_att = s->hp.att;
     a.hp_lnatt = s->hp.lnatt; a.hp_act = s->hp.act; a.hp_cos = s->hp.cosabs;
     a.s_fold = s->smp.fold; a.s_a64 = s->smp.fold_a64;

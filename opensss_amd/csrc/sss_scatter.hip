@@ -146,6 +146,47 @@ __device__ __forceinline__ bool pair_eval(const ScatterArgs& a, int sp, int d1, 
     return true;
 }
 
+// pair_eval for the FP32 fold, from the two half paths' PairHalf records
+// (two 16-byte loads each, then the energy tables): the same filters and
+// the same float terms.
+__device__ __forceinline__ bool pair_eval_rec(const ScatterArgs& a, int sp, int d1, int d2, double lor2,
+                                              double eff_ref, Pair<float>& p)
+{
+    const int64_t per_sp = (int64_t)a.nru * a.ndu;
+    const PairHalf r1 = a.hp_rec[sp * per_sp + d1];
+    const PairHalf r2 = a.hp_rec[sp * per_sp + d2];
+    const int32_t f = r1.n_flags | r2.n_flags;
+    if ((f & kHalfAttZero) || !(f & kHalfActive)) return false;
+    const double dist1 = r1.dist, dist2 = r2.dist;
+    const double s1q = dist1 * dist1, s2q = dist2 * dist2;
+    double carg = (s1q + s2q - lor2) / (2.0 * sqrt(s1q * s2q));
+    carg = carg < -1.0 ? -1.0 : (carg > 1.0 ? 1.0 : carg);
+    const double e_kev = floor(511.0 / (2.0 + carg) + 0.5);
+    int erow = (int)(e_kev * 2.0) - 1;
+    erow = erow < 0 ? 0 : (erow >= a.n_energy ? a.n_energy - 1 : erow);
+    const double eff = a.eff[erow] * eff_ref;
+    if (eff == 0.0) return false;
+    const float ratio = a.ratio[erow];
+    const float log2e = 1.4426950408889634f;
+    const float a1s = fast_exp2(-ratio * r1.lnatt * log2e);
+    const float a2s = fast_exp2(-ratio * r2.lnatt * log2e);
+    const float c = (float)carg;
+    const float rho = 1.0f / (1.0f + (float)a.kn_gamma * (1.0f + c));
+    const float kn = rho * rho * (rho + 1.0f / rho - (1.0f - c * c));
+    const float geo = (float)a.g_const * r1.cosabs * r2.cosabs / ((float)s1q * (float)s2q);
+    const float pre = geo * (float)eff * kn;
+    p.coef1 = pre * r1.att * a2s;
+    p.coef2 = pre * r2.att * a1s;
+    p.shift = (int64_t)rint(((dist1 - dist2) * (1.0 / kLightCmPerPs) + 0.5) * kFixOne);
+    p.n1 = (r1.n_flags & kHalfActive) ? (r1.n_flags & kHalfCountMask) : 0;
+    p.n2 = (r2.n_flags & kHalfActive) ? (r2.n_flags & kHalfCountMask) : 0;
+    p.b1 = r1.off;
+    p.b2 = r2.off;
+    p.act1 = p.act2 = 0.0f;   // the one-bin path reads pair_eval's
+    p.counts = ((r1.n_flags & r2.n_flags) & kHalfActive) ? 2.0f : 1.0f;
+    return true;
+}
+
 // TOF kernel constants in R (for float: rounded once on the host, so they
 // stay in scalar registers).
 template <typename R>
@@ -560,7 +601,8 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
         Pair<R> p;
         bool found = false;
         while (cursor < p_end) {
-            found = pair_eval<R>(a, cursor, d1, d2, lor2, eff_ref, p);
+            if constexpr (sizeof(R) == 4 && !TRIVIAL) found = pair_eval_rec(a, cursor, d1, d2, lor2, eff_ref, p);
+            else found = pair_eval<R>(a, cursor, d1, d2, lor2, eff_ref, p);
             ++cursor;
             if (found) break;
         }
@@ -617,7 +659,7 @@ __global__ __launch_bounds__(256) void k_pair_stats(ScatterArgs a, unsigned long
     const int p_end = a.chunk_sp[a.n_chunks];
     for (int sp = a.chunk_sp[0]; sp < p_end; ++sp) {
         Pair<float> p;
-        if (!pair_eval<float>(a, sp, d1, d2, lor2, eff_ref, p)) continue;
+        if (!pair_eval_rec(a, sp, d1, d2, lor2, eff_ref, p)) continue;
         ++pairs;
         samples += (unsigned long long)(p.n1 + p.n2);
     }

@@ -257,6 +257,30 @@ hipError_t launch_activity_refresh(const HalfPathTable t, SampleTable s, int64_t
     return hipGetLastError();
 }
 
+// The FP32 pair evaluation's record of each half path (after the fold
+// table's offsets are scanned).
+__global__ void k_pair_half(HalfPathTable t, int64_t n_hp)
+{
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= n_hp) return;
+    PairHalf r;
+    r.dist = t.dist[h];
+    r.off = t.off_fold[h];
+    const double att = t.att[h];
+    r.n_flags = t.n_fold[h] | (att == 0.0 ? kHalfAttZero : 0) | (t.act[h] != 0.0 ? kHalfActive : 0);
+    r.cosabs = t.cosabs[h];
+    r.lnatt = (float)t.lnatt[h];
+    r.att = (float)att;
+    t.rec[h] = r;
+}
+
+hipError_t launch_pair_half(const HalfPathTable t, int64_t n_hp, hipStream_t st)
+{
+    if (n_hp == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pair_half, dim3(blocks_for(n_hp, 256)), dim3(256), 0, st, t, n_hp);
+    return hipGetLastError();
+}
+
 hipError_t launch_fold_compact(const HalfPathTable t, SampleTable s, int64_t n_hp, hipStream_t st)
 {
     if (n_hp == 0) return hipSuccess;
