@@ -19,7 +19,9 @@
  *   sss_subject_output_dense  the dense (B, N+1, N/2, NRU^2) layout of :171,551
  *   sss_interpolate        Python/openSSS/Interpolation.py:95-399
  *   sss_line_integrals     Python/openSSS/TailsMask.py:84-163,230-309 traces
- *   sss_backproject_fit    Python/openSSS/BackProjection.py:31-392
+ *   sss_tail_mask          Python/openSSS/TailsMask.py:28-376 (the per-LOR mask loops)
+ *   sss_tail_fit           Python/openSSS/BackProjection.py:31-392 (back-projections
+ *                          and the scale of Scaling.py:27-121)
  */
 #ifndef SSS_HIP_H
 #define SSS_HIP_H
