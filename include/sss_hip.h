@@ -83,7 +83,8 @@ typedef struct {
 
 typedef struct {
     int64_t n_halfpaths;
-    int64_t n_samples;           /* voxel samples over all half paths */
+    int64_t n_samples;           /* sample records: each block of 64 half paths holds 64 x its
+                                    longest walk */
     int64_t n_cells;             /* compact cells per bin = NRU^2 * NDU(NDU+1)/2 */
     int64_t n_pairs_enumerated;  /* n_sp * LORs per scatter point */
     int64_t evals;               /* n_pairs_enumerated * n_bins */

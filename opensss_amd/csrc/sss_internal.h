@@ -53,8 +53,10 @@ struct Grid {
 
 // Per-half-path scalars (structure of arrays, one entry per (sp, ring, det)).
 struct HalfPathTable {
-    int64_t* off;      // first sample
+    int64_t* off;      // first sample; sample k at off + k * kWave (see SampleTable)
     int32_t* n;        // samples
+    int32_t* blk;      // per block of kWave half paths: kWave * its largest n
+    int64_t* blk_off;  // per block: its first record (scan of blk)
     int64_t* off_fold; // first record in the fold table
     int32_t* n_fold;   // samples with lam L != 0 (the ones the fold visits)
     double* dist;      // |D - S|, cm
@@ -100,9 +102,11 @@ struct alignas(16) SampleRec {
     int32_t vox;       // 0-based voxel (reference linear index - 1)
 };
 
-// Per-sample arrays.  rec/len/a64: every voxel sample of every half path
-// (offsets HalfPathTable::off), built by the tracer; a and a64 rewritten per
-// activity image.  fold/fold_a64: per activity image, the samples with
+// Per-sample arrays.  rec/len/a64: every voxel sample of every half path,
+// built by the tracer; a and a64 rewritten per activity image.  The half
+// paths h of one block of kWave (h / kWave) are interleaved: sample k of h is
+// record blk_off[h / kWave] + k * kWave + h % kWave, so a wave that walks the
+// block's paths in step writes and reads kWave consecutive records.  fold/fold_a64: per activity image, the samples with
 // lam L != 0 only (their terms are exactly zero otherwise), offsets
 // HalfPathTable::off_fold, padded per half path to whole groups, kSamplePad
 // spare records at the end.  The fold table packs each group of 4 records

@@ -198,6 +198,7 @@ struct sss_subject {
         dev_free(d_eff); dev_free(d_ratio); dev_free(d_tiles); dev_free(d_slice_ptr);
         dev_free(d_chunk_sp); dev_free(d_chunk_slice); dev_free(d_dets); dev_free(d_flag); dev_free(d_scan_tmp);
         dev_free(hp.off); dev_free(hp.n); dev_free(hp.off_fold); dev_free(hp.n_fold); dev_free(hp.rec);
+        dev_free(hp.blk); dev_free(hp.blk_off);
         dev_free(hp.dist); dev_free(hp.att); dev_free(hp.lnatt);
         dev_free(hp.act); dev_free(hp.cosabs); dev_free(hp.status);
         dev_free(smp.rec); dev_free(smp.len); dev_free(smp.a64); dev_free(smp.fold); dev_free(smp.fold_a64);
@@ -476,6 +477,8 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
         (rc = dev_alloc(&s->hp.off_fold, (size_t)s->n_hp + 1, &t)) ||
         (rc = dev_alloc(&s->hp.n_fold, (size_t)s->n_hp + 1, &t)) ||
         (rc = dev_alloc(&s->hp.rec, (size_t)s->n_hp + 1, &t)) ||
+        (rc = dev_alloc(&s->hp.blk, (size_t)(s->n_hp + kWave - 1) / kWave + 1, &t)) ||
+        (rc = dev_alloc(&s->hp.blk_off, (size_t)(s->n_hp + kWave - 1) / kWave + 1, &t)) ||
         (rc = dev_alloc(&s->hp.dist, (size_t)s->n_hp + 1, &t)) || (rc = dev_alloc(&s->hp.att, (size_t)s->n_hp + 1, &t)) ||
         (rc = dev_alloc(&s->hp.lnatt, (size_t)s->n_hp + 1, &t)) || (rc = dev_alloc(&s->hp.act, (size_t)s->n_hp + 1, &t)) ||
         (rc = dev_alloc(&s->hp.cosabs, (size_t)s->n_hp + 1, &t)) ||
@@ -528,13 +531,13 @@ int sss_subject_destroy(sss_subject* s)
     return SSS_OK;
 }
 
-// Exclusive scan of n_hp + 1 per-half-path record counts (slot n_hp is 0,
-// so out[n_hp] is the total) into int64 offsets.
+// Exclusive scan of n_items + 1 record counts (slot n_items is 0, so
+// out[n_items] is the total) into int64 offsets.
 extern "C++" template <typename It>
-static int scan_counts(sss_subject* s, It counts, int64_t* out, hipStream_t st)
+static int scan_counts(sss_subject* s, It counts, int64_t* out, int64_t n_items, hipStream_t st)
 {
     size_t need = 0;
-    SSS_HIP(rocprim::exclusive_scan(nullptr, need, counts, out, (int64_t)0, (size_t)s->n_hp + 1,
+    SSS_HIP(rocprim::exclusive_scan(nullptr, need, counts, out, (int64_t)0, (size_t)n_items + 1,
                                     rocprim::plus<int64_t>(), st));
     if (need > s->scan_tmp_bytes) {
         dev_free(s->d_scan_tmp);
@@ -544,7 +547,7 @@ static int scan_counts(sss_subject* s, It counts, int64_t* out, hipStream_t st)
         s->d_scan_tmp = tmp;
         s->scan_tmp_bytes = need;
     }
-    SSS_HIP(rocprim::exclusive_scan(s->d_scan_tmp, need, counts, out, (int64_t)0, (size_t)s->n_hp + 1,
+    SSS_HIP(rocprim::exclusive_scan(s->d_scan_tmp, need, counts, out, (int64_t)0, (size_t)n_items + 1,
                                     rocprim::plus<int64_t>(), st));
     return SSS_OK;
 }
@@ -556,7 +559,7 @@ static int refresh_fold(sss_subject* s)
     hipStream_t st = s->ctx->stream;
     SSS_HIP(launch_activity_refresh(s->hp, s->smp, s->n_hp, s->d_lam, st));
     const auto padded = rocprim::make_transform_iterator(s->hp.n_fold, PaddedRecords{});
-    if (int rc = scan_counts(s, padded, s->hp.off_fold, st)) return rc;
+    if (int rc = scan_counts(s, padded, s->hp.off_fold, s->n_hp, st)) return rc;
     SSS_HIP(launch_fold_compact(s->hp, s->smp, s->n_hp, st));
     SSS_HIP(launch_pair_half(s->hp, s->n_hp, st));
     return SSS_OK;
@@ -575,12 +578,13 @@ int sss_subject_trace(sss_subject* s)
     }
     SSS_HIP(launch_halfpath_count(s->grid, s->d_sp, s->n_sp, s->d_detpos, s->d_detnorm, s->nru, s->ndu,
                                   s->hp, st));
-    // slot n_hp holds a zero count so off[n_hp] is the total
-    SSS_HIP(hipMemsetAsync(s->hp.n + s->n_hp, 0, sizeof(int32_t), st));
+    // the last slot holds a zero count so the scan's last entry is the total
+    const int64_t n_blk = (s->n_hp + kWave - 1) / kWave;
+    SSS_HIP(hipMemsetAsync(s->hp.blk + n_blk, 0, sizeof(int32_t), st));
     SSS_HIP(hipMemsetAsync(s->hp.n_fold + s->n_hp, 0, sizeof(int32_t), st));
-    if (int rc = scan_counts(s, s->hp.n, s->hp.off, st)) return rc;
+    if (int rc = scan_counts(s, s->hp.blk, s->hp.blk_off, n_blk, st)) return rc;
     int64_t total = 0;
-    SSS_HIP(hipMemcpyAsync(&total, s->hp.off + s->n_hp, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    SSS_HIP(hipMemcpyAsync(&total, s->hp.blk_off + n_blk, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     SSS_HIP(hipStreamSynchronize(st));
     if (total > s->samples_cap || !s->smp.rec) {
         dev_free(s->smp.rec); dev_free(s->smp.len); dev_free(s->smp.a64);

@@ -38,13 +38,21 @@ __global__ void k_halfpath_count(Grid g, const double* __restrict__ sp_xyz, int 
 {
     const int64_t n_hp = (int64_t)n_sp * nru * ndu;
     const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= n_hp) return;
+    int n = 0;
     double lc[6];
-    int det;
-    halfpath_line(sp_xyz, detpos, nru, ndu, h, lc, det);
-    Walk w;
-    walk_setup(g, lc, w);
-    t.n[h] = w.n;
+    int det = 0;
+    if (h < n_hp) {
+        halfpath_line(sp_xyz, detpos, nru, ndu, h, lc, det);
+        Walk w;
+        walk_setup(g, lc, w);
+        n = w.n;
+    }
+    // records of this wave's block of half paths: kWave x its longest walk
+    int m = n;
+    for (int o = kWave / 2; o > 0; o /= 2) m = max(m, __shfl_xor(m, o, kWave));
+    if (h >= n_hp) return;
+    if (h % kWave == 0) t.blk[h / kWave] = m * kWave;
+    t.n[h] = n;
     // ScatterVector length and detector-normal angle (SingleScatterSimulationTOF.py:424-430,442-444)
     const double sv0 = lc[3] - lc[0], sv1 = lc[4] - lc[1], sv2 = lc[5] - lc[2];
     t.dist[h] = sqrt(sv0 * sv0 + sv1 * sv1 + sv2 * sv2);
@@ -67,8 +75,9 @@ __global__ void k_halfpath_fill(Grid g, const double* __restrict__ sp_xyz, int n
     Walk w;
     double mu_int = 0;
     int status = 0;
+    const int64_t base = t.blk_off[h / kWave] + h % kWave;
+    t.off[h] = base;
     if (walk_setup(g, lc, w)) {
-        const int64_t base = t.off[h];
         const double two_over_c = 2.0 / kLightCmPerPs;
         bool truncated = false;
         const int got = walk_run(g, w, [&](int k, int vox, double L, double smp) {
@@ -77,8 +86,8 @@ __global__ void k_halfpath_fill(Grid g, const double* __restrict__ sp_xyz, int n
             r.nq = -(int64_t)rint(smp * two_over_c * kFixOne);
             r.a = 0.0f;
             r.vox = vox;
-            s.rec[base + k] = r;
-            s.len[base + k] = L;
+            s.rec[base + (int64_t)k * kWave] = r;
+            s.len[base + (int64_t)k * kWave] = L;
             mu_int += mu[vox] * L;
         }, &truncated);
         if (truncated) status = 1;
@@ -91,40 +100,33 @@ __global__ void k_halfpath_fill(Grid g, const double* __restrict__ sp_xyz, int n
 
 // Emission terms lam[v] L per sample, their float64 sum per half path
 // (ActivityRay and ActPath, SingleScatterSimulationTOF.py:417-418) and the
-// number of non-zero terms (the samples the fold visits).  A group of
-// kRefreshLanes lanes walks one half path with consecutive samples on
-// consecutive lanes (coalesced), then sums its partial sums; the terms are
-// non-negative, so the exact-zero tests on the sum do not depend on the order.
-constexpr int kRefreshLanes = 16;
-
+// number of non-zero terms (the samples the fold visits).  One thread per
+// half path: the wave reads its block's interleaved records in step.  The
+// terms are non-negative, so the exact-zero tests on the sum do not depend on
+// the order.
 __global__ void k_activity_refresh(HalfPathTable t, SampleTable s, int64_t n_hp,
                                    const double* __restrict__ lam)
 {
-    const int64_t h = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kRefreshLanes;
-    const int j = threadIdx.x % kRefreshLanes;
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= n_hp) return;
+    const int64_t base = t.off[h];
+    const int n = t.n[h];
     double act = 0;
     int nz = 0;
-    if (h < n_hp) {
-        const int64_t base = t.off[h];
-        const int n = t.n[h];
-        for (int k = j; k < n; k += kRefreshLanes) {
-            const double ar = lam[s.rec[base + k].vox] * s.len[base + k];
-            s.rec[base + k].a = (float)ar;
-            if (s.a64) s.a64[base + k] = ar;
-            act += ar;
-            // the fold's own precision decides which terms are zero
-            nz += s.a64 ? (ar != 0.0) : ((float)ar != 0.0f);
-        }
+    for (int k = 0; k < n; ++k) {
+        const int64_t i = base + (int64_t)k * kWave;
+        const double ar = lam[s.rec[i].vox] * s.len[i];
+        s.rec[i].a = (float)ar;
+        if (s.a64) s.a64[i] = ar;
+        act += ar;
+        // the fold's own precision decides which terms are zero
+        nz += s.a64 ? (ar != 0.0) : ((float)ar != 0.0f);
     }
-    for (int o = kRefreshLanes / 2; o > 0; o /= 2) {
-        act += __shfl_xor(act, o, kRefreshLanes);
-        nz += __shfl_xor(nz, o, kRefreshLanes);
-    }
-    if (h < n_hp && j == 0) {
-        t.act[h] = act;
-        t.n_fold[h] = nz;
-    }
+    t.act[h] = act;
+    t.n_fold[h] = nz;
 }
+
+constexpr int kCompactLanes = 16;   // lanes per half path in k_fold_compact
 
 // Record r of the fold table (packed groups of kSampleGroup records).
 __device__ __forceinline__ void fold_store(SampleTable s, int64_t r, int64_t nq, float a)
@@ -139,12 +141,12 @@ __device__ __forceinline__ void fold_store(SampleTable s, int64_t r, int64_t nq,
 // Copies the samples with lam L != 0 of each half path, in order, to the
 // fold table at off_fold (scanned from the padded n_fold), then pads the
 // half path to a whole group with copies of its last kept record of weight
-// 0.  One group of kRefreshLanes lanes per half path; a lane's place among
+// 0.  One group of kCompactLanes lanes per half path; a lane's place among
 // the kept samples of one pass comes from the wave's ballot.
 __global__ void k_fold_compact(HalfPathTable t, SampleTable s, int64_t n_hp)
 {
-    const int64_t h = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kRefreshLanes;
-    const int j = threadIdx.x % kRefreshLanes;
+    const int64_t h = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kCompactLanes;
+    const int j = threadIdx.x % kCompactLanes;
     const int g0 = (threadIdx.x % kWave) - j;   // the group's first lane in the wave
     if (h >= n_hp) return;
     const int64_t base = t.off[h];
@@ -152,18 +154,18 @@ __global__ void k_fold_compact(HalfPathTable t, SampleTable s, int64_t n_hp)
     const int64_t dst = t.off_fold[h];
     int kept = 0;
     SampleRec last{};
-    for (int k0 = 0; k0 < n; k0 += kRefreshLanes) {
+    for (int k0 = 0; k0 < n; k0 += kCompactLanes) {
         const int k = k0 + j;
         SampleRec r{};
         double a64 = 0.0;
         if (k < n) {
-            r = s.rec[base + k];
-            if (s.fold_a64) a64 = s.a64[base + k];
+            r = s.rec[base + (int64_t)k * kWave];
+            if (s.fold_a64) a64 = s.a64[base + (int64_t)k * kWave];
         }
         // the same test as the count in k_activity_refresh
         const bool keep = k < n && (s.fold_a64 ? a64 != 0.0 : r.a != 0.0f);
         const uint64_t wave_mask = __ballot(keep);
-        const uint32_t gm = (uint32_t)(wave_mask >> g0) & ((1u << kRefreshLanes) - 1);
+        const uint32_t gm = (uint32_t)(wave_mask >> g0) & ((1u << kCompactLanes) - 1);
         if (keep) {
             const int at = kept + __popc(gm & ((1u << j) - 1));
             fold_store(s, dst + at, r.nq, r.a);
@@ -252,8 +254,7 @@ hipError_t launch_activity_refresh(const HalfPathTable t, SampleTable s, int64_t
                                    const double* lam, hipStream_t st)
 {
     if (n_hp == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_activity_refresh, dim3(blocks_for(n_hp * kRefreshLanes, 256)), dim3(256), 0, st,
-                       t, s, n_hp, lam);
+    hipLaunchKernelGGL(k_activity_refresh, dim3(blocks_for(n_hp, 256)), dim3(256), 0, st, t, s, n_hp, lam);
     return hipGetLastError();
 }
 
@@ -284,7 +285,7 @@ hipError_t launch_pair_half(const HalfPathTable t, int64_t n_hp, hipStream_t st)
 hipError_t launch_fold_compact(const HalfPathTable t, SampleTable s, int64_t n_hp, hipStream_t st)
 {
     if (n_hp == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fold_compact, dim3(blocks_for(n_hp * kRefreshLanes, 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_fold_compact, dim3(blocks_for(n_hp * kCompactLanes, 256)), dim3(256), 0, st,
                        t, s, n_hp);
     return hipGetLastError();
 }
