@@ -95,26 +95,22 @@ constexpr double kFixOne = 4294967296.0;   // 2^32
 constexpr int kSampleGroup = 4;
 constexpr int kSamplePad = 8;              // fold records after the last one (prefetch reads up to 7 ahead)
 
-// One voxel sample of a half path, read by the fold with a single 16-byte load.
-struct alignas(16) SampleRec {
-    int64_t nq;        // -round(2^32 q), q = 2 s_k / c in ps
-    float a;           // lam * len (FP32 fold input, rewritten per activity image)
-    int32_t vox;       // 0-based voxel (reference linear index - 1)
-};
-
-// Per-sample arrays.  rec/len/a64: every voxel sample of every half path,
-// built by the tracer; a and a64 rewritten per activity image.  The half
-// paths h of one block of kWave (h / kWave) are interleaved: sample k of h is
-// record blk_off[h / kWave] + k * kWave + h % kWave, so a wave that walks the
-// block's paths in step writes and reads kWave consecutive records.  fold/fold_a64: per activity image, the samples with
-// lam L != 0 only (their terms are exactly zero otherwise), offsets
+// Per-sample arrays.  nq/vox/len: every voxel sample of every half path,
+// built by the tracer; a/a64: its emission term, rewritten per activity
+// image.  The half paths h of one block of kWave (h / kWave) are
+// interleaved: sample k of h is entry blk_off[h / kWave] + k * kWave +
+// h % kWave, so a wave that walks the block's paths in step writes and reads
+// kWave consecutive entries.  fold/fold_a64: per activity image, the samples
+// with lam L != 0 only (their terms are exactly zero otherwise), offsets
 // HalfPathTable::off_fold, padded per half path to whole groups, kSamplePad
 // spare records at the end.  The fold table packs each group of 4 records
 // into 12 words (three 16-byte loads): nq of records 0..3 (low, high word),
 // then a of records 0..3.
 struct SampleTable {
-    SampleRec* rec;
+    int64_t* nq;       // -round(2^32 q), q = 2 s_k / c in ps
+    int32_t* vox;      // 0-based voxel (reference linear index - 1)
     double* len;       // intersection length, cm
+    float* a;          // lam * len (FP32 fold input)
     double* a64;       // lam * len in float64 (FP64 validation path only, else null)
     uint32_t* fold;    // kFoldGroupWords per group of kSampleGroup records
     double* fold_a64;  // FP64 path only

@@ -201,7 +201,8 @@ struct sss_subject {
         dev_free(hp.blk); dev_free(hp.blk_off);
         dev_free(hp.dist); dev_free(hp.att); dev_free(hp.lnatt);
         dev_free(hp.act); dev_free(hp.cosabs); dev_free(hp.status);
-        dev_free(smp.rec); dev_free(smp.len); dev_free(smp.a64); dev_free(smp.fold); dev_free(smp.fold_a64);
+        dev_free(smp.nq); dev_free(smp.vox); dev_free(smp.len); dev_free(smp.a); dev_free(smp.a64);
+        dev_free(smp.fold); dev_free(smp.fold_a64);
         dev_free(d_out); dev_free(d_part_sum); dev_free(d_part_cnt); dev_free(d_items);
     }
 };
@@ -586,8 +587,8 @@ int sss_subject_trace(sss_subject* s)
     int64_t total = 0;
     SSS_HIP(hipMemcpyAsync(&total, s->hp.blk_off + n_blk, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     SSS_HIP(hipStreamSynchronize(st));
-    if (total > s->samples_cap || !s->smp.rec) {
-        dev_free(s->smp.rec); dev_free(s->smp.len); dev_free(s->smp.a64);
+    if (total > s->samples_cap || !s->smp.nq) {
+        dev_free(s->smp.nq); dev_free(s->smp.vox); dev_free(s->smp.len); dev_free(s->smp.a); dev_free(s->smp.a64);
         dev_free(s->smp.fold); dev_free(s->smp.fold_a64);
         s->samples_cap = 0;
         // the fold table holds at most every sample plus the group padding,
@@ -596,8 +597,10 @@ int sss_subject_trace(sss_subject* s)
         fold_cap = (fold_cap + kSampleGroup - 1) / kSampleGroup * kSampleGroup;
         const bool fp64 = s->precision == SSS_PRECISION_FP64;
         int rc;
-        if ((rc = dev_alloc(&s->smp.rec, (size_t)total, &s->device_bytes)) ||
+        if ((rc = dev_alloc(&s->smp.nq, (size_t)total, &s->device_bytes)) ||
+            (rc = dev_alloc(&s->smp.vox, (size_t)total, &s->device_bytes)) ||
             (rc = dev_alloc(&s->smp.len, (size_t)total, &s->device_bytes)) ||
+            (rc = dev_alloc(&s->smp.a, (size_t)total, &s->device_bytes)) ||
             (rc = dev_alloc(&s->smp.fold, fold_cap / kSampleGroup * kFoldGroupWords, &s->device_bytes)) ||
             (fp64 && ((rc = dev_alloc(&s->smp.a64, (size_t)total, &s->device_bytes)) ||
                       (rc = dev_alloc(&s->smp.fold_a64, fold_cap, &s->device_bytes)))))

@@ -81,13 +81,11 @@ __global__ void k_halfpath_fill(Grid g, const double* __restrict__ sp_xyz, int n
         const double two_over_c = 2.0 / kLightCmPerPs;
         bool truncated = false;
         const int got = walk_run(g, w, [&](int k, int vox, double L, double smp) {
-            SampleRec r;
+            const int64_t i = base + (int64_t)k * kWave;
             // q * 2^32 is exact in float64; rint drops bits below 2^-32 ps
-            r.nq = -(int64_t)rint(smp * two_over_c * kFixOne);
-            r.a = 0.0f;
-            r.vox = vox;
-            s.rec[base + (int64_t)k * kWave] = r;
-            s.len[base + (int64_t)k * kWave] = L;
+            s.nq[i] = -(int64_t)rint(smp * two_over_c * kFixOne);
+            s.vox[i] = vox;
+            s.len[i] = L;
             mu_int += mu[vox] * L;
         }, &truncated);
         if (truncated) status = 1;
@@ -115,8 +113,8 @@ __global__ void k_activity_refresh(HalfPathTable t, SampleTable s, int64_t n_hp,
     int nz = 0;
     for (int k = 0; k < n; ++k) {
         const int64_t i = base + (int64_t)k * kWave;
-        const double ar = lam[s.rec[i].vox] * s.len[i];
-        s.rec[i].a = (float)ar;
+        const double ar = lam[s.vox[i]] * s.len[i];
+        s.a[i] = (float)ar;
         if (s.a64) s.a64[i] = ar;
         act += ar;
         // the fold's own precision decides which terms are zero
@@ -153,34 +151,33 @@ __global__ void k_fold_compact(HalfPathTable t, SampleTable s, int64_t n_hp)
     const int n = t.n[h];
     const int64_t dst = t.off_fold[h];
     int kept = 0;
-    SampleRec last{};
+    int64_t last_nq = 0;
     for (int k0 = 0; k0 < n; k0 += kCompactLanes) {
         const int k = k0 + j;
-        SampleRec r{};
+        int64_t nq = 0;
+        float a = 0.0f;
         double a64 = 0.0;
         if (k < n) {
-            r = s.rec[base + (int64_t)k * kWave];
-            if (s.fold_a64) a64 = s.a64[base + (int64_t)k * kWave];
+            const int64_t i = base + (int64_t)k * kWave;
+            nq = s.nq[i];
+            a = s.a[i];
+            if (s.fold_a64) a64 = s.a64[i];
         }
         // the same test as the count in k_activity_refresh
-        const bool keep = k < n && (s.fold_a64 ? a64 != 0.0 : r.a != 0.0f);
+        const bool keep = k < n && (s.fold_a64 ? a64 != 0.0 : a != 0.0f);
         const uint64_t wave_mask = __ballot(keep);
         const uint32_t gm = (uint32_t)(wave_mask >> g0) & ((1u << kCompactLanes) - 1);
         if (keep) {
             const int at = kept + __popc(gm & ((1u << j) - 1));
-            fold_store(s, dst + at, r.nq, r.a);
+            fold_store(s, dst + at, nq, a);
             if (s.fold_a64) s.fold_a64[dst + at] = a64;
         }
-        if (gm) {
-            const int src = g0 + 31 - __clz(gm);
-            last.nq = __shfl(r.nq, src, kWave);
-            last.vox = __shfl(r.vox, src, kWave);
-        }
+        if (gm) last_nq = __shfl(nq, g0 + 31 - __clz(gm), kWave);
         kept += __popc(gm);
     }
     const int padded = (kept + kSampleGroup - 1) / kSampleGroup * kSampleGroup;
     if (kept + j < padded) {
-        fold_store(s, dst + kept + j, last.nq, 0.0f);
+        fold_store(s, dst + kept + j, last_nq, 0.0f);
         if (s.fold_a64) s.fold_a64[dst + kept + j] = 0.0;
     }
 }
