@@ -16,14 +16,14 @@ constexpr int kWave = 64;
 // A wave owns 8 x 8 cells: 8 rows (ring r1, detector i1) x 8 columns (ring
 // r2, detector i2), each a block of 2^k rings x 8/2^k detectors.  Path
 // lengths vary far less across neighbouring rings than across neighbouring
-// detectors, so ring blocks keep the lanes' trip counts close (simulated on
-// config 2: 83% of lane slots busy with 4 rings x 2 detectors, 58% with
-// 1 ring x 8 detectors).
+// detectors, so ring blocks keep the lanes' trip counts close (a CPU model
+// of config 2 with the lanes taking the scatter points in step: 88% of lane
+// slots busy with 4 rings x 2 detectors, 78% with 2 x 4, 61% with 1 x 8).
 constexpr int kTileRows = 8;
 constexpr int kTileCols = 8;
 constexpr int kTileMaxRings = 4;
 constexpr int kScatterWaves = 4;          // waves per workgroup of the scatter launch
-constexpr int kAutoChunkPoints = 16;      // scatter points per chunk by default
+constexpr int kAutoChunkPoints = 64;      // at most this many scatter points per chunk by default
 constexpr int kMaxBins = 32;
 
 // TOF weight evaluation of the fold (see tof_weights in sss_scatter.hip)
