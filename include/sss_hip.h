@@ -137,8 +137,10 @@ int sss_subject_scatter(sss_subject* s);
 int sss_subject_run(sss_subject* s);
 int sss_subject_info_get(sss_subject* s, sss_subject_info* out);
 /* Pairs passing the reference's filters (A != 0, some activity, efficiency
- * != 0; SingleScatterSimulationTOF.py:451-509) and the voxel samples their TOF
- * folds visit, for the algorithmic work count.  Synchronous. */
+ * != 0; SingleScatterSimulationTOF.py:451-509) and the voxel samples of their
+ * half paths with activity (n1 [Act1 != 0] + n2 [Act2 != 0], every voxel as
+ * the reference's fold visits them; the fold here skips the zero terms), for
+ * the algorithmic work count.  Synchronous. */
 int sss_subject_work_stats(sss_subject* s, int64_t* pairs, int64_t* samples);
 /* Duration (HIP events on the context stream) of the last scatter kernel. */
 int sss_subject_scatter_kernel_ms(sss_subject* s, float* ms);

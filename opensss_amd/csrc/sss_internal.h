@@ -126,6 +126,7 @@ struct ScatterArgs {
     const PairHalf* hp_rec;      // per half path, FP32 pair evaluation
     const int64_t* hp_off;       // first fold record of each half path
     const int32_t* hp_n;         // fold records (samples with lam L != 0)
+    const int32_t* hp_n_all;     // all samples of each half path
     const double* hp_dist;
     const double* hp_att;
     const double* hp_lnatt;

@@ -651,7 +651,7 @@ static ScatterArgs scatter_args(const sss_subject* s)
     a.n_energy = s->n_energy; a.eff_ref_row = s->eff_ref_row;
     a.n_tiles = s->n_tiles; a.n_chunks = s->n_chunks; a.n_cells_plane = s->n_cells_plane;
     a.hp_rec = s->hp.rec;
-    a.hp_off = s->hp.off_fold; a.hp_n = s->hp.n_fold; a.hp_dist = s->hp.dist; a.hp_att = s->hp.att;
+    a.hp_off = s->hp.off_fold; a.hp_n = s->hp.n_fold; a.hp_n_all = s->hp.n; a.hp_dist = s->hp.dist; a.hp_att = s->hp.att;
     a.hp_lnatt = s->hp.lnatt; a.hp_act = s->hp.act; a.hp_cos = s->hp.cosabs;
     a.s_fold = s->smp.fold; a.s_a64 = s->smp.fold_a64;
     a.detpos = s->d_detpos; a.eff = s->d_eff; a.ratio = s->d_ratio; a.tiles = s->d_tiles;

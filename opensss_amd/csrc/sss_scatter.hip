@@ -636,8 +636,9 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
 }
 
 // Counts the (scatter point, LOR) pairs that pass the reference's filters and
-// the voxel samples their TOF folds visit: the work the algorithmic FLOP count
-// F = sum_pairs [2 B (n1 [act1 != 0] + n2 [act2 != 0]) + 64] is made of.
+// the voxel samples of their half paths with activity: the work the
+// algorithmic FLOP count F = sum_pairs [2 B (n1 [act1 != 0] + n2 [act2 != 0])
+// + 64] is made of (n: all samples of the half path, zero terms included).
 __global__ __launch_bounds__(256) void k_pair_stats(ScatterArgs a, unsigned long long* out)
 {
     const int wave = threadIdx.x / kWave;
@@ -656,12 +657,17 @@ __global__ __launch_bounds__(256) void k_pair_stats(ScatterArgs a, unsigned long
     }
     const double eff_ref = a.eff[a.eff_ref_row];
     unsigned long long pairs = 0, samples = 0;
+    const int64_t per_sp = (int64_t)a.nru * a.ndu;
     const int p_end = a.chunk_sp[a.n_chunks];
     for (int sp = a.chunk_sp[0]; sp < p_end; ++sp) {
         Pair<float> p;
         if (!pair_eval_rec(a, sp, d1, d2, lor2, eff_ref, p)) continue;
         ++pairs;
-        samples += (unsigned long long)(p.n1 + p.n2);
+        // every voxel sample of a path with activity, as the reference folds
+        // them (the fold here skips the zero terms)
+        const int64_t h1 = sp * per_sp + d1, h2 = sp * per_sp + d2;
+        if (a.hp_rec[h1].n_flags & kHalfActive) samples += (unsigned long long)a.hp_n_all[h1];
+        if (a.hp_rec[h2].n_flags & kHalfActive) samples += (unsigned long long)a.hp_n_all[h2];
     }
     atomicAdd(out + 0, pairs);
     atomicAdd(out + 1, samples);

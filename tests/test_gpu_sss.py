@@ -246,3 +246,65 @@ def test_invalid_inputs_raise():
             s.scatter()          # no activity yet
     finally:
         s.close()
+
+
+def test_work_stats_count_the_reference_work(oracle):
+    """Pairs kept by the reference's filters and the voxel samples of their
+    active half paths (the algorithmic work count), against a brute-force
+    count over the oracle's traces: the reference's energy index
+    (acos/degrees/cos round trip) and efficiency test, Counts' activity tests
+    and every voxel of each active half path."""
+    import math
+    from opensss_amd.tables import (energy_tables, scatter_point_coordinates, detectors_used,
+                                    rings_used)
+    w = phantoms.small_tof(seed=5, n_sp=12)
+    NR, N = w.geometry.shape[0], w.geometry.shape[1]
+    rings = rings_used(NR, w.n_rings_used)
+    dets = detectors_used(N, w.n_det_used)
+    _, eff, ref = energy_tables(WATER, w.energy_resolution, w.energy_threshold)
+    xyz = scatter_point_coordinates(w.scatter_points, w.attenuation.shape, w.image_size)
+    lam = w.activity.ravel(order='F')
+    mu = w.attenuation.ravel(order='F')
+    grid = np.array(w.attenuation.shape)
+    NRU, NDU = len(rings), len(dets)
+    det = np.array([[[w.geometry[r, d, 0], w.geometry[r, d, 1], w.geometry[r, 0, 2]] for d in dets]
+                    for r in rings])
+    pairs = samples = 0
+    for p in xyz:
+        n = np.zeros((NRU, NDU), int)
+        act = np.zeros((NRU, NDU))
+        att = np.zeros((NRU, NDU))
+        dist = np.zeros((NRU, NDU))
+        for a in range(NRU):
+            for b in range(NDU):
+                L, idx, _ = oracle.trace(grid, w.image_size, np.concatenate([p, det[a, b]]))
+                n[a, b] = len(L)
+                act[a, b] = np.sum(lam[idx - 1] * L)
+                att[a, b] = math.exp(-np.sum(mu[idx - 1] * L))
+                dist[a, b] = math.sqrt(np.sum((det[a, b] - p) ** 2))
+        for r1 in range(NRU):
+            for i1 in range(NDU):
+                for r2 in range(NRU):
+                    for i2 in range(NDU):
+                        if i1 > i2 or (i1 == i2 and r1 == r2):
+                            continue
+                        if att[r1, i1] == 0 or att[r2, i2] == 0 or (act[r1, i1] == 0 and act[r2, i2] == 0):
+                            continue
+                        s1q, s2q = dist[r1, i1] ** 2, dist[r2, i2] ** 2
+                        lor = np.sum((det[r1, i1] - det[r2, i2]) ** 2)
+                        theta = 180 - math.degrees(math.acos(min(1.0, max(-1.0, (s1q + s2q - lor)
+                                                                           / (2 * math.sqrt(s1q * s2q))))))
+                        e = math.floor(511.0 / (1 + (1 - math.cos(math.radians(theta)))) + 0.5)
+                        k = min(max(int(e * 2) - 1, 0), len(eff) - 1)
+                        if eff[k] * eff[ref] == 0:
+                            continue
+                        pairs += 1
+                        samples += (n[r1, i1] if act[r1, i1] != 0 else 0) + (n[r2, i2] if act[r2, i2] != 0 else 0)
+    s = gpu_subject(w, 'fp32')
+    try:
+        s.run(w.activity)
+        got_pairs, got_samples = s.work_stats()
+    finally:
+        s.close()
+    assert (got_pairs, got_samples) == (pairs, samples)
+    assert 0 < pairs < w.n_rings_used ** 2 * w.n_det_used ** 2 * len(xyz)
