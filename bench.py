@@ -36,7 +36,61 @@ def parse():
     p.add_argument('--cpu-threads', type=int, default=0)
     p.add_argument('--calibrate-copy', action='store_true',
                    help='run only a 1 GiB device copy (3 launches) to scale FETCH_SIZE/WRITE_SIZE')
+    p.add_argument('--batch', type=int, default=0, metavar='N',
+                   help='instead of the default line: N uneven random instances (seeds 0..N-1), '
+                        'the one-instance path against the batched path')
+    p.add_argument('--streams', type=int, default=4, help='contexts of the batched path')
+    p.add_argument('--batch-kind', default='random', choices=['random', 'config1'],
+                   help='uneven random instances (seeds 0..N-1) or config 1 instances of N seeds')
     return p.parse_args()
+
+
+def instance_evals(w):
+    nru, ndu = w.n_rings_used, w.n_det_used
+    lors = nru * (ndu * (ndu - 1) // 2) + nru * (nru - 1) * (ndu * (ndu + 1) // 2)
+    return len(w.scatter_points) * lors * w.n_bins
+
+
+def batch_bench(args):
+    """Batched/served reading: N uneven instances through the one-instance path
+    (create, trace, scatter, download, destroy, one at a time) and through
+    opensss_amd.batch.run_batch (several in flight on several contexts, device
+    blocks reused); both outputs compared bit for bit."""
+    from opensss_amd import phantoms
+    from opensss_amd.batch import run_batch, run_plain
+    water = np.load(os.path.join(ROOT, 'opensss_amd', 'data', 'attenuation_water.npy'))
+    if args.batch_kind == 'config1':
+        ws = [phantoms.config1(seed=seed) for seed in range(args.batch)]
+    else:
+        ws = [phantoms.random_instance(seed) for seed in range(args.batch)]
+    evals = sum(instance_evals(w) for w in ws)
+    print(f'[bench] batch of {len(ws)} instances, {evals:.3e} evals', file=sys.stderr, flush=True)
+    run_plain(ws[:1], water)   # library and device warm-up
+    t0 = time.perf_counter()
+    plain = run_plain(ws, water)
+    t_plain = time.perf_counter() - t0
+    print(f'[bench] one-instance path {t_plain:.3f} s', file=sys.stderr, flush=True)
+    from opensss_amd import _native
+    ctxs = [_native.Context(0) for _ in range(max(1, args.streams))]
+    run_batch(ws[:2 * len(ctxs)], water, contexts=ctxs)   # contexts and their caches warm, as the default one
+    t0 = time.perf_counter()
+    batched = run_batch(ws, water, contexts=ctxs)
+    t_batch = time.perf_counter() - t0
+    for c in ctxs:
+        c.close()
+    print(f'[bench] batched path {t_batch:.3f} s', file=sys.stderr, flush=True)
+    same = all(np.array_equal(a, b) for a, b in zip(plain, batched))
+    line = {
+        'metric': 'batched scatter-pt x LOR x TOF-bin evals/s over uneven instances', 'value': evals / t_batch,
+        'unit': 'evals/s', 'n_gpus': 1, 'higher_is_better': True, 'dtype': 'f32', 'data': 'synthetic',
+        'config': {'workload': f'{args.batch_kind}-instances-0..{len(ws) - 1}', 'instances': len(ws),
+                   'evals': int(evals),
+                   'streams': args.streams},
+        'one_instance_path': {'value': evals / t_plain, 'seconds': t_plain},
+        'batched_path': {'value': evals / t_batch, 'seconds': t_batch},
+        'speedup': t_plain / t_batch, 'outputs_identical': bool(same),
+    }
+    print(json.dumps(line))
 
 
 def calibrate_copy():
@@ -133,6 +187,9 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.batch:
+        batch_bench(args)
         return
     w = make_workload(args.workload)
     print(f'[bench] workload {w.name}: {len(w.scatter_points)} sp, grid {w.grid}, '

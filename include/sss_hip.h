@@ -105,6 +105,16 @@ int sss_ctx_destroy(sss_ctx* ctx);
 int sss_ctx_synchronize(sss_ctx* ctx);
 /* The context's hipStream_t, for callers that time or order work on it. */
 void* sss_ctx_stream(sss_ctx* ctx);
+/* Device memory cache: a context keeps the blocks its destroyed subjects
+ * freed and hands them to later subjects (no counterpart in the reference;
+ * for callers that estimate many instances in a row).  release_cache frees
+ * the idle blocks; cached_bytes reports their size. */
+int sss_ctx_release_cache(sss_ctx* ctx);
+int sss_ctx_cached_bytes(sss_ctx* ctx, int64_t* bytes);
+/* Pinned host buffers of a context, kept for reuse after sss_host_free
+ * (hipHostMalloc/hipHostFree synchronise the device). */
+int sss_host_alloc(sss_ctx* ctx, int64_t bytes, void** out);
+int sss_host_free(sss_ctx* ctx, void* p);
 
 /* Trace n_segs segments (6 doubles each: x1 y1 z1 x2 y2 z2) through the grid.
  * Outputs have cap = X+Y+Z slots per segment: lengths (f64), 1-based voxel
@@ -126,6 +136,12 @@ int sss_subject_destroy(sss_subject* s);
 /* Trace every scatter-point -> detector half path and build the resident
  * table (voxel ids, lengths, TOF sample times, attenuation, angles). */
 int sss_subject_trace(sss_subject* s);
+/* sss_subject_trace in two halves, so a caller with several subjects on
+ * several contexts waits once for all of them: begin queues the walk counts
+ * (asynchronous); end waits for this subject's counts, sizes the sample
+ * tables and queues the walks (asynchronous after that wait). */
+int sss_subject_trace_begin(sss_subject* s);
+int sss_subject_trace_end(sss_subject* s);
 /* Upload lam (X*Y*Z f64, host) and refresh the per-sample emission terms. */
 int sss_subject_set_activity(sss_subject* s, const double* lam);
 /* Same from a device pointer (X*Y*Z f64), no host copy. */
@@ -147,6 +163,13 @@ int sss_subject_scatter_kernel_ms(sss_subject* s, float* ms);
 /* Device pointer of the compact output (valid until destroy). */
 const void* sss_subject_output_device(const sss_subject* s);
 int sss_subject_output_compact(sss_subject* s, float* host_out);
+/* The compact download queued on the context's stream (host_out should come
+ * from sss_host_alloc for the copy to run asynchronously); sss_subject_wait
+ * waits for it and for the work queued before it on this subject.  For
+ * callers that keep several subjects in flight (no counterpart in the
+ * reference). */
+int sss_subject_output_compact_async(sss_subject* s, float* host_out);
+int sss_subject_wait(sss_subject* s);
 /* Dense reference layout (B, N+1, N/2, NRU^2) float32 on the host, using
  * lor_coords (N, N, 2) int16 = (angular, radial). */
 int sss_subject_output_dense(sss_subject* s, const int16_t* lor_coords, float* host_out);

@@ -71,6 +71,10 @@ SIGNATURES = [
     ('sss_ctx_destroy', ctypes.c_int, [ctypes.c_void_p]),
     ('sss_ctx_synchronize', ctypes.c_int, [ctypes.c_void_p]),
     ('sss_ctx_stream', ctypes.c_void_p, [ctypes.c_void_p]),
+    ('sss_ctx_release_cache', ctypes.c_int, [ctypes.c_void_p]),
+    ('sss_ctx_cached_bytes', ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
+    ('sss_host_alloc', ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)]),
+    ('sss_host_free', ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ('sss_trace_segments', ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
@@ -81,6 +85,8 @@ SIGNATURES = [
      [ctypes.c_void_p, ctypes.POINTER(SubjectDesc), ctypes.POINTER(ctypes.c_void_p)]),
     ('sss_subject_destroy', ctypes.c_int, [ctypes.c_void_p]),
     ('sss_subject_trace', ctypes.c_int, [ctypes.c_void_p]),
+    ('sss_subject_trace_begin', ctypes.c_int, [ctypes.c_void_p]),
+    ('sss_subject_trace_end', ctypes.c_int, [ctypes.c_void_p]),
     ('sss_subject_set_activity', ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ('sss_subject_set_activity_device', ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ('sss_subject_scatter', ctypes.c_int, [ctypes.c_void_p]),
@@ -92,6 +98,8 @@ SIGNATURES = [
      [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
     ('sss_subject_output_device', ctypes.c_void_p, [ctypes.c_void_p]),
     ('sss_subject_output_compact', ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ('sss_subject_output_compact_async', ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ('sss_subject_wait', ctypes.c_int, [ctypes.c_void_p]),
     ('sss_subject_output_dense', ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ('sss_buffer_alloc', ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)]),
     ('sss_buffer_free', ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
@@ -202,6 +210,25 @@ class Context:
 
     def synchronize(self) -> None:
         check(self._lib.sss_ctx_synchronize(self.handle), 'sss_ctx_synchronize')
+
+    def release_cache(self) -> None:
+        """Free the device blocks kept for later subjects."""
+        check(self._lib.sss_ctx_release_cache(self.handle), 'sss_ctx_release_cache')
+
+    def host_alloc(self, nbytes: int) -> int:
+        """A pinned host buffer (address) from the context's cache."""
+        p = ctypes.c_void_p()
+        check(self._lib.sss_host_alloc(self.handle, int(nbytes), ctypes.byref(p)), 'sss_host_alloc')
+        return int(p.value)
+
+    def host_free(self, addr: int) -> None:
+        check(self._lib.sss_host_free(self.handle, ctypes.c_void_p(addr)), 'sss_host_free')
+
+    @property
+    def cached_bytes(self) -> int:
+        v = ctypes.c_int64()
+        check(self._lib.sss_ctx_cached_bytes(self.handle, ctypes.byref(v)), 'sss_ctx_cached_bytes')
+        return int(v.value)
 
     def close(self) -> None:
         if getattr(self, 'handle', None):

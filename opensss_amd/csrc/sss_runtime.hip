@@ -10,6 +10,9 @@
 #include <rocprim/rocprim.hpp>
 
 #include <algorithm>
+#include <map>
+#include <type_traits>
+#include <unordered_map>
 #include <climits>
 #include <cmath>
 #include <new>
@@ -141,10 +144,128 @@ std::vector<int2> xcd_work_items(const std::vector<int4>& tiles, int nru, int n_
 
 using namespace sss;
 
+namespace sss {
+namespace {
+
+// Device memory cache of one context.  Blocks freed by its subjects are kept
+// for later subjects: hipMalloc of the GB-sized tables costs milliseconds and
+// hipFree synchronises the device.  Every user of a context's blocks works on
+// the context's stream, so a block's next use is ordered after its last one.
+struct DevicePool {
+    std::multimap<size_t, void*> idle;          // size -> block
+    std::unordered_map<void*, size_t> owned;    // every block of the pool
+    size_t idle_bytes = 0;
+
+    static size_t rounded(size_t b)
+    {
+        const size_t q = b < (size_t(1) << 20) ? 256 : (size_t(2) << 20);
+        return (b + q - 1) / q * q;
+    }
+    hipError_t get(size_t bytes, void** p)
+    {
+        const size_t b = rounded(bytes);
+        auto it = idle.lower_bound(b);
+        if (it != idle.end() && it->first <= b + b / 4 + (size_t(4) << 20)) {
+            *p = it->second;
+            idle_bytes -= it->first;
+            idle.erase(it);
+            return hipSuccess;
+        }
+        hipError_t e = hipMalloc(p, b);
+        if (e != hipSuccess && !idle.empty()) {   // out of memory: drop the cache, try once more
+            (void)hipGetLastError();
+            release_idle();
+            e = hipMalloc(p, b);
+        }
+        if (e == hipSuccess) owned[*p] = b;
+        return e;
+    }
+    void put(void* p)
+    {
+        auto it = owned.find(p);
+        if (it == owned.end()) return;
+        idle.emplace(it->second, p);
+        idle_bytes += it->second;
+    }
+    void release_idle()
+    {
+        for (auto& kv : idle) {
+            (void)hipFree(kv.second);
+            owned.erase(kv.second);
+        }
+        idle.clear();
+        idle_bytes = 0;
+    }
+};
+
+}  // namespace
+}  // namespace sss
+
 struct sss_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    sss::DevicePool pool;
+    // pinned host words for the subjects' sample counts (hipHostMalloc and
+    // hipHostFree synchronise the device, so they are not done per subject)
+    int64_t* pinned = nullptr;
+    std::vector<int> pinned_free;
+    // pinned host buffers handed out by sss_host_alloc, kept for reuse
+    std::multimap<size_t, void*> host_idle;
+    std::unordered_map<void*, size_t> host_owned;
 };
+
+static constexpr int kPinnedSlots = 4096;
+
+namespace sss {
+namespace {
+
+template <typename T>
+int pool_alloc(sss_ctx* c, T** p, size_t count, int64_t* tally)
+{
+    *p = nullptr;
+    if (count == 0) return SSS_OK;
+    void* v = nullptr;
+    hipError_t e = c->pool.get(count * sizeof(T), &v);
+    if (e != hipSuccess) return fail(SSS_ERR_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    *p = static_cast<T*>(v);
+    if (tally) *tally += (int64_t)(count * sizeof(T));
+    return SSS_OK;
+}
+
+template <typename T>
+void pool_free(sss_ctx* c, T*& p)
+{
+    if (p) c->pool.put(static_cast<void*>(const_cast<std::remove_const_t<T>*>(p)));
+    p = nullptr;
+}
+
+// Pinned host blocks of a context, kept for reuse (hipHostMalloc/hipHostFree
+// synchronise the device).
+int host_block(sss_ctx* c, size_t bytes, void** out)
+{
+    const size_t b = std::max<size_t>(4096, (bytes + 4095) / 4096 * 4096);
+    auto it = c->host_idle.lower_bound(b);
+    if (it != c->host_idle.end() && it->first <= 2 * b) {
+        *out = it->second;
+        c->host_idle.erase(it);
+        return SSS_OK;
+    }
+    SSS_HIP(hipSetDevice(c->device));
+    void* p = nullptr;
+    SSS_HIP(hipHostMalloc(&p, b, hipHostMallocDefault));
+    c->host_owned[p] = b;
+    *out = p;
+    return SSS_OK;
+}
+
+void host_block_free(sss_ctx* c, void* p)
+{
+    auto it = c->host_owned.find(p);
+    if (it != c->host_owned.end()) c->host_idle.emplace(it->second, p);
+}
+
+}  // namespace
+}  // namespace sss
 
 struct sss_subject {
     sss_ctx* ctx = nullptr;
@@ -187,23 +308,43 @@ struct sss_subject {
     float* d_part_cnt = nullptr;          // [chunk][slot]
     int64_t n_slots = 0;
     std::vector<int32_t> dets;
+    // pinned host sources of the queued uploads (kept until destroy)
+    void* h_upload = nullptr;
+    double* h_lam = nullptr;
+    hipEvent_t ev_lam = nullptr;          // after the last activity upload
+    bool lam_queued = false;
+    hipEvent_t ev_upload = nullptr;       // after the uploads queued at create
     hipEvent_t ev_scatter[2] = {nullptr, nullptr};   // around the last scatter kernel
     bool ev_recorded = false;
+    // two-phase trace: the sample count reaches the host through pinned memory
+    int64_t* h_total = nullptr;
+    hipEvent_t ev_count = nullptr;
+    hipEvent_t ev_out = nullptr;          // after the last output download
+    bool trace_begun = false;
 
     ~sss_subject()
     {
         for (auto& e : ev_scatter)
             if (e) (void)hipEventDestroy(e);
-        dev_free(d_mu); dev_free(d_lam); dev_free(d_sp); dev_free(d_detpos); dev_free(d_detnorm);
-        dev_free(d_eff); dev_free(d_ratio); dev_free(d_tiles); dev_free(d_slice_ptr);
-        dev_free(d_chunk_sp); dev_free(d_chunk_slice); dev_free(d_dets); dev_free(d_flag); dev_free(d_scan_tmp);
-        dev_free(hp.off); dev_free(hp.n); dev_free(hp.off_fold); dev_free(hp.n_fold); dev_free(hp.rec);
-        dev_free(hp.blk); dev_free(hp.blk_off);
-        dev_free(hp.dist); dev_free(hp.att); dev_free(hp.lnatt);
-        dev_free(hp.act); dev_free(hp.cosabs); dev_free(hp.status);
-        dev_free(smp.nq); dev_free(smp.vox); dev_free(smp.len); dev_free(smp.a); dev_free(smp.a64);
-        dev_free(smp.fold); dev_free(smp.fold_a64);
-        dev_free(d_out); dev_free(d_part_sum); dev_free(d_part_cnt); dev_free(d_items);
+        if (h_total) ctx->pinned_free.push_back((int)(h_total - ctx->pinned));
+        if (h_upload) host_block_free(ctx, h_upload);
+        if (h_lam) host_block_free(ctx, h_lam);
+        if (ev_count) (void)hipEventDestroy(ev_count);
+        if (ev_out) (void)hipEventDestroy(ev_out);
+        if (ev_lam) (void)hipEventDestroy(ev_lam);
+        if (ev_upload) (void)hipEventDestroy(ev_upload);
+        sss_ctx* c = ctx;
+        pool_free(c, d_mu); pool_free(c, d_lam); pool_free(c, d_sp); pool_free(c, d_detpos);
+        pool_free(c, d_detnorm); pool_free(c, d_eff); pool_free(c, d_ratio); pool_free(c, d_tiles);
+        pool_free(c, d_slice_ptr); pool_free(c, d_chunk_sp); pool_free(c, d_chunk_slice); pool_free(c, d_dets);
+        pool_free(c, d_flag); pool_free(c, d_scan_tmp);
+        pool_free(c, hp.off); pool_free(c, hp.n); pool_free(c, hp.off_fold); pool_free(c, hp.n_fold);
+        pool_free(c, hp.rec); pool_free(c, hp.blk); pool_free(c, hp.blk_off); pool_free(c, hp.dist);
+        pool_free(c, hp.att); pool_free(c, hp.lnatt); pool_free(c, hp.act); pool_free(c, hp.cosabs);
+        pool_free(c, hp.status);
+        pool_free(c, smp.nq); pool_free(c, smp.vox); pool_free(c, smp.len); pool_free(c, smp.a);
+        pool_free(c, smp.a64); pool_free(c, smp.fold); pool_free(c, smp.fold_a64);
+        pool_free(c, d_out); pool_free(c, d_part_sum); pool_free(c, d_part_cnt); pool_free(c, d_items);
     }
 };
 
@@ -225,10 +366,14 @@ int sss_ctx_create(int device, sss_ctx** out)
     if (!c) return fail(SSS_ERR_NOMEM, "sss_ctx_create: out of host memory");
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess)
+        e = hipHostMalloc(reinterpret_cast<void**>(&c->pinned), kPinnedSlots * sizeof(int64_t), hipHostMallocDefault);
     if (e != hipSuccess) {
+        if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
-        return hip_fail(e, "hipStreamCreateWithFlags");
+        return hip_fail(e, "sss_ctx_create");
     }
+    for (int i = kPinnedSlots - 1; i >= 0; --i) c->pinned_free.push_back(i);
     *out = c;
     return SSS_OK;
 }
@@ -238,8 +383,41 @@ int sss_ctx_destroy(sss_ctx* ctx)
     if (!ctx) return SSS_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->pool.release_idle();
+    for (auto& kv : ctx->host_owned) (void)hipHostFree(kv.first);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+    return SSS_OK;
+}
+
+int sss_ctx_release_cache(sss_ctx* ctx)
+{
+    if (!ctx) return fail(SSS_ERR_INVALID, "sss_ctx_release_cache: NULL context");
+    SSS_HIP(hipSetDevice(ctx->device));
+    SSS_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->pool.release_idle();
+    return SSS_OK;
+}
+
+int sss_host_alloc(sss_ctx* ctx, int64_t bytes, void** out)
+{
+    if (!ctx || !out || bytes < 0) return fail(SSS_ERR_INVALID, "sss_host_alloc: bad argument");
+    return host_block(ctx, (size_t)bytes, out);
+}
+
+int sss_host_free(sss_ctx* ctx, void* p)
+{
+    if (!ctx) return fail(SSS_ERR_INVALID, "sss_host_free: NULL context");
+    if (!ctx->host_owned.count(p)) return fail(SSS_ERR_INVALID, "sss_host_free: not from sss_host_alloc");
+    host_block_free(ctx, p);
+    return SSS_OK;
+}
+
+int sss_ctx_cached_bytes(sss_ctx* ctx, int64_t* bytes)
+{
+    if (!ctx || !bytes) return fail(SSS_ERR_INVALID, "sss_ctx_cached_bytes: NULL argument");
+    *bytes = (int64_t)ctx->pool.idle_bytes;
     return SSS_OK;
 }
 
@@ -464,58 +642,86 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
     std::vector<float> ratio32(d->ratio, d->ratio + d->n_energy);
 
     int64_t& t = s->device_bytes;
-    if ((rc = dev_alloc(&s->d_mu, (size_t)s->n_vox, &t)) || (rc = dev_alloc(&s->d_lam, (size_t)s->n_vox, &t)) ||
-        (rc = dev_alloc(&s->d_sp, (size_t)std::max(1, d->n_sp) * 3, &t)) ||
-        (rc = dev_alloc(&s->d_detpos, detpos.size(), &t)) || (rc = dev_alloc(&s->d_detnorm, detnorm.size(), &t)) ||
-        (rc = dev_alloc(&s->d_eff, (size_t)d->n_energy, &t)) || (rc = dev_alloc(&s->d_ratio, (size_t)d->n_energy, &t)) ||
-        (rc = dev_alloc(&s->d_tiles, tiles.size() + 1, &t)) ||
-        (rc = dev_alloc(&s->d_items, items.size() + 1, &t)) ||
-        (rc = dev_alloc(&s->d_slice_ptr, (size_t)d->n_slices + 1, &t)) ||
-        (rc = dev_alloc(&s->d_chunk_sp, chunk_sp.size(), &t)) ||
-        (rc = dev_alloc(&s->d_chunk_slice, chunk_slice.size() + 1, &t)) ||
-        (rc = dev_alloc(&s->d_dets, (size_t)d->ndu, &t)) || (rc = dev_alloc(&s->d_flag, 1, &t)) ||
-        (rc = dev_alloc(&s->hp.off, (size_t)s->n_hp + 1, &t)) || (rc = dev_alloc(&s->hp.n, (size_t)s->n_hp + 1, &t)) ||
-        (rc = dev_alloc(&s->hp.off_fold, (size_t)s->n_hp + 1, &t)) ||
-        (rc = dev_alloc(&s->hp.n_fold, (size_t)s->n_hp + 1, &t)) ||
-        (rc = dev_alloc(&s->hp.rec, (size_t)s->n_hp + 1, &t)) ||
-        (rc = dev_alloc(&s->hp.blk, (size_t)(s->n_hp + kWave - 1) / kWave + 1, &t)) ||
-        (rc = dev_alloc(&s->hp.blk_off, (size_t)(s->n_hp + kWave - 1) / kWave + 1, &t)) ||
-        (rc = dev_alloc(&s->hp.dist, (size_t)s->n_hp + 1, &t)) || (rc = dev_alloc(&s->hp.att, (size_t)s->n_hp + 1, &t)) ||
-        (rc = dev_alloc(&s->hp.lnatt, (size_t)s->n_hp + 1, &t)) || (rc = dev_alloc(&s->hp.act, (size_t)s->n_hp + 1, &t)) ||
-        (rc = dev_alloc(&s->hp.cosabs, (size_t)s->n_hp + 1, &t)) ||
-        (rc = dev_alloc(&s->hp.status, (size_t)s->n_hp + 1, &t)) ||
-        (rc = dev_alloc(reinterpret_cast<unsigned char**>(&s->d_out), (size_t)s->out_elems * elem + 1, &t)) ||
-        (rc = dev_alloc(reinterpret_cast<unsigned char**>(&s->d_part_sum),
+    if ((rc = pool_alloc(ctx, &s->d_mu, (size_t)s->n_vox, &t)) || (rc = pool_alloc(ctx, &s->d_lam, (size_t)s->n_vox, &t)) ||
+        (rc = pool_alloc(ctx, &s->d_sp, (size_t)std::max(1, d->n_sp) * 3, &t)) ||
+        (rc = pool_alloc(ctx, &s->d_detpos, detpos.size(), &t)) || (rc = pool_alloc(ctx, &s->d_detnorm, detnorm.size(), &t)) ||
+        (rc = pool_alloc(ctx, &s->d_eff, (size_t)d->n_energy, &t)) || (rc = pool_alloc(ctx, &s->d_ratio, (size_t)d->n_energy, &t)) ||
+        (rc = pool_alloc(ctx, &s->d_tiles, tiles.size() + 1, &t)) ||
+        (rc = pool_alloc(ctx, &s->d_items, items.size() + 1, &t)) ||
+        (rc = pool_alloc(ctx, &s->d_slice_ptr, (size_t)d->n_slices + 1, &t)) ||
+        (rc = pool_alloc(ctx, &s->d_chunk_sp, chunk_sp.size(), &t)) ||
+        (rc = pool_alloc(ctx, &s->d_chunk_slice, chunk_slice.size() + 1, &t)) ||
+        (rc = pool_alloc(ctx, &s->d_dets, (size_t)d->ndu, &t)) || (rc = pool_alloc(ctx, &s->d_flag, 1, &t)) ||
+        (rc = pool_alloc(ctx, &s->hp.off, (size_t)s->n_hp + 1, &t)) || (rc = pool_alloc(ctx, &s->hp.n, (size_t)s->n_hp + 1, &t)) ||
+        (rc = pool_alloc(ctx, &s->hp.off_fold, (size_t)s->n_hp + 1, &t)) ||
+        (rc = pool_alloc(ctx, &s->hp.n_fold, (size_t)s->n_hp + 1, &t)) ||
+        (rc = pool_alloc(ctx, &s->hp.rec, (size_t)s->n_hp + 1, &t)) ||
+        (rc = pool_alloc(ctx, &s->hp.blk, (size_t)(s->n_hp + kWave - 1) / kWave + 1, &t)) ||
+        (rc = pool_alloc(ctx, &s->hp.blk_off, (size_t)(s->n_hp + kWave - 1) / kWave + 1, &t)) ||
+        (rc = pool_alloc(ctx, &s->hp.dist, (size_t)s->n_hp + 1, &t)) || (rc = pool_alloc(ctx, &s->hp.att, (size_t)s->n_hp + 1, &t)) ||
+        (rc = pool_alloc(ctx, &s->hp.lnatt, (size_t)s->n_hp + 1, &t)) || (rc = pool_alloc(ctx, &s->hp.act, (size_t)s->n_hp + 1, &t)) ||
+        (rc = pool_alloc(ctx, &s->hp.cosabs, (size_t)s->n_hp + 1, &t)) ||
+        (rc = pool_alloc(ctx, &s->hp.status, (size_t)s->n_hp + 1, &t)) ||
+        (rc = pool_alloc(ctx, reinterpret_cast<unsigned char**>(&s->d_out), (size_t)s->out_elems * elem + 1, &t)) ||
+        (rc = pool_alloc(ctx, reinterpret_cast<unsigned char**>(&s->d_part_sum),
                         (size_t)s->n_slots * s->n_bins * elem * s->n_chunks + 1, &t)) ||
-        (rc = dev_alloc(&s->d_part_cnt, (size_t)s->n_slots * s->n_chunks + 1, &t))) {
+        (rc = pool_alloc(ctx, &s->d_part_cnt, (size_t)s->n_slots * s->n_chunks + 1, &t))) {
         delete s;
         return rc;
     }
     hipStream_t st = ctx->stream;
+    // The uploads are queued behind whatever the context's stream holds, from
+    // one pinned block that the subject keeps until destroy: nothing here
+    // waits for the stream.
+    struct Upload {
+        void* dst;
+        const void* src;
+        size_t bytes;
+    };
+    const Upload ups[] = {
+        {s->d_mu, d->mu, (size_t)s->n_vox * sizeof(double)},
+        {s->d_sp, d->sp_xyz, (size_t)d->n_sp * 3 * sizeof(double)},
+        {s->d_detpos, detpos.data(), detpos.size() * sizeof(double)},
+        {s->d_detnorm, detnorm.data(), detnorm.size() * sizeof(double)},
+        {s->d_eff, d->eff, (size_t)d->n_energy * sizeof(double)},
+        {s->d_ratio, ratio32.data(), ratio32.size() * sizeof(float)},
+        {s->d_tiles, tiles.data(), tiles.size() * sizeof(int4)},
+        {s->d_items, items.data(), items.size() * sizeof(int2)},
+        {s->d_slice_ptr, d->slice_ptr, ((size_t)d->n_slices + 1) * sizeof(int32_t)},
+        {s->d_chunk_sp, chunk_sp.data(), chunk_sp.size() * sizeof(int32_t)},
+        {s->d_chunk_slice, chunk_slice.data(), chunk_slice.size() * sizeof(int32_t)},
+        {s->d_dets, d->dets, (size_t)d->ndu * sizeof(int32_t)},
+    };
+    size_t up_bytes = 0;
+    for (const Upload& u : ups) up_bytes += (u.bytes + 255) / 256 * 256;
+    if ((rc = host_block(ctx, up_bytes, &s->h_upload))) {
+        delete s;
+        return rc;
+    }
     // cells no lane owns (i1 == i2 on same-ring planes) are never written: zero once
     hipError_t e = hipMemsetAsync(s->d_out, 0, (size_t)s->out_elems * elem, st);
-    if (e == hipSuccess)
-        e = hipMemsetAsync(s->d_part_sum, 0, (size_t)s->n_slots * s->n_bins * elem * s->n_chunks + 1, st);
-    if (e == hipSuccess)
-        e = hipMemsetAsync(s->d_part_cnt, 0, ((size_t)s->n_slots * s->n_chunks + 1) * sizeof(float), st);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(s->d_mu, d->mu, (size_t)s->n_vox * sizeof(double), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess && d->n_sp > 0)
-        e = hipMemcpyAsync(s->d_sp, d->sp_xyz, (size_t)d->n_sp * 3 * sizeof(double), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(s->d_detpos, detpos.data(), detpos.size() * sizeof(double), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(s->d_detnorm, detnorm.data(), detnorm.size() * sizeof(double), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(s->d_eff, d->eff, (size_t)d->n_energy * sizeof(double), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(s->d_ratio, ratio32.data(), ratio32.size() * sizeof(float), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess && !tiles.empty()) e = hipMemcpyAsync(s->d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess && !items.empty()) e = hipMemcpyAsync(s->d_items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(s->d_slice_ptr, d->slice_ptr, ((size_t)d->n_slices + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(s->d_chunk_sp, chunk_sp.data(), chunk_sp.size() * sizeof(int32_t), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess && !chunk_slice.empty()) e = hipMemcpyAsync(s->d_chunk_slice, chunk_slice.data(), chunk_slice.size() * sizeof(int32_t), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(s->d_dets, d->dets, (size_t)d->ndu * sizeof(int32_t), hipMemcpyHostToDevice, st);
+    size_t off = 0;
+    for (const Upload& u : ups) {
+        if (e != hipSuccess || u.bytes == 0) continue;
+        unsigned char* stage = static_cast<unsigned char*>(s->h_upload) + off;
+        std::memcpy(stage, u.src, u.bytes);
+        e = hipMemcpyAsync(u.dst, stage, u.bytes, hipMemcpyHostToDevice, st);
+        off += (u.bytes + 255) / 256 * 256;
+    }
     if (e == hipSuccess) e = hipEventCreate(&s->ev_scatter[0]);
     if (e == hipSuccess) e = hipEventCreate(&s->ev_scatter[1]);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_count, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_out, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_lam, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_upload, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(s->ev_upload, st);
+    if (e == hipSuccess && ctx->pinned_free.empty()) e = hipErrorOutOfMemory;   // kPinnedSlots live subjects
+    if (e == hipSuccess) {
+        s->h_total = ctx->pinned + ctx->pinned_free.back();
+        ctx->pinned_free.pop_back();
+    }
     if (e != hipSuccess) {
+        (void)hipStreamSynchronize(st);   // queued uploads read the subject's host copies
         delete s;
         return hip_fail(e, "sss_subject_create: upload");
     }
@@ -527,7 +733,13 @@ int sss_subject_destroy(sss_subject* s)
 {
     if (!s) return SSS_OK;
     (void)hipSetDevice(s->ctx->device);
-    (void)hipStreamSynchronize(s->ctx->stream);
+    // the device blocks go back to the context's cache: their next users are
+    // queued behind this subject's work on the same stream; the pinned count
+    // may still be the target of a queued copy
+    if (s->trace_begun) (void)hipEventSynchronize(s->ev_count);
+    // the queued uploads read host memory the subject owns
+    if (s->ev_upload) (void)hipEventSynchronize(s->ev_upload);
+    if (s->lam_queued) (void)hipEventSynchronize(s->ev_lam);
     delete s;
     return SSS_OK;
 }
@@ -541,9 +753,9 @@ static int scan_counts(sss_subject* s, It counts, int64_t* out, int64_t n_items,
     SSS_HIP(rocprim::exclusive_scan(nullptr, need, counts, out, (int64_t)0, (size_t)n_items + 1,
                                     rocprim::plus<int64_t>(), st));
     if (need > s->scan_tmp_bytes) {
-        dev_free(s->d_scan_tmp);
+        pool_free(s->ctx, s->d_scan_tmp);
         unsigned char* tmp = nullptr;
-        int rc = dev_alloc(&tmp, need, &s->device_bytes);
+        int rc = pool_alloc(s->ctx, &tmp, need, &s->device_bytes);
         if (rc) return rc;
         s->d_scan_tmp = tmp;
         s->scan_tmp_bytes = need;
@@ -566,17 +778,15 @@ static int refresh_fold(sss_subject* s)
     return SSS_OK;
 }
 
-int sss_subject_trace(sss_subject* s)
+int sss_subject_trace_begin(sss_subject* s)
 {
-    if (!s) return fail(SSS_ERR_INVALID, "sss_subject_trace: NULL subject");
+    if (!s) return fail(SSS_ERR_INVALID, "sss_subject_trace_begin: NULL subject");
     SSS_HIP(hipSetDevice(s->ctx->device));
     hipStream_t st = s->ctx->stream;
     s->traced = false;
-    if (s->n_hp == 0) {
-        s->n_samples = 0;
-        s->traced = true;
-        return SSS_OK;
-    }
+    s->trace_begun = true;
+    *s->h_total = 0;
+    if (s->n_hp == 0) return SSS_OK;
     SSS_HIP(launch_halfpath_count(s->grid, s->d_sp, s->n_sp, s->d_detpos, s->d_detnorm, s->nru, s->ndu,
                                   s->hp, st));
     // the last slot holds a zero count so the scan's last entry is the total
@@ -584,26 +794,44 @@ int sss_subject_trace(sss_subject* s)
     SSS_HIP(hipMemsetAsync(s->hp.blk + n_blk, 0, sizeof(int32_t), st));
     SSS_HIP(hipMemsetAsync(s->hp.n_fold + s->n_hp, 0, sizeof(int32_t), st));
     if (int rc = scan_counts(s, s->hp.blk, s->hp.blk_off, n_blk, st)) return rc;
-    int64_t total = 0;
-    SSS_HIP(hipMemcpyAsync(&total, s->hp.blk_off + n_blk, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    SSS_HIP(hipStreamSynchronize(st));
+    SSS_HIP(hipMemcpyAsync(s->h_total, s->hp.blk_off + n_blk, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    SSS_HIP(hipEventRecord(s->ev_count, st));
+    return SSS_OK;
+}
+
+int sss_subject_trace_end(sss_subject* s)
+{
+    if (!s) return fail(SSS_ERR_INVALID, "sss_subject_trace_end: NULL subject");
+    if (!s->trace_begun) return fail(SSS_ERR_INVALID, "sss_subject_trace_end: call sss_subject_trace_begin first");
+    s->trace_begun = false;
+    SSS_HIP(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->ctx->stream;
+    if (s->n_hp == 0) {
+        s->n_samples = 0;
+        s->traced = true;
+        return SSS_OK;
+    }
+    SSS_HIP(hipEventSynchronize(s->ev_count));
+    const int64_t total = *s->h_total;
     if (total > s->samples_cap || !s->smp.nq) {
-        dev_free(s->smp.nq); dev_free(s->smp.vox); dev_free(s->smp.len); dev_free(s->smp.a); dev_free(s->smp.a64);
-        dev_free(s->smp.fold); dev_free(s->smp.fold_a64);
+        sss_ctx* c = s->ctx;
+        pool_free(c, s->smp.nq); pool_free(c, s->smp.vox); pool_free(c, s->smp.len); pool_free(c, s->smp.a);
+        pool_free(c, s->smp.a64); pool_free(c, s->smp.fold); pool_free(c, s->smp.fold_a64);
         s->samples_cap = 0;
         // the fold table holds at most every sample plus the group padding,
         // and spare records after the last one (the fold prefetches ahead)
         size_t fold_cap = (size_t)total + (size_t)(kSampleGroup - 1) * s->n_hp + kSamplePad;
         fold_cap = (fold_cap + kSampleGroup - 1) / kSampleGroup * kSampleGroup;
         const bool fp64 = s->precision == SSS_PRECISION_FP64;
+        int64_t* tally = &s->device_bytes;
         int rc;
-        if ((rc = dev_alloc(&s->smp.nq, (size_t)total, &s->device_bytes)) ||
-            (rc = dev_alloc(&s->smp.vox, (size_t)total, &s->device_bytes)) ||
-            (rc = dev_alloc(&s->smp.len, (size_t)total, &s->device_bytes)) ||
-            (rc = dev_alloc(&s->smp.a, (size_t)total, &s->device_bytes)) ||
-            (rc = dev_alloc(&s->smp.fold, fold_cap / kSampleGroup * kFoldGroupWords, &s->device_bytes)) ||
-            (fp64 && ((rc = dev_alloc(&s->smp.a64, (size_t)total, &s->device_bytes)) ||
-                      (rc = dev_alloc(&s->smp.fold_a64, fold_cap, &s->device_bytes)))))
+        if ((rc = pool_alloc(c, &s->smp.nq, (size_t)total, tally)) ||
+            (rc = pool_alloc(c, &s->smp.vox, (size_t)total, tally)) ||
+            (rc = pool_alloc(c, &s->smp.len, (size_t)total, tally)) ||
+            (rc = pool_alloc(c, &s->smp.a, (size_t)total, tally)) ||
+            (rc = pool_alloc(c, &s->smp.fold, fold_cap / kSampleGroup * kFoldGroupWords, tally)) ||
+            (fp64 && ((rc = pool_alloc(c, &s->smp.a64, (size_t)total, tally)) ||
+                      (rc = pool_alloc(c, &s->smp.fold_a64, fold_cap, tally)))))
             return rc;
         s->samples_cap = total;
     }
@@ -619,6 +847,12 @@ int sss_subject_trace(sss_subject* s)
     return SSS_OK;
 }
 
+int sss_subject_trace(sss_subject* s)
+{
+    if (int rc = sss_subject_trace_begin(s)) return rc;
+    return sss_subject_trace_end(s);
+}
+
 static int refresh_activity(sss_subject* s)
 {
     s->has_activity = true;
@@ -630,8 +864,19 @@ int sss_subject_set_activity(sss_subject* s, const double* lam)
 {
     if (!s || !lam) return fail(SSS_ERR_INVALID, "sss_subject_set_activity: NULL argument");
     SSS_HIP(hipSetDevice(s->ctx->device));
-    SSS_HIP(hipMemcpyAsync(s->d_lam, lam, (size_t)s->n_vox * sizeof(double), hipMemcpyHostToDevice,
+    // the caller's buffer is copied before the call returns; the queued upload
+    // reads the subject's copy (the previous upload from it has finished)
+    if (s->lam_queued) SSS_HIP(hipEventSynchronize(s->ev_lam));
+    if (!s->h_lam) {
+        void* p = nullptr;
+        if (int rc = host_block(s->ctx, (size_t)s->n_vox * sizeof(double), &p)) return rc;
+        s->h_lam = static_cast<double*>(p);
+    }
+    std::memcpy(s->h_lam, lam, (size_t)s->n_vox * sizeof(double));
+    SSS_HIP(hipMemcpyAsync(s->d_lam, s->h_lam, (size_t)s->n_vox * sizeof(double), hipMemcpyHostToDevice,
                            s->ctx->stream));
+    SSS_HIP(hipEventRecord(s->ev_lam, s->ctx->stream));
+    s->lam_queued = true;
     return refresh_activity(s);
 }
 
@@ -787,6 +1032,25 @@ int sss_subject_info_get(sss_subject* s, sss_subject_info* out)
 
 const void* sss_subject_output_device(const sss_subject* s) { return s ? s->d_out : nullptr; }
 
+int sss_subject_output_compact_async(sss_subject* s, float* host_out)
+{
+    if (!s || !host_out) return fail(SSS_ERR_INVALID, "sss_subject_output_compact_async: NULL argument");
+    SSS_HIP(hipSetDevice(s->ctx->device));
+    const size_t elem = s->precision == SSS_PRECISION_FP64 ? sizeof(double) : sizeof(float);
+    SSS_HIP(hipMemcpyAsync(host_out, s->d_out, (size_t)s->out_elems * elem, hipMemcpyDeviceToHost,
+                           s->ctx->stream));
+    SSS_HIP(hipEventRecord(s->ev_out, s->ctx->stream));
+    return SSS_OK;
+}
+
+int sss_subject_wait(sss_subject* s)
+{
+    if (!s) return fail(SSS_ERR_INVALID, "sss_subject_wait: NULL subject");
+    SSS_HIP(hipSetDevice(s->ctx->device));
+    SSS_HIP(hipEventSynchronize(s->ev_out));
+    return SSS_OK;
+}
+
 int sss_subject_output_compact(sss_subject* s, float* host_out)
 {
     if (!s || !host_out) return fail(SSS_ERR_INVALID, "sss_subject_output_compact: NULL argument");
@@ -794,7 +1058,9 @@ int sss_subject_output_compact(sss_subject* s, float* host_out)
     const size_t elem = s->precision == SSS_PRECISION_FP64 ? sizeof(double) : sizeof(float);
     SSS_HIP(hipMemcpyAsync(host_out, s->d_out, (size_t)s->out_elems * elem, hipMemcpyDeviceToHost,
                            s->ctx->stream));
-    SSS_HIP(hipStreamSynchronize(s->ctx->stream));
+    // wait for this subject's work only, not for what later calls queued
+    SSS_HIP(hipEventRecord(s->ev_out, s->ctx->stream));
+    SSS_HIP(hipEventSynchronize(s->ev_out));
     return SSS_OK;
 }
 

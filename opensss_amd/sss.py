@@ -116,6 +116,15 @@ class ScatterSubject:
         _native.check(self._lib.sss_subject_trace(self.handle), 'sss_subject_trace')
         self._traced = True
 
+    def trace_begin(self):
+        """Queue the walk counts (asynchronous); finish with trace_end."""
+        _native.check(self._lib.sss_subject_trace_begin(self.handle), 'sss_subject_trace_begin')
+
+    def trace_end(self):
+        """Wait for this subject's counts, size its tables, queue the walks."""
+        _native.check(self._lib.sss_subject_trace_end(self.handle), 'sss_subject_trace_end')
+        self._traced = True
+
     def set_activity(self, ActivityMap):
         lam = np.asarray(ActivityMap)
         if tuple(lam.shape) != self.grid:
@@ -179,6 +188,27 @@ class ScatterSubject:
         _native.check(self._lib.sss_subject_output_compact(self.handle, _native.ptr(out)),
                       'sss_subject_output_compact')
         return out
+
+    def output_compact_async(self, host_addr: int):
+        """Queue the compact download into pinned host memory at host_addr
+        (Context.host_alloc, nbytes = output_nbytes); wait() waits for it."""
+        _native.check(self._lib.sss_subject_output_compact_async(self.handle, _native.ctypes.c_void_p(host_addr)),
+                      'sss_subject_output_compact_async')
+
+    def wait(self):
+        _native.check(self._lib.sss_subject_wait(self.handle), 'sss_subject_wait')
+
+    @property
+    def output_shape(self):
+        return (self.n_bins, self.nru * self.nru, self.n_cells_plane)
+
+    @property
+    def output_dtype(self):
+        return np.float64 if self.precision == 'fp64' else np.float32
+
+    @property
+    def output_nbytes(self) -> int:
+        return int(np.prod(self.output_shape)) * np.dtype(self.output_dtype).itemsize
 
     def output_dense(self, SinogramCoordinates) -> np.ndarray:
         """float32 (B, N+1, N//2, NRU**2) as the reference returns it."""
