@@ -11,6 +11,7 @@ sum, FP32 VALU), the error against an FP64 run of the same kernels, and a CPU
 baseline (the float64 oracle timed on a bounded scatter-point sample).
 """
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -42,6 +43,15 @@ def parse():
     p.add_argument('--streams', type=int, default=4, help='contexts of the batched path')
     p.add_argument('--batch-kind', default='random', choices=['random', 'config1'],
                    help='uneven random instances (seeds 0..N-1) or config 1 instances of N seeds')
+    p.add_argument('--serve', type=float, default=0.0, metavar='SECONDS',
+                   help='instead of the default line: an open-loop Poisson request stream of this '
+                        'length over resident subjects (throughput, p50/p99 latency)')
+    p.add_argument('--rate', type=float, default=50.0, help='requests per second of --serve')
+    p.add_argument('--subjects', type=int, default=16, help='subjects of --serve (uneven random instances)')
+    p.add_argument('--subject-kind', default='random', choices=['random', 'config1'],
+                   help='subjects of --serve: uneven random instances or config 1 instances')
+    p.add_argument('--noise-fields', type=int, default=3,
+                   help='precomputed 5%% noise fields per subject that the requests cycle through')
     return p.parse_args()
 
 
@@ -89,6 +99,97 @@ def batch_bench(args):
         'one_instance_path': {'value': evals / t_plain, 'seconds': t_plain},
         'batched_path': {'value': evals / t_batch, 'seconds': t_batch},
         'speedup': t_plain / t_batch, 'outputs_identical': bool(same),
+    }
+    print(json.dumps(line))
+
+
+def serve_bench(args):
+    """Served reading: Poisson arrivals at --rate for --serve seconds, each
+    request a subject (uniform over --subjects) and its activity times
+    (1 + 0.05 N(0,1)) per voxel.  Subjects are loaded (created and traced)
+    before the stream starts and stay resident.  Reports completed
+    evaluations per second over the stream, request latency (arrival to
+    result in host memory) p50/p99, the one-instance path's rate on the
+    first requests, and the FP32 error against FP64 runs of sampled requests."""
+    from opensss_amd import phantoms
+    from opensss_amd.batch import run_plain
+    from opensss_amd.serve import Request, ScatterServer, serve
+    water = np.load(os.path.join(ROOT, 'opensss_amd', 'data', 'attenuation_water.npy'))
+    rng = np.random.default_rng(2024)
+    if args.subject_kind == 'config1':
+        subs = {k: phantoms.config1(seed=k) for k in range(args.subjects)}
+    else:
+        subs = {k: phantoms.random_instance(k) for k in range(args.subjects)}
+    fields = {k: [w.activity * (1.0 + 0.05 * rng.standard_normal(w.activity.shape))
+                  for _ in range(max(1, args.noise_fields))] for k, w in subs.items()}
+    n_req = int(rng.poisson(args.rate * args.serve))
+    arrivals = np.sort(rng.uniform(0.0, args.serve, size=n_req))
+    keys = rng.integers(0, args.subjects, size=n_req)
+    reqs = [Request(key=int(k), activity=fields[int(k)][i % len(fields[int(k)])], arrival=float(t), tag=i)
+            for i, (t, k) in enumerate(zip(arrivals, keys))]
+    evals = {k: instance_evals(w) for k, w in subs.items()}
+    print(f'[bench] serve: {n_req} requests over {args.serve:.0f} s, {args.subjects} subjects', file=sys.stderr,
+          flush=True)
+
+    server = ScatterServer(subs, water, n_contexts=args.streams)
+    t0 = time.perf_counter()
+    for k in subs:
+        server.load(k)
+    for c in server.ctxs:
+        c.synchronize()
+    t_load = time.perf_counter() - t0
+    # warm-up: one request per subject (not timed)
+    serve(server, [Request(key=k, activity=subs[k].activity) for k in subs], on_result=lambda r, v: None)
+    sample = set(range(0, n_req, max(1, n_req // 4)))
+    kept = {}
+
+    def on_result(req, view):
+        if req.tag in sample:
+            kept[req.tag] = view.copy()
+
+    start = serve(server, reqs, on_result=on_result)
+    t_end = max(r.done for r in reqs)
+    server_stats = dict(server.stats)
+    resident = server.resident_bytes()
+    server.close()
+    lat = np.array([r.done - r.arrival for r in reqs])
+    total = sum(evals[r.key] for r in reqs)
+    served_rate = total / t_end
+    # the one-instance path (create, trace, scatter, download, destroy) on the first requests
+    head = reqs[:min(16, n_req)]
+    plain_inst = []
+    for r in head:
+        w = subs[r.key]
+        plain_inst.append(dataclasses.replace(w, activity=r.activity))
+    run_plain(plain_inst[:1], water)
+    t0 = time.perf_counter()
+    run_plain(plain_inst, water)
+    t_plain = time.perf_counter() - t0
+    plain_rate = sum(evals[r.key] for r in head) / t_plain
+    # FP32 against FP64 on the sampled requests
+    errs = []
+    for i in sorted(kept):
+        r = reqs[i]
+        w = dataclasses.replace(subs[r.key], activity=r.activity)
+        ref = run_plain([w], water, precision='fp64')[0]
+        got = kept[i].astype(np.float64)
+        peak = np.abs(ref).max()
+        m = np.abs(ref) > 0.01 * peak
+        errs.append(float((np.abs(got - ref)[m] / np.abs(ref)[m]).max()) if m.any() else 0.0)
+    line = {
+        'metric': 'served scatter-pt x LOR x TOF-bin evals/s; p50/p99 request latency',
+        'value': served_rate, 'unit': 'evals/s', 'n_gpus': 1, 'higher_is_better': True, 'dtype': 'f32',
+        'data': 'synthetic',
+        'config': {'workload': f'{args.subject_kind}-subjects-0..{args.subjects - 1}', 'requests': n_req,
+                   'offered_rate_per_s': args.rate, 'seconds': args.serve, 'contexts': args.streams,
+                   'noise': '5% multiplicative per voxel'},
+        'completed_per_s': n_req / t_end,
+        'latency_ms': {'p50': float(np.percentile(lat, 50) * 1e3), 'p99': float(np.percentile(lat, 99) * 1e3),
+                       'max': float(lat.max() * 1e3)},
+        'subject_load_s': t_load, 'resident_bytes': int(resident), 'server': server_stats,
+        'one_instance_path': {'value': plain_rate, 'requests': len(head), 'seconds': t_plain},
+        'speedup': served_rate / plain_rate,
+        'max_rel_err_vs_fp64': max(errs) if errs else None,
     }
     print(json.dumps(line))
 
@@ -182,7 +283,7 @@ def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py runs on one GPU only: the scatter estimate of one instance is measured '
-              'on one MI355X (multi-GPU sharding over z slices is described in DESIGN.md, not run)',
+              'on one MI355X (the path would shard over z slices across GPUs; not run here)',
               file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:
@@ -190,6 +291,9 @@ def main():
         return
     if args.batch:
         batch_bench(args)
+        return
+    if args.serve > 0:
+        serve_bench(args)
         return
     w = make_workload(args.workload)
     print(f'[bench] workload {w.name}: {len(w.scatter_points)} sp, grid {w.grid}, '

@@ -79,7 +79,14 @@ typedef struct {
     int32_t precision;           /* SSS_PRECISION_* */
     int32_t n_chunks;            /* target number of scatter-point chunks (each inside one
                                     slice) for the scatter launch; 0 = auto */
+    int32_t volume_order;        /* layout of mu here and of every activity image (host or
+                                    device) given to the subject: SSS_VOLUME_X_FASTEST (index
+                                    x + X y + X Y z, the reference's flatten(order='F')) or
+                                    SSS_VOLUME_Z_FASTEST (C order of an (X, Y, Z) array; the
+                                    subject transposes on the device) */
 } sss_subject_desc;
+
+enum { SSS_VOLUME_X_FASTEST = 0, SSS_VOLUME_Z_FASTEST = 1 };
 
 typedef struct {
     int64_t n_halfpaths;
@@ -111,6 +118,9 @@ void* sss_ctx_stream(sss_ctx* ctx);
  * the idle blocks; cached_bytes reports their size. */
 int sss_ctx_release_cache(sss_ctx* ctx);
 int sss_ctx_cached_bytes(sss_ctx* ctx, int64_t* bytes);
+/* Free and total device memory of the context's GPU (hipMemGetInfo), for
+ * callers that admit work by its device-memory footprint. */
+int sss_ctx_memory(sss_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
 /* Pinned host buffers of a context, kept for reuse after sss_host_free
  * (hipHostMalloc/hipHostFree synchronise the device). */
 int sss_host_alloc(sss_ctx* ctx, int64_t bytes, void** out);
@@ -142,9 +152,10 @@ int sss_subject_trace(sss_subject* s);
  * tables and queues the walks (asynchronous after that wait). */
 int sss_subject_trace_begin(sss_subject* s);
 int sss_subject_trace_end(sss_subject* s);
-/* Upload lam (X*Y*Z f64, host) and refresh the per-sample emission terms. */
+/* Upload lam (X*Y*Z f64, host, in the subject's volume_order) and refresh
+ * the per-sample emission terms. */
 int sss_subject_set_activity(sss_subject* s, const double* lam);
-/* Same from a device pointer (X*Y*Z f64), no host copy. */
+/* Same from a device pointer (X*Y*Z f64, volume_order), no host copy. */
 int sss_subject_set_activity_device(sss_subject* s, const void* lam_dev);
 /* Scatter sum into the subject's device output (compact layout, float32
  * [B][NRU^2][NDU(NDU+1)/2], cell (i1 <= i2) at i1*NDU - i1(i1-1)/2 + i2 - i1). */
@@ -170,6 +181,8 @@ int sss_subject_output_compact(sss_subject* s, float* host_out);
  * reference). */
 int sss_subject_output_compact_async(sss_subject* s, float* host_out);
 int sss_subject_wait(sss_subject* s);
+/* done = 1 when the last queued download (and the work before it) finished. */
+int sss_subject_query(sss_subject* s, int32_t* done);
 /* Dense reference layout (B, N+1, N/2, NRU^2) float32 on the host, using
  * lor_coords (N, N, 2) int16 = (angular, radial). */
 int sss_subject_output_dense(sss_subject* s, const int16_t* lor_coords, float* host_out);

@@ -18,6 +18,8 @@ LIB_PATH = os.path.join(_HERE, '_lib', 'libsss_hip.so')
 SSS_OK = 0
 SSS_PRECISION_FP32 = 0
 SSS_PRECISION_FP64 = 1
+SSS_VOLUME_X_FASTEST = 0
+SSS_VOLUME_Z_FASTEST = 1
 
 _c_i32p = ctypes.POINTER(ctypes.c_int32)
 _c_f64p = ctypes.POINTER(ctypes.c_double)
@@ -49,7 +51,7 @@ class SubjectDesc(ctypes.Structure):
         ('tof_beta', ctypes.c_double), ('tof_half_window', ctypes.c_int32),
         ('tof_t_lo', ctypes.c_int32), ('tof_t_hi', ctypes.c_int32),
         ('tof_trivial', ctypes.c_int32), ('precision', ctypes.c_int32),
-        ('n_chunks', ctypes.c_int32),
+        ('n_chunks', ctypes.c_int32), ('volume_order', ctypes.c_int32),
     ]
 
 
@@ -73,6 +75,7 @@ SIGNATURES = [
     ('sss_ctx_stream', ctypes.c_void_p, [ctypes.c_void_p]),
     ('sss_ctx_release_cache', ctypes.c_int, [ctypes.c_void_p]),
     ('sss_ctx_cached_bytes', ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
+    ('sss_ctx_memory', ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     ('sss_host_alloc', ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)]),
     ('sss_host_free', ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ('sss_trace_segments', ctypes.c_int,
@@ -100,6 +103,7 @@ SIGNATURES = [
     ('sss_subject_output_compact', ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ('sss_subject_output_compact_async', ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ('sss_subject_wait', ctypes.c_int, [ctypes.c_void_p]),
+    ('sss_subject_query', ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
     ('sss_subject_output_dense', ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ('sss_buffer_alloc', ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)]),
     ('sss_buffer_free', ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
@@ -210,6 +214,12 @@ class Context:
 
     def synchronize(self) -> None:
         check(self._lib.sss_ctx_synchronize(self.handle), 'sss_ctx_synchronize')
+
+    def memory(self):
+        """(free, total) device bytes of the context's GPU."""
+        f, t = ctypes.c_int64(), ctypes.c_int64()
+        check(self._lib.sss_ctx_memory(self.handle, ctypes.byref(f), ctypes.byref(t)), 'sss_ctx_memory')
+        return f.value, t.value
 
     def release_cache(self) -> None:
         """Free the device blocks kept for later subjects."""

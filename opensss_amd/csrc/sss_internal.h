@@ -23,6 +23,12 @@ constexpr int kTileRows = 8;
 constexpr int kTileCols = 8;
 constexpr int kTileMaxRings = 4;
 constexpr int kScatterWaves = 4;          // waves per workgroup of the scatter launch
+// Energy rows a pair can reach: e = floor(511 / (2 + cos) + 1/2) keV for
+// cos in [-1, 1] is 170..511 keV, row 2e - 1.  The LDS energy table of the
+// one-bin scatter holds these rows and row 0 (reached by a NaN cosine only).
+constexpr int kEnergyRowLo = 2 * 170 - 1;
+constexpr int kEnergyRowHi = 2 * 511 - 1;
+constexpr int kEnergyRows = kEnergyRowHi - kEnergyRowLo + 2;
 constexpr int kAutoChunkPoints = 64;      // at most this many scatter points per chunk by default
 constexpr int kMaxBins = 32;
 
@@ -70,10 +76,14 @@ struct HalfPathTable {
 
 // One half path as the FP32 pair evaluation reads it: two 16-byte loads.
 // The zero tests of the float64 attenuation and activity are kept as flags,
-// so they stay exact.
+// so they stay exact.  A one-bin subject has no fold table: its records hold
+// the activity integral in place of the first fold record.
 struct alignas(16) PairHalf {
     double dist;       // |D - S|, cm
-    int64_t off;       // first fold record
+    union {
+        int64_t off;   // first fold record (TOF subjects)
+        double act;    // sum lam L (one-bin subjects)
+    };
     int32_t n_flags;   // fold records | kHalfAttZero | kHalfActive
     float cosabs;
     float lnatt;       // sum mu L
@@ -124,6 +134,8 @@ struct ScatterArgs {
     int64_t n_cells_plane;       // NDU (NDU+1)/2
     // tables
     const PairHalf* hp_rec;      // per half path, FP32 pair evaluation
+    const float2* etab;          // kEnergyRows: (eff * eff_511 or -1 where 0, ratio)
+    int etab_lo, etab_hi;        // rows kEnergyRowLo/Hi clamped into the tables
     const int64_t* hp_off;       // first fold record of each half path
     const int32_t* hp_n;         // fold records (samples with lam L != 0)
     const int32_t* hp_n_all;     // all samples of each half path
@@ -190,8 +202,11 @@ hipError_t launch_halfpath_fill(const Grid& g, const double* sp_xyz, int n_sp, c
 // the non-zero terms into the fold table.
 hipError_t launch_activity_refresh(const HalfPathTable t, SampleTable s, int64_t n_hp,
                                    const double* lam, hipStream_t st);
+hipError_t launch_volume_x_fastest(const double* src, double* dst, int X, int Y, int Z, hipStream_t st);
+hipError_t launch_activity_sum(const HalfPathTable t, SampleTable s, int64_t n_hp,
+                               const double* lam, hipStream_t st);
 hipError_t launch_fold_compact(const HalfPathTable t, SampleTable s, int64_t n_hp, hipStream_t st);
-hipError_t launch_pair_half(const HalfPathTable t, int64_t n_hp, hipStream_t st);
+hipError_t launch_pair_half(const HalfPathTable t, int64_t n_hp, bool one_bin, hipStream_t st);
 hipError_t launch_trace_segments(const Grid& g, const double* segs, int64_t n, int cap,
                                  int32_t* counts, double* L, int32_t* idx, double* s,
                                  hipStream_t st);

@@ -289,8 +289,14 @@ struct sss_subject {
     double* d_sp = nullptr;
     double* d_detpos = nullptr;
     double* d_detnorm = nullptr;
+    unsigned char* d_upload = nullptr;    // one block holding every array uploaded at create
+    bool z_fastest = false;               // volumes arrive in C order of (X, Y, Z)
+    double* d_vol_in = nullptr;           // z fastest: mu's upload slot, later each activity upload
+    double* d_mu_own = nullptr;           // z fastest: mu, x fastest
     double* d_eff = nullptr;
     float* d_ratio = nullptr;
+    float2* d_etab = nullptr;             // kEnergyRows energy terms of the one-bin scatter
+    int etab_lo = 0, etab_hi = 0;
     int4* d_tiles = nullptr;
     int2* d_items = nullptr;          // scatter launch: (tile, chunk) per wave, XCD-aware order
     int n_items = 0;
@@ -334,17 +340,16 @@ struct sss_subject {
         if (ev_lam) (void)hipEventDestroy(ev_lam);
         if (ev_upload) (void)hipEventDestroy(ev_upload);
         sss_ctx* c = ctx;
-        pool_free(c, d_mu); pool_free(c, d_lam); pool_free(c, d_sp); pool_free(c, d_detpos);
-        pool_free(c, d_detnorm); pool_free(c, d_eff); pool_free(c, d_ratio); pool_free(c, d_tiles);
-        pool_free(c, d_slice_ptr); pool_free(c, d_chunk_sp); pool_free(c, d_chunk_slice); pool_free(c, d_dets);
-        pool_free(c, d_flag); pool_free(c, d_scan_tmp);
+        pool_free(c, d_upload);   // d_mu (x fastest) .. d_dets live in it
+        pool_free(c, d_mu_own);
+        pool_free(c, d_lam); pool_free(c, d_flag); pool_free(c, d_scan_tmp);
         pool_free(c, hp.off); pool_free(c, hp.n); pool_free(c, hp.off_fold); pool_free(c, hp.n_fold);
         pool_free(c, hp.rec); pool_free(c, hp.blk); pool_free(c, hp.blk_off); pool_free(c, hp.dist);
         pool_free(c, hp.att); pool_free(c, hp.lnatt); pool_free(c, hp.act); pool_free(c, hp.cosabs);
         pool_free(c, hp.status);
         pool_free(c, smp.nq); pool_free(c, smp.vox); pool_free(c, smp.len); pool_free(c, smp.a);
         pool_free(c, smp.a64); pool_free(c, smp.fold); pool_free(c, smp.fold_a64);
-        pool_free(c, d_out); pool_free(c, d_part_sum); pool_free(c, d_part_cnt); pool_free(c, d_items);
+        pool_free(c, d_out); pool_free(c, d_part_sum); pool_free(c, d_part_cnt);
     }
 };
 
@@ -418,6 +423,17 @@ int sss_ctx_cached_bytes(sss_ctx* ctx, int64_t* bytes)
 {
     if (!ctx || !bytes) return fail(SSS_ERR_INVALID, "sss_ctx_cached_bytes: NULL argument");
     *bytes = (int64_t)ctx->pool.idle_bytes;
+    return SSS_OK;
+}
+
+int sss_ctx_memory(sss_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes)
+{
+    if (!ctx || !free_bytes || !total_bytes) return fail(SSS_ERR_INVALID, "sss_ctx_memory: NULL argument");
+    SSS_HIP(hipSetDevice(ctx->device));
+    size_t f = 0, t = 0;
+    SSS_HIP(hipMemGetInfo(&f, &t));
+    *free_bytes = (int64_t)f;
+    *total_bytes = (int64_t)t;
     return SSS_OK;
 }
 
@@ -547,6 +563,8 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
         return fail(SSS_ERR_INVALID, "sss_subject_create: bad energy tables");
     if (d->precision != SSS_PRECISION_FP32 && d->precision != SSS_PRECISION_FP64)
         return fail(SSS_ERR_INVALID, "sss_subject_create: bad precision");
+    if (d->volume_order != SSS_VOLUME_X_FASTEST && d->volume_order != SSS_VOLUME_Z_FASTEST)
+        return fail(SSS_ERR_INVALID, "sss_subject_create: bad volume_order");
     for (int r = 0; r < d->nru; ++r)
         if (d->rings[r] < 0 || d->rings[r] >= d->n_rings)
             return fail(SSS_ERR_INVALID, "sss_subject_create: ring id out of range");
@@ -576,6 +594,7 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
     const double ds = d->det_size[0] * d->det_size[1] * 1e-2;
     s->g_const = ds * ds / (4.0 * M_PI);
     s->n_vox = (int64_t)g.nx * g.ny * g.nz;
+    s->z_fastest = d->volume_order == SSS_VOLUME_Z_FASTEST;
     s->n_hp = (int64_t)d->n_sp * d->nru * d->ndu;
     s->n_cells_plane = (int64_t)d->ndu * (d->ndu + 1) / 2;
     s->out_elems = (int64_t)d->n_bins * d->nru * d->nru * s->n_cells_plane;
@@ -641,17 +660,49 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
     s->n_items = (int)items.size();
     std::vector<float> ratio32(d->ratio, d->ratio + d->n_energy);
 
+    // energy terms of the one-bin scatter: the rows a pair can reach, then row 0
+    std::vector<float2> etab(kEnergyRows);
+    s->etab_lo = std::min(kEnergyRowLo, d->n_energy - 1);
+    s->etab_hi = std::min(kEnergyRowHi, d->n_energy - 1);
+    {
+        const double eff_ref = d->eff[d->eff_ref_row];
+        for (int k = 0; k < kEnergyRows; ++k) {
+            const int row = k == kEnergyRows - 1 ? 0 : std::min(s->etab_lo + k, s->etab_hi);
+            const double e = d->eff[row] * eff_ref;
+            etab[k] = make_float2(e == 0.0 ? -1.0f : (float)e, ratio32[row]);
+        }
+    }
+    // Every array uploaded here shares one device block and one pinned
+    // staging block: one copy per subject.  Each array's slot has room for
+    // one element more than it holds (256-byte aligned).
+    struct Upload {
+        void** dst;
+        const void* src;
+        size_t bytes;
+    };
+    const Upload ups[] = {
+        {reinterpret_cast<void**>(s->z_fastest ? &s->d_vol_in : &s->d_mu), d->mu, (size_t)s->n_vox * sizeof(double)},
+        {reinterpret_cast<void**>(&s->d_sp), d->sp_xyz, (size_t)d->n_sp * 3 * sizeof(double)},
+        {reinterpret_cast<void**>(&s->d_detpos), detpos.data(), detpos.size() * sizeof(double)},
+        {reinterpret_cast<void**>(&s->d_detnorm), detnorm.data(), detnorm.size() * sizeof(double)},
+        {reinterpret_cast<void**>(&s->d_eff), d->eff, (size_t)d->n_energy * sizeof(double)},
+        {reinterpret_cast<void**>(&s->d_ratio), ratio32.data(), ratio32.size() * sizeof(float)},
+        {reinterpret_cast<void**>(&s->d_etab), etab.data(), etab.size() * sizeof(float2)},
+        {reinterpret_cast<void**>(&s->d_tiles), tiles.data(), tiles.size() * sizeof(int4)},
+        {reinterpret_cast<void**>(&s->d_items), items.data(), items.size() * sizeof(int2)},
+        {reinterpret_cast<void**>(&s->d_slice_ptr), d->slice_ptr, ((size_t)d->n_slices + 1) * sizeof(int32_t)},
+        {reinterpret_cast<void**>(&s->d_chunk_sp), chunk_sp.data(), chunk_sp.size() * sizeof(int32_t)},
+        {reinterpret_cast<void**>(&s->d_chunk_slice), chunk_slice.data(), chunk_slice.size() * sizeof(int32_t)},
+        {reinterpret_cast<void**>(&s->d_dets), d->dets, (size_t)d->ndu * sizeof(int32_t)},
+    };
+    auto slot_bytes = [](size_t b) { return (b + 16 + 255) / 256 * 256; };
+    size_t up_bytes = 0;
+    for (const Upload& u : ups) up_bytes += slot_bytes(u.bytes);
+
     int64_t& t = s->device_bytes;
-    if ((rc = pool_alloc(ctx, &s->d_mu, (size_t)s->n_vox, &t)) || (rc = pool_alloc(ctx, &s->d_lam, (size_t)s->n_vox, &t)) ||
-        (rc = pool_alloc(ctx, &s->d_sp, (size_t)std::max(1, d->n_sp) * 3, &t)) ||
-        (rc = pool_alloc(ctx, &s->d_detpos, detpos.size(), &t)) || (rc = pool_alloc(ctx, &s->d_detnorm, detnorm.size(), &t)) ||
-        (rc = pool_alloc(ctx, &s->d_eff, (size_t)d->n_energy, &t)) || (rc = pool_alloc(ctx, &s->d_ratio, (size_t)d->n_energy, &t)) ||
-        (rc = pool_alloc(ctx, &s->d_tiles, tiles.size() + 1, &t)) ||
-        (rc = pool_alloc(ctx, &s->d_items, items.size() + 1, &t)) ||
-        (rc = pool_alloc(ctx, &s->d_slice_ptr, (size_t)d->n_slices + 1, &t)) ||
-        (rc = pool_alloc(ctx, &s->d_chunk_sp, chunk_sp.size(), &t)) ||
-        (rc = pool_alloc(ctx, &s->d_chunk_slice, chunk_slice.size() + 1, &t)) ||
-        (rc = pool_alloc(ctx, &s->d_dets, (size_t)d->ndu, &t)) || (rc = pool_alloc(ctx, &s->d_flag, 1, &t)) ||
+    if ((s->z_fastest && (rc = pool_alloc(ctx, &s->d_mu_own, (size_t)s->n_vox, &t))) ||
+        (rc = pool_alloc(ctx, &s->d_upload, up_bytes, &t)) || (rc = pool_alloc(ctx, &s->d_lam, (size_t)s->n_vox, &t)) ||
+        (rc = pool_alloc(ctx, &s->d_flag, 1, &t)) ||
         (rc = pool_alloc(ctx, &s->hp.off, (size_t)s->n_hp + 1, &t)) || (rc = pool_alloc(ctx, &s->hp.n, (size_t)s->n_hp + 1, &t)) ||
         (rc = pool_alloc(ctx, &s->hp.off_fold, (size_t)s->n_hp + 1, &t)) ||
         (rc = pool_alloc(ctx, &s->hp.n_fold, (size_t)s->n_hp + 1, &t)) ||
@@ -670,43 +721,28 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
         return rc;
     }
     hipStream_t st = ctx->stream;
-    // The uploads are queued behind whatever the context's stream holds, from
-    // one pinned block that the subject keeps until destroy: nothing here
+    // The upload is queued behind whatever the context's stream holds, from
+    // the pinned block that the subject keeps until destroy: nothing here
     // waits for the stream.
-    struct Upload {
-        void* dst;
-        const void* src;
-        size_t bytes;
-    };
-    const Upload ups[] = {
-        {s->d_mu, d->mu, (size_t)s->n_vox * sizeof(double)},
-        {s->d_sp, d->sp_xyz, (size_t)d->n_sp * 3 * sizeof(double)},
-        {s->d_detpos, detpos.data(), detpos.size() * sizeof(double)},
-        {s->d_detnorm, detnorm.data(), detnorm.size() * sizeof(double)},
-        {s->d_eff, d->eff, (size_t)d->n_energy * sizeof(double)},
-        {s->d_ratio, ratio32.data(), ratio32.size() * sizeof(float)},
-        {s->d_tiles, tiles.data(), tiles.size() * sizeof(int4)},
-        {s->d_items, items.data(), items.size() * sizeof(int2)},
-        {s->d_slice_ptr, d->slice_ptr, ((size_t)d->n_slices + 1) * sizeof(int32_t)},
-        {s->d_chunk_sp, chunk_sp.data(), chunk_sp.size() * sizeof(int32_t)},
-        {s->d_chunk_slice, chunk_slice.data(), chunk_slice.size() * sizeof(int32_t)},
-        {s->d_dets, d->dets, (size_t)d->ndu * sizeof(int32_t)},
-    };
-    size_t up_bytes = 0;
-    for (const Upload& u : ups) up_bytes += (u.bytes + 255) / 256 * 256;
     if ((rc = host_block(ctx, up_bytes, &s->h_upload))) {
         delete s;
         return rc;
     }
+    {
+        size_t off = 0;
+        unsigned char* stage = static_cast<unsigned char*>(s->h_upload);
+        for (const Upload& u : ups) {
+            *u.dst = s->d_upload + off;
+            if (u.bytes) std::memcpy(stage + off, u.src, u.bytes);
+            off += slot_bytes(u.bytes);
+        }
+    }
     // cells no lane owns (i1 == i2 on same-ring planes) are never written: zero once
     hipError_t e = hipMemsetAsync(s->d_out, 0, (size_t)s->out_elems * elem, st);
-    size_t off = 0;
-    for (const Upload& u : ups) {
-        if (e != hipSuccess || u.bytes == 0) continue;
-        unsigned char* stage = static_cast<unsigned char*>(s->h_upload) + off;
-        std::memcpy(stage, u.src, u.bytes);
-        e = hipMemcpyAsync(u.dst, stage, u.bytes, hipMemcpyHostToDevice, st);
-        off += (u.bytes + 255) / 256 * 256;
+    if (e == hipSuccess) e = hipMemcpyAsync(s->d_upload, s->h_upload, up_bytes, hipMemcpyHostToDevice, st);
+    if (s->z_fastest) {
+        s->d_mu = s->d_mu_own;
+        if (e == hipSuccess) e = launch_volume_x_fastest(s->d_vol_in, s->d_mu, g.nx, g.ny, g.nz, st);
     }
     if (e == hipSuccess) e = hipEventCreate(&s->ev_scatter[0]);
     if (e == hipSuccess) e = hipEventCreate(&s->ev_scatter[1]);
@@ -770,11 +806,16 @@ static int scan_counts(sss_subject* s, It counts, int64_t* out, int64_t n_items,
 static int refresh_fold(sss_subject* s)
 {
     hipStream_t st = s->ctx->stream;
+    if (s->trivial) {   // one bin: the pair terms need the integrals only
+        SSS_HIP(launch_activity_sum(s->hp, s->smp, s->n_hp, s->d_lam, st));
+        SSS_HIP(launch_pair_half(s->hp, s->n_hp, true, st));
+        return SSS_OK;
+    }
     SSS_HIP(launch_activity_refresh(s->hp, s->smp, s->n_hp, s->d_lam, st));
     const auto padded = rocprim::make_transform_iterator(s->hp.n_fold, PaddedRecords{});
     if (int rc = scan_counts(s, padded, s->hp.off_fold, s->n_hp, st)) return rc;
     SSS_HIP(launch_fold_compact(s->hp, s->smp, s->n_hp, st));
-    SSS_HIP(launch_pair_half(s->hp, s->n_hp, st));
+    SSS_HIP(launch_pair_half(s->hp, s->n_hp, false, st));
     return SSS_OK;
 }
 
@@ -823,15 +864,16 @@ int sss_subject_trace_end(sss_subject* s)
         size_t fold_cap = (size_t)total + (size_t)(kSampleGroup - 1) * s->n_hp + kSamplePad;
         fold_cap = (fold_cap + kSampleGroup - 1) / kSampleGroup * kSampleGroup;
         const bool fp64 = s->precision == SSS_PRECISION_FP64;
+        const bool fold = !s->trivial;   // one-bin subjects sum the samples, no fold table
         int64_t* tally = &s->device_bytes;
         int rc;
         if ((rc = pool_alloc(c, &s->smp.nq, (size_t)total, tally)) ||
             (rc = pool_alloc(c, &s->smp.vox, (size_t)total, tally)) ||
             (rc = pool_alloc(c, &s->smp.len, (size_t)total, tally)) ||
-            (rc = pool_alloc(c, &s->smp.a, (size_t)total, tally)) ||
-            (rc = pool_alloc(c, &s->smp.fold, fold_cap / kSampleGroup * kFoldGroupWords, tally)) ||
-            (fp64 && ((rc = pool_alloc(c, &s->smp.a64, (size_t)total, tally)) ||
-                      (rc = pool_alloc(c, &s->smp.fold_a64, fold_cap, tally)))))
+            (fold && (rc = pool_alloc(c, &s->smp.a, (size_t)total, tally))) ||
+            (fold && (rc = pool_alloc(c, &s->smp.fold, fold_cap / kSampleGroup * kFoldGroupWords, tally))) ||
+            (fold && fp64 && ((rc = pool_alloc(c, &s->smp.a64, (size_t)total, tally)) ||
+                              (rc = pool_alloc(c, &s->smp.fold_a64, fold_cap, tally)))))
             return rc;
         s->samples_cap = total;
     }
@@ -873,8 +915,10 @@ int sss_subject_set_activity(sss_subject* s, const double* lam)
         s->h_lam = static_cast<double*>(p);
     }
     std::memcpy(s->h_lam, lam, (size_t)s->n_vox * sizeof(double));
-    SSS_HIP(hipMemcpyAsync(s->d_lam, s->h_lam, (size_t)s->n_vox * sizeof(double), hipMemcpyHostToDevice,
-                           s->ctx->stream));
+    SSS_HIP(hipMemcpyAsync(s->z_fastest ? s->d_vol_in : s->d_lam, s->h_lam, (size_t)s->n_vox * sizeof(double),
+                           hipMemcpyHostToDevice, s->ctx->stream));
+    if (s->z_fastest)
+        SSS_HIP(launch_volume_x_fastest(s->d_vol_in, s->d_lam, s->grid.nx, s->grid.ny, s->grid.nz, s->ctx->stream));
     SSS_HIP(hipEventRecord(s->ev_lam, s->ctx->stream));
     s->lam_queued = true;
     return refresh_activity(s);
@@ -884,8 +928,12 @@ int sss_subject_set_activity_device(sss_subject* s, const void* lam_dev)
 {
     if (!s || !lam_dev) return fail(SSS_ERR_INVALID, "sss_subject_set_activity_device: NULL argument");
     SSS_HIP(hipSetDevice(s->ctx->device));
-    SSS_HIP(hipMemcpyAsync(s->d_lam, lam_dev, (size_t)s->n_vox * sizeof(double), hipMemcpyDeviceToDevice,
-                           s->ctx->stream));
+    if (s->z_fastest)
+        SSS_HIP(launch_volume_x_fastest(static_cast<const double*>(lam_dev), s->d_lam, s->grid.nx, s->grid.ny,
+                                        s->grid.nz, s->ctx->stream));
+    else
+        SSS_HIP(hipMemcpyAsync(s->d_lam, lam_dev, (size_t)s->n_vox * sizeof(double), hipMemcpyDeviceToDevice,
+                               s->ctx->stream));
     return refresh_activity(s);
 }
 
@@ -896,6 +944,9 @@ static ScatterArgs scatter_args(const sss_subject* s)
     a.n_energy = s->n_energy; a.eff_ref_row = s->eff_ref_row;
     a.n_tiles = s->n_tiles; a.n_chunks = s->n_chunks; a.n_cells_plane = s->n_cells_plane;
     a.hp_rec = s->hp.rec;
+    a.etab = s->d_etab;
+    a.etab_lo = s->etab_lo;
+    a.etab_hi = s->etab_hi;
     a.hp_off = s->hp.off_fold; a.hp_n = s->hp.n_fold; a.hp_n_all = s->hp.n; a.hp_dist = s->hp.dist; a.hp_att = s->hp.att;
     a.hp_lnatt = s->hp.lnatt; a.hp_act = s->hp.act; a.hp_cos = s->hp.cosabs;
     a.s_fold = s->smp.fold; a.s_a64 = s->smp.fold_a64;
@@ -1040,6 +1091,20 @@ int sss_subject_output_compact_async(sss_subject* s, float* host_out)
     SSS_HIP(hipMemcpyAsync(host_out, s->d_out, (size_t)s->out_elems * elem, hipMemcpyDeviceToHost,
                            s->ctx->stream));
     SSS_HIP(hipEventRecord(s->ev_out, s->ctx->stream));
+    return SSS_OK;
+}
+
+int sss_subject_query(sss_subject* s, int32_t* done)
+{
+    if (!s || !done) return fail(SSS_ERR_INVALID, "sss_subject_query: NULL argument");
+    SSS_HIP(hipSetDevice(s->ctx->device));
+    const hipError_t e = hipEventQuery(s->ev_out);
+    if (e == hipErrorNotReady) {
+        *done = 0;
+        return SSS_OK;
+    }
+    SSS_HIP(e);
+    *done = 1;
     return SSS_OK;
 }
 

@@ -146,6 +146,39 @@ __device__ __forceinline__ bool pair_eval(const ScatterArgs& a, int sp, int d1, 
     return true;
 }
 
+// Energy row of a pair (law of cosines at the scatter point, FP64 as the
+// reference rounds it); carg, s1q, s2q are kept for the float terms.
+__device__ __forceinline__ int energy_row(const ScatterArgs& a, double dist1, double dist2, double lor2,
+                                          double& carg, double& s1q, double& s2q)
+{
+    s1q = dist1 * dist1;
+    s2q = dist2 * dist2;
+    carg = (s1q + s2q - lor2) / (2.0 * sqrt(s1q * s2q));
+    carg = carg < -1.0 ? -1.0 : (carg > 1.0 ? 1.0 : carg);
+    const double e_kev = floor(511.0 / (2.0 + carg) + 0.5);
+    const int erow = (int)(e_kev * 2.0) - 1;
+    return erow < 0 ? 0 : (erow >= a.n_energy ? a.n_energy - 1 : erow);
+}
+
+// The float terms of pair_eval from the two records, the energy row's
+// efficiency (times the 511 keV row's) and attenuation ratio.
+__device__ __forceinline__ void pair_coef(const ScatterArgs& a, const PairHalf& r1, const PairHalf& r2,
+                                          double carg, double s1q, double s2q, float eff, float ratio,
+                                          Pair<float>& p)
+{
+    const float log2e = 1.4426950408889634f;
+    const float a1s = fast_exp2(-ratio * r1.lnatt * log2e);
+    const float a2s = fast_exp2(-ratio * r2.lnatt * log2e);
+    const float c = (float)carg;
+    const float rho = 1.0f / (1.0f + (float)a.kn_gamma * (1.0f + c));
+    const float kn = rho * rho * (rho + 1.0f / rho - (1.0f - c * c));
+    const float geo = (float)a.g_const * r1.cosabs * r2.cosabs / ((float)s1q * (float)s2q);
+    const float pre = geo * eff * kn;
+    p.coef1 = pre * r1.att * a2s;
+    p.coef2 = pre * r2.att * a1s;
+    p.counts = ((r1.n_flags & r2.n_flags) & kHalfActive) ? 2.0f : 1.0f;
+}
+
 // pair_eval for the FP32 fold, from the two half paths' PairHalf records
 // (two 16-byte loads each, then the energy tables): the same filters and
 // the same float terms.
@@ -157,33 +190,43 @@ __device__ __forceinline__ bool pair_eval_rec(const ScatterArgs& a, int sp, int 
     const PairHalf r2 = a.hp_rec[sp * per_sp + d2];
     const int32_t f = r1.n_flags | r2.n_flags;
     if ((f & kHalfAttZero) || !(f & kHalfActive)) return false;
-    const double dist1 = r1.dist, dist2 = r2.dist;
-    const double s1q = dist1 * dist1, s2q = dist2 * dist2;
-    double carg = (s1q + s2q - lor2) / (2.0 * sqrt(s1q * s2q));
-    carg = carg < -1.0 ? -1.0 : (carg > 1.0 ? 1.0 : carg);
-    const double e_kev = floor(511.0 / (2.0 + carg) + 0.5);
-    int erow = (int)(e_kev * 2.0) - 1;
-    erow = erow < 0 ? 0 : (erow >= a.n_energy ? a.n_energy - 1 : erow);
+    double carg, s1q, s2q;
+    const int erow = energy_row(a, r1.dist, r2.dist, lor2, carg, s1q, s2q);
     const double eff = a.eff[erow] * eff_ref;
     if (eff == 0.0) return false;
-    const float ratio = a.ratio[erow];
-    const float log2e = 1.4426950408889634f;
-    const float a1s = fast_exp2(-ratio * r1.lnatt * log2e);
-    const float a2s = fast_exp2(-ratio * r2.lnatt * log2e);
-    const float c = (float)carg;
-    const float rho = 1.0f / (1.0f + (float)a.kn_gamma * (1.0f + c));
-    const float kn = rho * rho * (rho + 1.0f / rho - (1.0f - c * c));
-    const float geo = (float)a.g_const * r1.cosabs * r2.cosabs / ((float)s1q * (float)s2q);
-    const float pre = geo * (float)eff * kn;
-    p.coef1 = pre * r1.att * a2s;
-    p.coef2 = pre * r2.att * a1s;
-    p.shift = (int64_t)rint(((dist1 - dist2) * (1.0 / kLightCmPerPs) + 0.5) * kFixOne);
+    pair_coef(a, r1, r2, carg, s1q, s2q, (float)eff, a.ratio[erow], p);
+    p.shift = (int64_t)rint(((r1.dist - r2.dist) * (1.0 / kLightCmPerPs) + 0.5) * kFixOne);
     p.n1 = (r1.n_flags & kHalfActive) ? (r1.n_flags & kHalfCountMask) : 0;
     p.n2 = (r2.n_flags & kHalfActive) ? (r2.n_flags & kHalfCountMask) : 0;
     p.b1 = r1.off;
     p.b2 = r2.off;
     p.act1 = p.act2 = 0.0f;   // the one-bin path reads pair_eval's
-    p.counts = ((r1.n_flags & r2.n_flags) & kHalfActive) ? 2.0f : 1.0f;
+    return true;
+}
+
+// Entry of the energy table in LDS for an energy row: rows kEnergyRowLo..
+// kEnergyRowHi (clamped into the table) in order, then row 0, which only a
+// NaN cosine reaches.
+__device__ __forceinline__ int energy_slot(const ScatterArgs& a, int erow)
+{
+    return (erow >= a.etab_lo && erow <= a.etab_hi) ? erow - a.etab_lo : kEnergyRows - 1;
+}
+
+// pair_eval for one-bin subjects in FP32: records loaded by the caller, the
+// energy terms from the LDS table (x: efficiency times the 511 keV row's
+// efficiency in float, negative where that product is 0; y: the ratio).
+__device__ __forceinline__ bool pair_eval_one_bin(const ScatterArgs& a, const float2* et, const PairHalf& r1,
+                                                  const PairHalf& r2, double lor2, Pair<float>& p)
+{
+    const int32_t f = r1.n_flags | r2.n_flags;
+    if ((f & kHalfAttZero) || !(f & kHalfActive)) return false;
+    double carg, s1q, s2q;
+    const int erow = energy_row(a, r1.dist, r2.dist, lor2, carg, s1q, s2q);
+    const float2 e = et[energy_slot(a, erow)];
+    if (e.x < 0.0f) return false;
+    pair_coef(a, r1, r2, carg, s1q, s2q, e.x, e.y, p);
+    p.act1 = (float)r1.act;
+    p.act2 = (float)r2.act;
     return true;
 }
 
@@ -551,8 +594,15 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
     static_assert(!SLIDE || (sizeof(R) == 4 && HM == 1 && !RANGE), "sliding window: float, h = 1, no range test");
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
-    // LDS: the sliding window's tables (shared), then one column per lane
+    // LDS: the sliding window's tables (shared), then one column per lane;
+    // the one-bin FP32 path: the energy table
+    constexpr bool ONE_BIN_REC = TRIVIAL && sizeof(R) == 4;
     SlideTables* tb = reinterpret_cast<SlideTables*>(smem_raw);
+    float2* et = reinterpret_cast<float2*>(smem_raw);
+    if constexpr (ONE_BIN_REC) {
+        for (int i = threadIdx.x; i < kEnergyRows; i += blockDim.x) et[i] = a.etab[i];
+        __syncthreads();   // every wave of the workgroup, before any exits
+    }
     if constexpr (SLIDE) {
         const int i = threadIdx.x;
         if (i < kMaxBins) {
@@ -599,24 +649,64 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
     // cells mostly keep or drop the same points, so fewer lane slots idle
     // than with a cursor per lane (a CPU model of config 2: 89% busy against
     // 86%).
+    const int p_begin = a.chunk_sp[chunk];
     const int p_end = a.chunk_sp[chunk + 1];
-    for (int sp = a.chunk_sp[chunk]; sp < p_end; ++sp) {
-        Pair<R> p;
-        bool found = false;
-        if (valid) {
-            if constexpr (sizeof(R) == 4 && !TRIVIAL) found = pair_eval_rec(a, sp, d1, d2, lor2, eff_ref, p);
-            else found = pair_eval<R>(a, sp, d1, d2, lor2, eff_ref, p);
-        }
-        if (found) {
-            cnt += p.counts;
-            if constexpr (TRIVIAL) {
+    if constexpr (ONE_BIN_REC) {
+        // The pair terms are short, so the records of the next two points
+        // are in flight while one pair is evaluated: three record pairs
+        // take turns (unrolled by three, so no register copy waits for a
+        // load), and the energy terms come from LDS (no global load queues
+        // behind the records).  Clamped detector indices keep the loads of
+        // invalid lanes in bounds, and indices past the chunk read its last
+        // point again.
+        const int64_t per_sp = (int64_t)a.nru * a.ndu;
+        const PairHalf* rec = a.hp_rec;
+        const int last = p_end - 1;
+        auto load = [&](int sp, PairHalf& r1, PairHalf& r2) {
+            const int64_t o = (int64_t)min(sp, last) * per_sp;
+            r1 = rec[o + d1];
+            r2 = rec[o + d2];
+        };
+        auto eval = [&](int sp, const PairHalf& r1, const PairHalf& r2) {
+            Pair<float> p;
+            if (valid && sp < p_end && pair_eval_one_bin(a, et, r1, r2, lor2, p)) {
+                cnt += p.counts;
                 num += p.coef1 * p.act1 + p.coef2 * p.act2;
-            } else if constexpr (SLIDE) {
-                if (p.n1 > 0) slide_path<-1>(a, tc, tb, col, sl, p.b1, p.n1, p.shift, p.coef1);
-                if (p.n2 > 0) slide_path<+1>(a, tc, tb, col, sl, p.b2, p.n2, p.shift, p.coef2);
-            } else if (a.t_hi >= a.t_lo) {   // else every TOF weight is zero
-                fold_path<R, HM, TM, RANGE, -1>(a, tc, col, win, p.b1, p.n1, p.shift, p.coef1);
-                fold_path<R, HM, TM, RANGE, +1>(a, tc, col, win, p.b2, p.n2, p.shift, p.coef2);
+            }
+        };
+        if (p_begin < p_end) {
+            PairHalf u1, u2, v1, v2, w1, w2;
+            load(p_begin, u1, u2);
+            load(p_begin + 1, v1, v2);
+            for (int sp = p_begin; sp < p_end; sp += 3) {
+                load(sp + 2, w1, w2);
+                eval(sp, u1, u2);
+                load(sp + 3, u1, u2);
+                eval(sp + 1, v1, v2);
+                load(sp + 4, v1, v2);
+                eval(sp + 2, w1, w2);
+            }
+        }
+    }
+    if constexpr (!ONE_BIN_REC) {
+        for (int sp = p_begin; sp < p_end; ++sp) {
+            Pair<R> p;
+            bool found = false;
+            if (valid) {
+                if constexpr (sizeof(R) == 4 && !TRIVIAL) found = pair_eval_rec(a, sp, d1, d2, lor2, eff_ref, p);
+                else found = pair_eval<R>(a, sp, d1, d2, lor2, eff_ref, p);
+            }
+            if (found) {
+                cnt += p.counts;
+                if constexpr (TRIVIAL) {
+                    num += p.coef1 * p.act1 + p.coef2 * p.act2;
+                } else if constexpr (SLIDE) {
+                    if (p.n1 > 0) slide_path<-1>(a, tc, tb, col, sl, p.b1, p.n1, p.shift, p.coef1);
+                    if (p.n2 > 0) slide_path<+1>(a, tc, tb, col, sl, p.b2, p.n2, p.shift, p.coef2);
+                } else if (a.t_hi >= a.t_lo) {   // else every TOF weight is zero
+                    fold_path<R, HM, TM, RANGE, -1>(a, tc, col, win, p.b1, p.n1, p.shift, p.coef1);
+                    fold_path<R, HM, TM, RANGE, +1>(a, tc, col, win, p.b2, p.n2, p.shift, p.coef2);
+                }
             }
         }
     }
@@ -781,7 +871,8 @@ hipError_t launch_variant(const ScatterArgs& a, hipStream_t st)
 {
     // one workgroup per kScatterWaves work items (the item list is padded to that)
     dim3 grid((unsigned)(a.n_items / kScatterWaves));
-    size_t lds = TRIVIAL ? 0 : (size_t)kScatterWaves * (a.n_bins + 2 * HM) * kWave * sizeof(R);
+    size_t lds = TRIVIAL ? (sizeof(R) == 4 ? kEnergyRows * sizeof(float2) : 0)
+                         : (size_t)kScatterWaves * (a.n_bins + 2 * HM) * kWave * sizeof(R);
     if (TM == kTofSlide) lds += sizeof(SlideTables);
     hipLaunchKernelGGL((k_scatter<R, HM, TM, TRIVIAL, RANGE>), grid,
                        dim3(kScatterWaves * kWave), lds, st, a);

@@ -124,6 +124,33 @@ __global__ void k_activity_refresh(HalfPathTable t, SampleTable s, int64_t n_hp,
     t.n_fold[h] = nz;
 }
 
+// k_activity_refresh for one-bin subjects, which have no fold table: only
+// the integral (the same terms added in the same order).  Four samples'
+// loads are issued before their terms are added, so a thread keeps four
+// gathers in flight.
+__global__ void k_activity_sum(HalfPathTable t, SampleTable s, int64_t n_hp, const double* __restrict__ lam)
+{
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= n_hp) return;
+    const int64_t base = t.off[h];
+    const int n = t.n[h];
+    double act = 0;
+    int k = 0;
+    for (; k + 4 <= n; k += 4) {
+        const int64_t i = base + (int64_t)k * kWave;
+        double ar[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) ar[u] = lam[s.vox[i + u * kWave]] * s.len[i + u * kWave];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) act += ar[u];
+    }
+    for (; k < n; ++k) {
+        const int64_t i = base + (int64_t)k * kWave;
+        act += lam[s.vox[i]] * s.len[i];
+    }
+    t.act[h] = act;
+}
+
 constexpr int kCompactLanes = 16;   // lanes per half path in k_fold_compact
 
 // Record r of the fold table (packed groups of kSampleGroup records).
@@ -257,25 +284,63 @@ hipError_t launch_activity_refresh(const HalfPathTable t, SampleTable s, int64_t
 
 // The FP32 pair evaluation's record of each half path (after the fold
 // table's offsets are scanned).
-__global__ void k_pair_half(HalfPathTable t, int64_t n_hp)
+__global__ void k_pair_half(HalfPathTable t, int64_t n_hp, bool one_bin)
 {
     const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= n_hp) return;
     PairHalf r;
     r.dist = t.dist[h];
-    r.off = t.off_fold[h];
+    const double act = t.act[h];
+    if (one_bin) r.act = act;
+    else r.off = t.off_fold[h];
     const double att = t.att[h];
-    r.n_flags = t.n_fold[h] | (att == 0.0 ? kHalfAttZero : 0) | (t.act[h] != 0.0 ? kHalfActive : 0);
+    r.n_flags = (one_bin ? 0 : t.n_fold[h]) | (att == 0.0 ? kHalfAttZero : 0) | (act != 0.0 ? kHalfActive : 0);
     r.cosabs = t.cosabs[h];
     r.lnatt = (float)t.lnatt[h];
     r.att = (float)att;
     t.rec[h] = r;
 }
 
-hipError_t launch_pair_half(const HalfPathTable t, int64_t n_hp, hipStream_t st)
+hipError_t launch_pair_half(const HalfPathTable t, int64_t n_hp, bool one_bin, hipStream_t st)
 {
     if (n_hp == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_pair_half, dim3(blocks_for(n_hp, 256)), dim3(256), 0, st, t, n_hp);
+    hipLaunchKernelGGL(k_pair_half, dim3(blocks_for(n_hp, 256)), dim3(256), 0, st, t, n_hp, one_bin);
+    return hipGetLastError();
+}
+
+// Volume in C order of an (X, Y, Z) array (z fastest) to the x-fastest
+// layout the kernels read: a 32 x 32 (x, z) tile per workgroup at one y,
+// through LDS, so both the reads and the writes are coalesced.
+__global__ __launch_bounds__(256) void k_volume_x_fastest(const double* __restrict__ src, double* __restrict__ dst,
+                                                          int X, int Y, int Z)
+{
+    __shared__ double tile[32][33];
+    const int z0 = blockIdx.x * 32, x0 = blockIdx.y * 32, y = blockIdx.z;
+    const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
+    for (int j = ty; j < 32; j += 8) {
+        const int x = x0 + j, z = z0 + tx;
+        if (x < X && z < Z) tile[j][tx] = src[((int64_t)x * Y + y) * Z + z];
+    }
+    __syncthreads();
+    for (int j = ty; j < 32; j += 8) {
+        const int z = z0 + j, x = x0 + tx;
+        if (x < X && z < Z) dst[x + (int64_t)X * (y + (int64_t)Y * z)] = tile[tx][j];
+    }
+}
+
+hipError_t launch_volume_x_fastest(const double* src, double* dst, int X, int Y, int Z, hipStream_t st)
+{
+    if ((int64_t)X * Y * Z == 0) return hipSuccess;
+    const dim3 grid((unsigned)((Z + 31) / 32), (unsigned)((X + 31) / 32), (unsigned)Y);
+    hipLaunchKernelGGL(k_volume_x_fastest, grid, dim3(256), 0, st, src, dst, X, Y, Z);
+    return hipGetLastError();
+}
+
+hipError_t launch_activity_sum(const HalfPathTable t, SampleTable s, int64_t n_hp, const double* lam,
+                               hipStream_t st)
+{
+    if (n_hp == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_activity_sum, dim3(blocks_for(n_hp, 256)), dim3(256), 0, st, t, s, n_hp, lam);
     return hipGetLastError();
 }
 

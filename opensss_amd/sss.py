@@ -24,6 +24,40 @@ def _f64(a):
     return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
 
 
+# Subjects of one scanner and energy window share their set-up tables; these
+# are pure functions of their arguments, so they are computed once per key.
+_ENERGY_CACHE: dict = {}
+_TOF_CACHE: dict = {}
+_CACHE_MAX = 32
+
+
+def _energy_tables(table, resolution, threshold):
+    t = _f64(table)
+    key = (t.shape, t.tobytes(), float(resolution), float(threshold))
+    hit = _ENERGY_CACHE.get(key)
+    if hit is None:
+        ratio, eff, ref = energy_tables(t, resolution, threshold)
+        ratio, eff = _f64(ratio), _f64(eff)
+        ratio.setflags(write=False)
+        eff.setflags(write=False)
+        hit = (ratio, eff, ref)
+        if len(_ENERGY_CACHE) >= _CACHE_MAX:
+            _ENERGY_CACHE.clear()
+        _ENERGY_CACHE[key] = hit
+    return hit
+
+
+def _tof_kernel(n_bins, tof_range, resolution, window_tol):
+    key = (int(n_bins), float(tof_range), float(resolution), float(window_tol))
+    hit = _TOF_CACHE.get(key)
+    if hit is None:
+        hit = tof_kernel(*key)
+        if len(_TOF_CACHE) >= _CACHE_MAX:
+            _TOF_CACHE.clear()
+        _TOF_CACHE[key] = hit
+    return hit
+
+
 def _i32(a):
     return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
 
@@ -34,7 +68,8 @@ class ScatterSubject:
     def __init__(self, AttenuationMap, ImageSize, Geometry, NormalVectors, DetectorSize,
                  AttenuationTable, EnergyResolution, EnergyThreshold, NrRingsUsed,
                  NrDetectorsUsed, TOFResolution, TOFRange, NrBins, *, scatter_points=None,
-                 SampleStep=(1, 1, 1), precision='fp32', context=None, n_chunks=0):
+                 SampleStep=(1, 1, 1), precision='fp32', context=None, n_chunks=0,
+                 upload_order='C'):
         mu = np.asarray(AttenuationMap)
         if mu.ndim != 3:
             raise ValueError('AttenuationMap must be 3-D (X, Y, Z)')
@@ -55,10 +90,10 @@ class ScatterSubject:
         self.ctx = context or _native.default_context()
         self._lib = _native.load_library()
 
-        ratio, eff, _ = energy_tables(AttenuationTable, EnergyResolution, EnergyThreshold)
+        ratio, eff, _ = _energy_tables(AttenuationTable, EnergyResolution, EnergyThreshold)
         # the FP64 validation path keeps every bin above 1e-16 of the nearest one
-        self.tof = tof_kernel(self.n_bins, float(TOFRange), float(TOFResolution),
-                              window_tol=1e-16 if precision == 'fp64' else 1e-9)
+        self.tof = _tof_kernel(self.n_bins, float(TOFRange), float(TOFResolution),
+                               1e-16 if precision == 'fp64' else 1e-9)
         self.rings = _i32(rings_used(self.n_rings, self.nru))
         self.dets = _i32(detectors_used(self.n_det, self.ndu))
         if scatter_points is None:
@@ -66,8 +101,14 @@ class ScatterSubject:
         pts, ptr, zs = group_by_slice(scatter_points)
         self.scatter_points, self.slice_ptr, self.slice_z = pts, _i32(ptr), zs
         self.sp_xyz = _f64(scatter_point_coordinates(pts, self.grid, self.image_size))
-        self._keep = dict(mu=_f64(mu.flatten(order='F')), geom=geom, normals=_f64(NormalVectors),
-                          ratio=_f64(ratio), eff=_f64(eff))
+        # upload_order 'C': volumes go over in C order (no host transpose) and
+        # the subject lays them out x fastest on the device; 'F': flattened
+        # x fastest on the host (the reference's flatten(order='F'))
+        if upload_order not in ('C', 'F'):
+            raise ValueError("upload_order must be 'C' or 'F'")
+        self._upload_order = upload_order
+        self._keep = dict(mu=self._volume(mu), geom=geom, normals=_f64(NormalVectors),
+                          ratio=ratio, eff=eff)
 
         d = _native.SubjectDesc()
         d.grid[:] = list(self.grid)
@@ -93,11 +134,18 @@ class ScatterSubject:
         d.tof_trivial = int(self.tof.trivial)
         d.precision = PRECISIONS[precision]
         d.n_chunks = int(n_chunks)
+        d.volume_order = (_native.SSS_VOLUME_Z_FASTEST if upload_order == 'C'
+                          else _native.SSS_VOLUME_X_FASTEST)
         h = _native.ctypes.c_void_p()
         _native.check(self._lib.sss_subject_create(self.ctx.handle, _native.ctypes.byref(d),
                                                    _native.ctypes.byref(h)), 'sss_subject_create')
         self.handle = h
         self._traced = False
+
+    def _volume(self, v):
+        if self._upload_order == 'C':
+            return _f64(v)
+        return _f64(np.asarray(v).flatten(order='F'))
 
     # -- lifecycle -------------------------------------------------------
     def close(self):
@@ -129,11 +177,14 @@ class ScatterSubject:
         lam = np.asarray(ActivityMap)
         if tuple(lam.shape) != self.grid:
             raise ValueError(f'ActivityMap shape {lam.shape} != {self.grid}')
-        flat = _f64(lam.flatten(order='F'))
+        flat = self._volume(lam)
         _native.check(self._lib.sss_subject_set_activity(self.handle, _native.ptr(flat)),
                       'sss_subject_set_activity')
 
     def set_activity_device(self, device_ptr: int):
+        """Activity from device memory: X*Y*Z float64 in C order of (X, Y, Z)
+        (a contiguous torch tensor of the grid's shape), or x fastest with
+        upload_order 'F'."""
         _native.check(self._lib.sss_subject_set_activity_device(self.handle,
                                                                 _native.ctypes.c_void_p(device_ptr)),
                       'sss_subject_set_activity_device')
@@ -197,6 +248,12 @@ class ScatterSubject:
 
     def wait(self):
         _native.check(self._lib.sss_subject_wait(self.handle), 'sss_subject_wait')
+
+    def done(self) -> bool:
+        """True once the last queued download has finished."""
+        v = _native.ctypes.c_int32()
+        _native.check(self._lib.sss_subject_query(self.handle, _native.ctypes.byref(v)), 'sss_subject_query')
+        return bool(v.value)
 
     @property
     def output_shape(self):

@@ -208,6 +208,34 @@ def test_resident_subject_activity_refresh(oracle):
     assert rel <= FP32_TOL and absn <= 1e-6
 
 
+@pytest.mark.parametrize('n_bins', [1, 7])
+def test_volume_upload_orders_agree(n_bins):
+    """Volumes sent in C order (transposed on the device, through a 32 x 32
+    tile per workgroup) give the results of volumes flattened x fastest on
+    the host, bit for bit; grid sizes not multiples of 32, two activity
+    images in a row (the second reuses the device staging slot)."""
+    from opensss_amd.sss import ScatterSubject
+    w = phantoms.small_tof(seed=4, n_bins=n_bins, shape=(40, 40, 21))
+    lam2 = w.activity * np.random.default_rng(2).uniform(0.5, 1.5, size=w.activity.shape)
+    outs = {}
+    for order in ('C', 'F'):
+        s = ScatterSubject(w.attenuation, w.image_size, w.geometry, w.normals, w.det_size, WATER,
+                           w.energy_resolution, w.energy_threshold, w.n_rings_used, w.n_det_used,
+                           w.tof_resolution, w.tof_range, w.n_bins, scatter_points=w.scatter_points,
+                           upload_order=order)
+        try:
+            s.run(w.activity)
+            first = s.output_compact()
+            s.set_activity(lam2)
+            s.scatter()
+            outs[order] = (first, s.output_compact())
+        finally:
+            s.close()
+    assert np.array_equal(outs['C'][0], outs['F'][0])
+    assert np.array_equal(outs['C'][1], outs['F'][1])
+    assert not np.array_equal(outs['C'][0], outs['C'][1])
+
+
 def test_drop_in_strided_matches_oracle(oracle, tmp_path):
     from opensss_amd.sss import SingleScatterSimulationTOF
     from opensss_amd.tables import strided_scatter_points
