@@ -31,7 +31,8 @@ def parse():
     p.add_argument('--gpus', type=int, default=1)
     p.add_argument('--steps', type=int, default=10)
     p.add_argument('--warmup', type=int, default=2)
-    p.add_argument('--workload', default='config2', choices=['config1', 'config2', 'small'])
+    p.add_argument('--workload', default='config2',
+                   help="config1, config2, small, or randomN (the uneven random instance of seed N)")
     p.add_argument('--no-cpu-baseline', action='store_true')
     p.add_argument('--no-fp64-check', action='store_true')
     p.add_argument('--cpu-threads', type=int, default=0)
@@ -213,6 +214,10 @@ def make_workload(name):
         return phantoms.config1()
     if name == 'small':
         return phantoms.small_tof()
+    if name.startswith('random'):
+        return phantoms.random_instance(int(name[len('random'):]))
+    if name != 'config2':
+        raise SystemExit(f'unknown workload {name!r}')
     return phantoms.config2()
 
 

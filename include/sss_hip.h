@@ -101,7 +101,9 @@ typedef struct {
     int32_t tof_fold;            /* TOF weights of the scatter launch: 0 one exp2 per window bin,
                                     1 one exp2 (h = 1, end bins masked), 2 one exp2 with the
                                     window centre inside the bins, 3 as 2 with a window that
-                                    slides along each half path (FP32) */
+                                    slides along each half path (FP32), 4 a sliding window
+                                    on the nearest bin with relative side weights, two
+                                    exp2 per sample (FP32, h = 1 or 2) */
 } sss_subject_info;
 
 const char* sss_last_error(void);

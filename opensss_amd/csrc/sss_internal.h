@@ -39,6 +39,10 @@ constexpr int kTofOneExpInner = 2;        // h = 1: one exp2, window centre kept
 // float only: kTofOneExpInner with a window that slides along a half path
 // (the centre bin moves one bin at a time, in one direction)
 constexpr int kTofSlide = 3;
+// float only, h = 1 or 2: a sliding window centred on the nearest bin with
+// the side weights in relative form, 2^(+-ku - beta2) (two exp2 per sample):
+// no float range limit on beta2 (h = 1) or on the offsets' reach
+constexpr int kTofSlideRel = 4;
 constexpr int kMaxHalfWindow = 8;
 
 void set_error(const std::string& msg);
@@ -172,6 +176,10 @@ struct ScatterArgs {
     int slide_row0;
     int slide_thr_d1[kMaxBins], slide_thr_d2[kMaxBins];
     float slide_kc[kMaxBins];
+    // kTofSlideRel: centre b in [0, B-1] (thresholds INT_MIN at the ends);
+    // slide_kc[b] = kappa2 (mid - b) W - beta2, slide_km[b] = -kappa2 (mid - b) W - beta2
+    float slide_km[kMaxBins];
+    float f_beta2, f_c2;         // beta2, 2^(-2 beta2)
     double g_const;              // (Dx Dy 1e-2)^2 / (4 pi)
     double kn_gamma;
     // output: per chunk, the un-normalised sums and the pair counts of every

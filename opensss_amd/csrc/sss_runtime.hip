@@ -974,6 +974,12 @@ static ScatterArgs scatter_args(const sss_subject* s)
                 a.tof_mode = kTofOneExpInner;
         }
     }
+    // float subjects whose fold is not the inner sliding window: the relative
+    // sliding window (h = 1 at any beta2; h = 2 while 2^(-2 beta2) is a
+    // normal float)
+    const bool rel = s->precision != SSS_PRECISION_FP64 && !s->range_check && a.tof_mode != kTofOneExpInner &&
+                     (s->half_window == 1 || (s->half_window == 2 && a.beta2 < 60.0));
+    if (rel) a.tof_mode = kTofSlideRel;
     a.range_check = s->range_check ? 1 : 0;
     a.f_inv_w = (float)a.inv_w_bins;
     a.f_kap = (float)a.kappa2;
@@ -984,11 +990,17 @@ static ScatterArgs scatter_args(const sss_subject* s)
     a.f_c = (float)a.c_beta;
     a.f_inv_c = (float)std::exp2(a.beta2);
     a.slide_row0 = s->half_window;
+    a.f_beta2 = (float)a.beta2;
+    a.f_c2 = (float)std::exp2(-2.0 * a.beta2);
+    // centre range: [1, B-2] for the inner window, [0, B-1] for the relative one
+    const int c_lo = rel ? 0 : 1, c_hi = rel ? s->n_bins - 1 : s->n_bins - 2;
     for (int b = 0; b < kMaxBins; ++b) {
         auto L = [&](int j) { return (int)std::ceil((j - a.mid - 0.5) * s->bin_w); };
-        a.slide_thr_d1[b] = (b > 1 && b < s->n_bins) ? L(b) : INT_MIN;
-        a.slide_thr_d2[b] = (b + 1 < s->n_bins - 1) ? 1 - L(b + 1) : INT_MIN;
-        a.slide_kc[b] = (float)((a.mid - b) * s->bin_w * a.kappa2);
+        a.slide_thr_d1[b] = (b > c_lo && b <= c_hi) ? L(b) : INT_MIN;
+        a.slide_thr_d2[b] = (b >= c_lo && b < c_hi) ? 1 - L(b + 1) : INT_MIN;
+        const double kc = (a.mid - b) * s->bin_w * a.kappa2;
+        a.slide_kc[b] = (float)(rel ? kc - a.beta2 : kc);
+        a.slide_km[b] = (float)(-kc - a.beta2);
     }
     a.g_const = s->g_const;
     a.kn_gamma = (511.0e3 * 1.6e-19) / (9.10938356e-31 * (299792458.0 * 299792458.0));

@@ -447,7 +447,8 @@ struct SlideWindow {
 struct SlideTables {
     int thr_d1[kMaxBins];   // path to D1 (t = hi falls): L(b) for b > 1, else INT_MIN
     int thr_d2[kMaxBins];   // path to D2 (t = -hi rises): 1 - L(b+1) for b < B-2, else INT_MIN
-    float kc[kMaxBins];     // kappa2 (mid - b) W
+    float kc[kMaxBins];     // kappa2 (mid - b) W  (kTofSlideRel: minus beta2)
+    float km[kMaxBins];     // kTofSlideRel: -kappa2 (mid - b) W - beta2
 };
 
 // Adds the window, times the path coefficient (ccoef = c coef), into its
@@ -583,6 +584,167 @@ __device__ __forceinline__ void slide_path(const ScatterArgs& a, const TofConst<
     slide_flush(win, coef, ccoef);
 }
 
+// Sliding window in relative form (kTofSlideRel, float, h = H in {1, 2}).
+// The centre is the nearest bin, b in [0, B-1] (thresholds as above, with
+// INT_MIN only at the two ends).  Relative to the centre the weights of bins
+// cur + d are 2^(d ku - d^2 beta2), ku = kappa2 (t - s_cur): for d = +-1
+// 2^(+-ku - beta2), one exp2 each, and for d = +-2 the square of those times
+// 2^(-2 beta2).  Inside the nearest bin |ku| <= beta2, so no term exceeds 1
+// whatever beta2 is (terms that underflow weigh under 2^-126 of the centre);
+// bins past the ends are masked by per-centre factors (their exponents are
+// capped first, so a masked term is 0, never inf * 0).  The window holds
+// the weights without the path coefficient, which is applied when a bin
+// leaves the window.
+template <int H>
+struct RelWindow {
+    float A[2 * H + 1];   // bins cur - H .. cur + H
+    int cur;
+    int thr;              // the centre changes once hi < thr
+    float kp, km;         // ku - beta2 = skap hi + kp, -ku - beta2 = -skap hi + km
+    float mp1, mm1;       // 1 where bin cur +- 1 exists, else 0
+    float c2p, c2m;       // 2^(-2 beta2) where bin cur +- 2 exists, else 0 (H = 2)
+    float* wp;            // LDS row of bin cur - H
+};
+
+template <int H>
+__device__ __forceinline__ void rel_centre(const ScatterArgs& a, const SlideTables* tb, RelWindow<H>& w, int b)
+{
+    w.cur = b;
+    w.kp = tb->kc[b];
+    w.km = tb->km[b];
+    w.mp1 = b + 1 < a.n_bins ? 1.0f : 0.0f;
+    w.mm1 = b > 0 ? 1.0f : 0.0f;
+    if constexpr (H == 2) {
+        w.c2p = b + 2 < a.n_bins ? a.f_c2 : 0.0f;
+        w.c2m = b > 1 ? a.f_c2 : 0.0f;
+    }
+}
+
+template <int H>
+__device__ __forceinline__ void rel_flush(RelWindow<H>& w, float coef)
+{
+#pragma unroll
+    for (int i = 0; i < 2 * H + 1; ++i) {
+        lds_add(w.wp + i * kWave, w.A[i] * coef);
+        w.A[i] = 0.0f;
+    }
+}
+
+template <int H, int SIGN>
+__device__ __forceinline__ void rel_sample(const ScatterArgs& a, const SlideTables* tb, float* col,
+                                           RelWindow<H>& w, int64_t x, int64_t nq, float av, float coef,
+                                           float skap)
+{
+    const int hi = (int)((x + nq) >> 32);
+    if (hi < w.thr) {
+        int b = w.cur + SIGN;
+        int thr = slide_thr<SIGN>(tb, b);
+        if (hi >= thr) {
+            // a move by one bin: the bin leaving the window goes into LDS
+            if constexpr (SIGN < 0) {
+                lds_add(w.wp + 2 * H * kWave, w.A[2 * H] * coef);
+#pragma unroll
+                for (int i = 2 * H; i > 0; --i) w.A[i] = w.A[i - 1];
+                w.A[0] = 0.0f;
+                w.wp -= kWave;
+            } else {
+                lds_add(w.wp, w.A[0] * coef);
+#pragma unroll
+                for (int i = 0; i < 2 * H; ++i) w.A[i] = w.A[i + 1];
+                w.A[2 * H] = 0.0f;
+                w.wp += kWave;
+            }
+        } else {
+            do {
+                b += SIGN;
+                thr = slide_thr<SIGN>(tb, b);
+            } while (hi < thr);
+            rel_flush(w, coef);
+            w.wp = col + (b - H + a.slide_row0) * kWave;
+        }
+        w.thr = thr;
+        rel_centre(a, tb, w, b);
+    }
+    const float fh = (float)hi;
+    const float tp = fast_exp2(fminf(__builtin_fmaf(fh, skap, w.kp), 30.0f)) * w.mp1;
+    const float tm = fast_exp2(fminf(__builtin_fmaf(fh, -skap, w.km), 30.0f)) * w.mm1;
+    float D = 1.0f + (tp + tm);
+    float tp2 = 0.0f, tm2 = 0.0f;
+    if constexpr (H == 2) {
+        tp2 = tp * tp * w.c2p;
+        tm2 = tm * tm * w.c2m;
+        D += tp2 + tm2;
+    }
+    const float g = av * fast_rcp(D);
+    w.A[H] += g;
+    w.A[H + 1] = __builtin_fmaf(g, tp, w.A[H + 1]);
+    w.A[H - 1] = __builtin_fmaf(g, tm, w.A[H - 1]);
+    if constexpr (H == 2) {
+        w.A[4] = __builtin_fmaf(g, tp2, w.A[4]);
+        w.A[0] = __builtin_fmaf(g, tm2, w.A[0]);
+    }
+}
+
+// Nearest bin of the path's first sample: the float estimate, then the
+// thresholds (INT_MIN at the ends stops both loops).
+template <int SIGN>
+__device__ __forceinline__ int rel_first(const ScatterArgs& a, const SlideTables* tb, int hi)
+{
+    const int t = SIGN < 0 ? hi : -hi;
+    int b = (int)rintf((float)t * a.f_inv_w + a.f_mid);
+    b = min(max(b, 0), a.n_bins - 1);
+    while (hi < slide_thr<SIGN>(tb, b)) b += SIGN;
+    while (b - SIGN >= 0 && b - SIGN <= a.n_bins - 1 && hi >= slide_thr<SIGN>(tb, b - SIGN)) b -= SIGN;
+    return b;
+}
+
+template <int H, int SIGN>
+__device__ __forceinline__ void rel_group(const ScatterArgs& a, const SlideTables* tb, float* col,
+                                          RelWindow<H>& w, int64_t x, const uint4& q01, const uint4& q23,
+                                          const uint4& av, float coef, float skap)
+{
+    rel_sample<H, SIGN>(a, tb, col, w, x, nq_lo(q01), __uint_as_float(av.x), coef, skap);
+    rel_sample<H, SIGN>(a, tb, col, w, x, nq_hi(q01), __uint_as_float(av.y), coef, skap);
+    rel_sample<H, SIGN>(a, tb, col, w, x, nq_lo(q23), __uint_as_float(av.z), coef, skap);
+    rel_sample<H, SIGN>(a, tb, col, w, x, nq_hi(q23), __uint_as_float(av.w), coef, skap);
+}
+
+// fold_path for kTofSlideRel: groups of 4 records, two groups in registers
+// (as slide_path).
+template <int H, int SIGN>
+__device__ __forceinline__ void rel_path(const ScatterArgs& a, const SlideTables* tb, float* __restrict__ col,
+                                         RelWindow<H>& w, int64_t base, int n, int64_t shift, float coef)
+{
+    const uint4* __restrict__ g = fold_groups(a, base);
+    int64_t x = SIGN < 0 ? shift : ((int64_t)0xffffffffLL - shift);
+    asm volatile("" : "+v"(x));   // keep x + nq as one 64-bit add per sample
+    const float skap = SIGN < 0 ? a.f_kap : -a.f_kap;
+    uint4 c0 = g[0], c1 = g[1], c2 = g[2];
+    uint4 n0 = g[3], n1 = g[4], n2 = g[5];
+    {
+        const int hi = (int)((x + nq_lo(c0)) >> 32);
+        const int b = rel_first<SIGN>(a, tb, hi);
+        w.thr = slide_thr<SIGN>(tb, b);
+        w.wp = col + (b - H + a.slide_row0) * kWave;
+        rel_centre(a, tb, w, b);
+#pragma unroll
+        for (int i = 0; i < 2 * H + 1; ++i) w.A[i] = 0.0f;
+    }
+    for (int k = n;; ) {
+        rel_group<H, SIGN>(a, tb, col, w, x, c0, c1, c2, coef, skap);
+        c0 = g[6]; c1 = g[7]; c2 = g[8];
+        k -= kSampleGroup;
+        g += 3;
+        if (k <= 0) break;
+        rel_group<H, SIGN>(a, tb, col, w, x, n0, n1, n2, coef, skap);
+        n0 = g[6]; n1 = g[7]; n2 = g[8];
+        k -= kSampleGroup;
+        g += 3;
+        if (k <= 0) break;
+    }
+    rel_flush(w, coef);
+}
+
 // At least 6 waves per SIMD (<= 80 VGPRs): the values that do not fit are
 // spilled outside the sample loops (per slice and per pair only).  Measured
 // on config 2: 4 waves 240 ms, 5 waves 203 ms, 6 waves 189 ms.
@@ -591,7 +753,10 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
 {
     extern __shared__ unsigned char smem_raw[];
     constexpr bool SLIDE = TM == kTofSlide;
+    constexpr bool REL = TM == kTofSlideRel;
     static_assert(!SLIDE || (sizeof(R) == 4 && HM == 1 && !RANGE), "sliding window: float, h = 1, no range test");
+    static_assert(!REL || (sizeof(R) == 4 && (HM == 1 || HM == 2) && !RANGE),
+                  "relative sliding window: float, h = 1 or 2, no range test");
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
     // LDS: the sliding window's tables (shared), then one column per lane;
@@ -603,12 +768,13 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
         for (int i = threadIdx.x; i < kEnergyRows; i += blockDim.x) et[i] = a.etab[i];
         __syncthreads();   // every wave of the workgroup, before any exits
     }
-    if constexpr (SLIDE) {
+    if constexpr (SLIDE || REL) {
         const int i = threadIdx.x;
         if (i < kMaxBins) {
             tb->thr_d1[i] = a.slide_thr_d1[i];
             tb->thr_d2[i] = a.slide_thr_d2[i];
             tb->kc[i] = a.slide_kc[i];
+            tb->km[i] = a.slide_km[i];
         }
         __syncthreads();   // every wave of the workgroup, before any exits
     }
@@ -619,7 +785,7 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
     const int chunk = item.y;
     const int B = a.n_bins;
     const int rows = B + 2 * HM;   // bins -HM .. B-1+HM (the window's reach)
-    R* col = reinterpret_cast<R*>(smem_raw + (SLIDE ? sizeof(SlideTables) : 0)) +
+    R* col = reinterpret_cast<R*>(smem_raw + ((SLIDE || REL) ? sizeof(SlideTables) : 0)) +
              (size_t)wave * rows * kWave + lane;
     if constexpr (!TRIVIAL) {
         for (int r = 0; r < rows; ++r) col[r * kWave] = (R)0;
@@ -644,6 +810,7 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
     win.reset((R)0);
     SlideWindow sl;
     sl.A = sl.M = sl.P = 0.0f;
+    RelWindow<(HM == 2 ? 2 : 1)> rw;
     // The lanes take the chunk's scatter points in step: the lanes of a tile
     // row (column) then read one half path at the same time, and neighbouring
     // cells mostly keep or drop the same points, so fewer lane slots idle
@@ -703,6 +870,9 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
                 } else if constexpr (SLIDE) {
                     if (p.n1 > 0) slide_path<-1>(a, tc, tb, col, sl, p.b1, p.n1, p.shift, p.coef1);
                     if (p.n2 > 0) slide_path<+1>(a, tc, tb, col, sl, p.b2, p.n2, p.shift, p.coef2);
+                } else if constexpr (REL) {
+                    if (p.n1 > 0) rel_path<HM, -1>(a, tb, col, rw, p.b1, p.n1, p.shift, p.coef1);
+                    if (p.n2 > 0) rel_path<HM, +1>(a, tb, col, rw, p.b2, p.n2, p.shift, p.coef2);
                 } else if (a.t_hi >= a.t_lo) {   // else every TOF weight is zero
                     fold_path<R, HM, TM, RANGE, -1>(a, tc, col, win, p.b1, p.n1, p.shift, p.coef1);
                     fold_path<R, HM, TM, RANGE, +1>(a, tc, col, win, p.b2, p.n2, p.shift, p.coef2);
@@ -712,7 +882,7 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
     }
     // the chunk's sums and pair count into the lane's slot (the reduce maps
     // slots to cells); window reach rows (bins beyond the ends) are dropped
-    if constexpr (!TRIVIAL && !SLIDE) win.spill(col);
+    if constexpr (!TRIVIAL && !SLIDE && !REL) win.spill(col);
     if (!valid) return;
     const int64_t n_slots = (int64_t)a.n_tiles * kWave;
     const int64_t slot = (int64_t)tile * kWave + lane;
@@ -873,7 +1043,7 @@ hipError_t launch_variant(const ScatterArgs& a, hipStream_t st)
     dim3 grid((unsigned)(a.n_items / kScatterWaves));
     size_t lds = TRIVIAL ? (sizeof(R) == 4 ? kEnergyRows * sizeof(float2) : 0)
                          : (size_t)kScatterWaves * (a.n_bins + 2 * HM) * kWave * sizeof(R);
-    if (TM == kTofSlide) lds += sizeof(SlideTables);
+    if (TM == kTofSlide || TM == kTofSlideRel) lds += sizeof(SlideTables);
     hipLaunchKernelGGL((k_scatter<R, HM, TM, TRIVIAL, RANGE>), grid,
                        dim3(kScatterWaves * kWave), lds, st, a);
     return hipGetLastError();
@@ -884,6 +1054,12 @@ hipError_t launch_hm(const ScatterArgs& a, hipStream_t st)
 {
     const int h = a.half_window;
     if (h == 0) return launch_variant<R, 0, kTofGeneral, false, RANGE>(a, st);
+    if constexpr (sizeof(R) == 4 && !RANGE) {
+        if (a.tof_mode == kTofSlideRel) {
+            if (h == 1) return launch_variant<R, 1, kTofSlideRel, false, false>(a, st);
+            if (h == 2) return launch_variant<R, 2, kTofSlideRel, false, false>(a, st);
+        }
+    }
     if (h == 1) {
         if constexpr (!RANGE) {
             if (a.tof_mode == kTofOneExpInner) {

@@ -114,10 +114,13 @@ def test_small_tof_workload_vs_oracle(oracle, precision, tol):
 TOF_VARIANTS = {
     'h1-15bins': ((15, 214.0, 3335.64), 3, 0),
     'h1-14bins-even': ((14, 214.0, 3335.64), 3, 0),
-    'h2-21bins': ((21, 214.0, 3335.64), 0, 0),
+    'h2-21bins': ((21, 214.0, 3335.64), 4, 0),
+    'h2-17bins': ((17, 214.0, 3335.64), 4, 0),
+    'h1-6bins-wide': ((6, 214.0, 3335.64), 4, 0),
+    'h1-2bins': ((2, 214.0, 3335.64), 4, 0),
     'h0-1bin-zero-columns': ((1, 214.0, 3335.64), 0, 0),
-    'h1-3bins-narrow': ((3, 214.0, 3335.64), 0, 0),
-    'h1-3bins-masked': ((3, 600.0, 2417.0), 1, 1),
+    'h1-3bins-narrow': ((3, 214.0, 3335.64), 4, 0),
+    'h1-3bins-masked': ((3, 600.0, 2417.0), 4, 1),
 }
 
 
@@ -125,9 +128,11 @@ TOF_VARIANTS = {
 def test_tof_kernel_variants_vs_oracle(oracle, case):
     """15 and 14 bins at 214 ps: one exp2 per sample with a sliding window (odd
     and even bin counts, bin centres at integer / half-integer multiples of W);
-    21 bins: h = 2; one bin: zero TOF columns beyond +-3508 ps; 3 bins at
-    214 ps: narrow kernel (two exp2); 3 bins of 1611 ps at 600 ps: offsets far
-    beyond the outer bins, so the window keeps masked end bins."""
+    21 and 17 bins: h = 2 (FP32: relative sliding window); 6, 3 and 2 bins at
+    214 ps: bins so wide (beta2 108 to 3e3) that only the relative window's
+    float range holds; one bin: zero TOF columns beyond +-3508 ps; 3 bins of
+    1611 ps at 600 ps: offsets far beyond the outer bins, so the window keeps
+    masked end bins."""
     import dataclasses
     (n_bins, res, rng), fold32, fold64 = TOF_VARIANTS[case]
     w = phantoms.small_tof(seed=12, n_bins=n_bins, n_sp=30)
