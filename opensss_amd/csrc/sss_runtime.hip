@@ -108,8 +108,13 @@ double tof_offset_bound(const Grid& g, const double* sp_xyz, int n_sp, const dou
 // (r1, r2) reads the half paths of rings r1 and r2, so each XCD is given the
 // planes of one pair of ring-group blocks {G_i x G_j, G_j x G_i} (rings in 4
 // groups; the diagonal blocks pair up as {G0 x G0, G1 x G1}, {G2 x G2, G3 x G3})
-// and reads half the rings.  Within an XCD the items run chunk-major, so the
-// waves resident at one time share one chunk's scatter points.
+// and reads half the rings.  That split is even only when the ring blocks
+// fall into 4 equal groups (NRU = 16 with 4-ring blocks); otherwise (NRU = 8
+// puts all tiles on 3 XCDs, one of them with half the work) the tiles, in
+// their order (ring-block pair, then detector blocks), are cut into 8
+// contiguous runs of equal size, so each XCD still reads few ring blocks.
+// Within an XCD the items run chunk-major, so the waves resident at one time
+// share one chunk's scatter points.
 std::vector<int2> xcd_work_items(const std::vector<int4>& tiles, int nru, int n_chunks,
                                  int waves_per_block)
 {
@@ -124,6 +129,13 @@ std::vector<int2> xcd_work_items(const std::vector<int4>& tiles, int nru, int n_
     std::vector<std::vector<int>> lists(kXcd);
     for (size_t t = 0; t < tiles.size(); ++t)
         lists[xcd_of(group(tiles[t].x), group(tiles[t].y))].push_back((int)t);
+    const size_t even = (tiles.size() + kXcd - 1) / kXcd;
+    size_t largest = 0;
+    for (const auto& l : lists) largest = std::max(largest, l.size());
+    if (largest * 8 > even * 9) {   // more than 12.5% above an even share
+        for (auto& l : lists) l.clear();
+        for (size_t t = 0; t < tiles.size(); ++t) lists[t * kXcd / tiles.size()].push_back((int)t);
+    }
     size_t per_xcd = 0;
     for (const auto& l : lists) per_xcd = std::max(per_xcd, l.size() * (size_t)n_chunks);
     const size_t blocks_per_xcd = (per_xcd + waves_per_block - 1) / waves_per_block;
