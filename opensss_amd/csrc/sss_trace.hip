@@ -98,57 +98,69 @@ __global__ void k_halfpath_fill(Grid g, const double* __restrict__ sp_xyz, int n
 
 // Emission terms lam[v] L per sample, their float64 sum per half path
 // (ActivityRay and ActPath, SingleScatterSimulationTOF.py:417-418) and the
-// number of non-zero terms (the samples the fold visits).  One thread per
-// half path: the wave reads its block's interleaved records in step.  The
-// terms are non-negative, so the exact-zero tests on the sum do not depend on
-// the order.
-__global__ void k_activity_refresh(HalfPathTable t, SampleTable s, int64_t n_hp,
-                                   const double* __restrict__ lam)
+// number of non-zero terms (the samples the fold visits).  A workgroup of
+// kRefreshWaves waves takes one block of 64 half paths: wave w the samples
+// k = w, w + kRefreshWaves, ... of every path of the block (each step reads
+// 64 consecutive interleaved records), and the waves' partial sums are added
+// in wave order.  The terms are non-negative, so the exact-zero tests on the
+// sum do not depend on the order.
+constexpr int kRefreshWaves = 4;
+
+template <bool FOLD>
+__device__ __forceinline__ void activity_block(HalfPathTable t, SampleTable s, int64_t n_hp,
+                                               const double* __restrict__ lam)
 {
-    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= n_hp) return;
-    const int64_t base = t.off[h];
-    const int n = t.n[h];
+    __shared__ double part_act[kRefreshWaves][kWave];
+    __shared__ int part_nz[kRefreshWaves][kWave];
+    const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+    const int64_t h = (int64_t)blockIdx.x * kWave + l;
     double act = 0;
     int nz = 0;
-    for (int k = 0; k < n; ++k) {
-        const int64_t i = base + (int64_t)k * kWave;
-        const double ar = lam[s.vox[i]] * s.len[i];
-        s.a[i] = (float)ar;
-        if (s.a64) s.a64[i] = ar;
-        act += ar;
-        // the fold's own precision decides which terms are zero
-        nz += s.a64 ? (ar != 0.0) : ((float)ar != 0.0f);
+    if (h < n_hp) {
+        const int64_t base = t.off[h];
+        const int n = t.n[h];
+        for (int k = w; k < n; k += kRefreshWaves) {
+            const int64_t i = base + (int64_t)k * kWave;
+            const double ar = lam[s.vox[i]] * s.len[i];
+            act += ar;
+            if constexpr (FOLD) {
+                s.a[i] = (float)ar;
+                if (s.a64) s.a64[i] = ar;
+                // the fold's own precision decides which terms are zero
+                nz += s.a64 ? (ar != 0.0) : ((float)ar != 0.0f);
+            }
+        }
     }
-    t.act[h] = act;
-    t.n_fold[h] = nz;
+    part_act[w][l] = act;
+    part_nz[w][l] = nz;
+    __syncthreads();
+    if (w == 0 && h < n_hp) {
+        double sum = part_act[0][l];
+        int cnt = part_nz[0][l];
+#pragma unroll
+        for (int j = 1; j < kRefreshWaves; ++j) {
+            sum += part_act[j][l];
+            cnt += part_nz[j][l];
+        }
+        t.act[h] = sum;
+        if constexpr (FOLD) t.n_fold[h] = cnt;
+    }
+}
+
+__global__ __launch_bounds__(kRefreshWaves * kWave) void k_activity_refresh(HalfPathTable t, SampleTable s,
+                                                                            int64_t n_hp,
+                                                                            const double* __restrict__ lam)
+{
+    activity_block<true>(t, s, n_hp, lam);
 }
 
 // k_activity_refresh for one-bin subjects, which have no fold table: only
-// the integral (the same terms added in the same order).  Four samples'
-// loads are issued before their terms are added, so a thread keeps four
-// gathers in flight.
-__global__ void k_activity_sum(HalfPathTable t, SampleTable s, int64_t n_hp, const double* __restrict__ lam)
+// the integral.
+__global__ __launch_bounds__(kRefreshWaves * kWave) void k_activity_sum(HalfPathTable t, SampleTable s,
+                                                                        int64_t n_hp,
+                                                                        const double* __restrict__ lam)
 {
-    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= n_hp) return;
-    const int64_t base = t.off[h];
-    const int n = t.n[h];
-    double act = 0;
-    int k = 0;
-    for (; k + 4 <= n; k += 4) {
-        const int64_t i = base + (int64_t)k * kWave;
-        double ar[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) ar[u] = lam[s.vox[i + u * kWave]] * s.len[i + u * kWave];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) act += ar[u];
-    }
-    for (; k < n; ++k) {
-        const int64_t i = base + (int64_t)k * kWave;
-        act += lam[s.vox[i]] * s.len[i];
-    }
-    t.act[h] = act;
+    activity_block<false>(t, s, n_hp, lam);
 }
 
 constexpr int kCompactLanes = 16;   // lanes per half path in k_fold_compact
@@ -278,7 +290,8 @@ hipError_t launch_activity_refresh(const HalfPathTable t, SampleTable s, int64_t
                                    const double* lam, hipStream_t st)
 {
     if (n_hp == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_activity_refresh, dim3(blocks_for(n_hp, 256)), dim3(256), 0, st, t, s, n_hp, lam);
+    hipLaunchKernelGGL(k_activity_refresh, dim3(blocks_for(n_hp, kWave)), dim3(kRefreshWaves * kWave), 0, st, t, s,
+                       n_hp, lam);
     return hipGetLastError();
 }
 
@@ -340,7 +353,8 @@ hipError_t launch_activity_sum(const HalfPathTable t, SampleTable s, int64_t n_h
                                hipStream_t st)
 {
     if (n_hp == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_activity_sum, dim3(blocks_for(n_hp, 256)), dim3(256), 0, st, t, s, n_hp, lam);
+    hipLaunchKernelGGL(k_activity_sum, dim3(blocks_for(n_hp, kWave)), dim3(kRefreshWaves * kWave), 0, st, t, s,
+                       n_hp, lam);
     return hipGetLastError();
 }
 
