@@ -89,16 +89,18 @@ double tof_offset_bound(const Grid& g, const double* sp_xyz, int n_sp, const dou
     const double qmax = std::sqrt(dx * dx + dy * dy + dz * dz) / kLightCmPerPs;
     double dd = 0;
     for (int p = 0; p < n_sp; ++p) {
+        // squared distances: sqrt is monotonic and correctly rounded, so the
+        // extremes of r are the roots of the extremes of r^2
         double lo = HUGE_VAL, hi = 0;
         for (int i = 0; i < n_det_pos; ++i) {
             const double ex = detpos[3 * i + 0] - sp_xyz[3 * p + 0];
             const double ey = detpos[3 * i + 1] - sp_xyz[3 * p + 1];
             const double ez = detpos[3 * i + 2] - sp_xyz[3 * p + 2];
-            const double r = std::sqrt(ex * ex + ey * ey + ez * ez);
-            lo = std::min(lo, r);
-            hi = std::max(hi, r);
+            const double r2 = ex * ex + ey * ey + ez * ez;
+            lo = std::min(lo, r2);
+            hi = std::max(hi, r2);
         }
-        dd = std::max(dd, hi - lo);
+        dd = std::max(dd, std::sqrt(hi) - std::sqrt(lo));
     }
     return dd / kLightCmPerPs + qmax + 1.0;
 }
@@ -224,7 +226,28 @@ struct sss_ctx {
     // pinned host buffers handed out by sss_host_alloc, kept for reuse
     std::multimap<size_t, void*> host_idle;
     std::unordered_map<void*, size_t> host_owned;
+    // events of destroyed subjects, kept for later subjects ([0] timing,
+    // [1] hipEventDisableTiming)
+    std::vector<hipEvent_t> events[2];
 };
+
+// An event from the context's cache, or a new one.
+static hipError_t ctx_event(sss_ctx* c, bool timing, hipEvent_t* e)
+{
+    auto& v = c->events[timing ? 0 : 1];
+    if (!v.empty()) {
+        *e = v.back();
+        v.pop_back();
+        return hipSuccess;
+    }
+    return timing ? hipEventCreate(e) : hipEventCreateWithFlags(e, hipEventDisableTiming);
+}
+
+static void ctx_event_put(sss_ctx* c, bool timing, hipEvent_t& e)
+{
+    if (e) c->events[timing ? 0 : 1].push_back(e);
+    e = nullptr;
+}
 
 static constexpr int kPinnedSlots = 4096;
 
@@ -342,15 +365,14 @@ struct sss_subject {
 
     ~sss_subject()
     {
-        for (auto& e : ev_scatter)
-            if (e) (void)hipEventDestroy(e);
+        for (auto& e : ev_scatter) ctx_event_put(ctx, true, e);
         if (h_total) ctx->pinned_free.push_back((int)(h_total - ctx->pinned));
         if (h_upload) host_block_free(ctx, h_upload);
         if (h_lam) host_block_free(ctx, h_lam);
-        if (ev_count) (void)hipEventDestroy(ev_count);
-        if (ev_out) (void)hipEventDestroy(ev_out);
-        if (ev_lam) (void)hipEventDestroy(ev_lam);
-        if (ev_upload) (void)hipEventDestroy(ev_upload);
+        ctx_event_put(ctx, false, ev_count);
+        ctx_event_put(ctx, false, ev_out);
+        ctx_event_put(ctx, false, ev_lam);
+        ctx_event_put(ctx, false, ev_upload);
         sss_ctx* c = ctx;
         pool_free(c, d_upload);   // d_mu (x fastest) .. d_dets live in it
         pool_free(c, d_mu_own);
@@ -401,6 +423,8 @@ int sss_ctx_destroy(sss_ctx* ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->pool.release_idle();
+    for (auto& v : ctx->events)
+        for (hipEvent_t e : v) (void)hipEventDestroy(e);
     for (auto& kv : ctx->host_owned) (void)hipHostFree(kv.first);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -756,12 +780,12 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
         s->d_mu = s->d_mu_own;
         if (e == hipSuccess) e = launch_volume_x_fastest(s->d_vol_in, s->d_mu, g.nx, g.ny, g.nz, st);
     }
-    if (e == hipSuccess) e = hipEventCreate(&s->ev_scatter[0]);
-    if (e == hipSuccess) e = hipEventCreate(&s->ev_scatter[1]);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_count, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_out, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_lam, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_upload, hipEventDisableTiming);
+    if (e == hipSuccess) e = ctx_event(ctx, true, &s->ev_scatter[0]);
+    if (e == hipSuccess) e = ctx_event(ctx, true, &s->ev_scatter[1]);
+    if (e == hipSuccess) e = ctx_event(ctx, false, &s->ev_count);
+    if (e == hipSuccess) e = ctx_event(ctx, false, &s->ev_out);
+    if (e == hipSuccess) e = ctx_event(ctx, false, &s->ev_lam);
+    if (e == hipSuccess) e = ctx_event(ctx, false, &s->ev_upload);
     if (e == hipSuccess) e = hipEventRecord(s->ev_upload, st);
     if (e == hipSuccess && ctx->pinned_free.empty()) e = hipErrorOutOfMemory;   // kPinnedSlots live subjects
     if (e == hipSuccess) {

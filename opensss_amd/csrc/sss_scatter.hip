@@ -170,9 +170,12 @@ __device__ __forceinline__ void pair_coef(const ScatterArgs& a, const PairHalf& 
     const float a1s = fast_exp2(-ratio * r1.lnatt * log2e);
     const float a2s = fast_exp2(-ratio * r2.lnatt * log2e);
     const float c = (float)carg;
-    const float rho = 1.0f / (1.0f + (float)a.kn_gamma * (1.0f + c));
-    const float kn = rho * rho * (rho + 1.0f / rho - (1.0f - c * c));
-    const float geo = (float)a.g_const * r1.cosabs * r2.cosabs / ((float)s1q * (float)s2q);
+    // rho = 1 / q, so 1 / rho is q itself; the two quotients use the
+    // hardware reciprocal (1 ulp), well inside the FP32 path's tolerance
+    const float q = 1.0f + (float)a.kn_gamma * (1.0f + c);
+    const float rho = fast_rcp(q);
+    const float kn = rho * rho * (rho + q - (1.0f - c * c));
+    const float geo = (float)a.g_const * r1.cosabs * r2.cosabs * fast_rcp((float)s1q * (float)s2q);
     const float pre = geo * eff * kn;
     p.coef1 = pre * r1.att * a2s;
     p.coef2 = pre * r2.att * a1s;

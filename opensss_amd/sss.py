@@ -47,6 +47,24 @@ def _energy_tables(table, resolution, threshold):
     return hit
 
 
+_SAMPLING_CACHE: dict = {}
+
+
+def _sampling(n_rings, nru, n_det, ndu):
+    key = (int(n_rings), int(nru), int(n_det), int(ndu))
+    hit = _SAMPLING_CACHE.get(key)
+    if hit is None:
+        rings = _i32(rings_used(key[0], key[1]))
+        dets = _i32(detectors_used(key[2], key[3]))
+        rings.setflags(write=False)
+        dets.setflags(write=False)
+        hit = (rings, dets)
+        if len(_SAMPLING_CACHE) >= _CACHE_MAX:
+            _SAMPLING_CACHE.clear()
+        _SAMPLING_CACHE[key] = hit
+    return hit
+
+
 def _tof_kernel(n_bins, tof_range, resolution, window_tol):
     key = (int(n_bins), float(tof_range), float(resolution), float(window_tol))
     hit = _TOF_CACHE.get(key)
@@ -94,8 +112,7 @@ class ScatterSubject:
         # the FP64 validation path keeps every bin above 1e-16 of the nearest one
         self.tof = _tof_kernel(self.n_bins, float(TOFRange), float(TOFResolution),
                                1e-16 if precision == 'fp64' else 1e-9)
-        self.rings = _i32(rings_used(self.n_rings, self.nru))
-        self.dets = _i32(detectors_used(self.n_det, self.ndu))
+        self.rings, self.dets = _sampling(self.n_rings, self.nru, self.n_det, self.ndu)
         if scatter_points is None:
             scatter_points = strided_scatter_points(mu, SampleStep)
         pts, ptr, zs = group_by_slice(scatter_points)
