@@ -189,7 +189,9 @@ def serve_bench(args):
                        'max': float(lat.max() * 1e3)},
         'subject_load_s': t_load, 'resident_bytes': int(resident), 'server': server_stats,
         'one_instance_path': {'value': plain_rate, 'requests': len(head), 'seconds': t_plain},
-        'speedup': served_rate / plain_rate,
+        # below saturation the served rate is the offered load, not a capacity
+        'saturated': bool(n_req / t_end < 0.95 * args.rate),
+        'speedup': served_rate / plain_rate if n_req / t_end < 0.95 * args.rate else None,
         'max_rel_err_vs_fp64': max(errs) if errs else None,
     }
     print(json.dumps(line))
