@@ -45,7 +45,7 @@ def _collect(s, addr):
 def run_batch(instances, attenuation_table, n_streams=4, precision='fp32', contexts=None):
     """Compact outputs (B, NRU^2, NDU(NDU+1)/2) of every instance, in order.
 
-    n_streams contexts take the instances round-robin; up to 2 x n_streams
+    n_streams contexts take the instances round-robin; up to 2 x n_streams + 1
     instances hold device memory at once.  An instance's download is queued
     right behind its scatter into pinned memory, so collecting it waits for
     that instance only.
