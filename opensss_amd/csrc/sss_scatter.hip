@@ -976,22 +976,40 @@ __global__ void k_chunk_reduce(ScatterArgs a)
     for (int b = 0; b < BMAX; ++b) total[b] = comp[b] = num[b] = (R)0;
     float cnt = 0.0f;
     int slice = a.n_chunks > 0 ? a.chunk_slice[0] : 0;
-    for (int c = 0; c < a.n_chunks; ++c) {
-        if (a.chunk_slice[c] != slice) {
-            if (cnt > 0.0f) {
+    // kReduceAhead chunks' partials are loaded before any is added (indices
+    // clamped into range; the extra values are not used), so a slot with
+    // many chunks and few bins (small subjects) keeps loads in flight
+    constexpr int kReduceAhead = 4;
+    for (int c0 = 0; c0 < a.n_chunks; c0 += kReduceAhead) {
+        R v[kReduceAhead][BMAX];
+        float vc[kReduceAhead];
 #pragma unroll
-                for (int b = 0; b < BMAX; ++b) kahan_add(total[b], comp[b], num[b] / (R)cnt);
+        for (int u = 0; u < kReduceAhead; ++u) {
+            const int c = min(c0 + u, a.n_chunks - 1);
+            const R* src = part_sum + (int64_t)c * B * n_slots + slot;
+#pragma unroll
+            for (int b = 0; b < BMAX; ++b) v[u][b] = src[min(b, nb - 1) * n_slots];
+            vc[u] = a.part_cnt[(int64_t)c * n_slots + slot];
+        }
+#pragma unroll
+        for (int u = 0; u < kReduceAhead; ++u) {
+            const int c = c0 + u;
+            if (c >= a.n_chunks) break;
+            if (a.chunk_slice[c] != slice) {
+                if (cnt > 0.0f) {
+#pragma unroll
+                    for (int b = 0; b < BMAX; ++b) kahan_add(total[b], comp[b], num[b] / (R)cnt);
+                }
+#pragma unroll
+                for (int b = 0; b < BMAX; ++b) num[b] = (R)0;
+                cnt = 0.0f;
+                slice = a.chunk_slice[c];
             }
 #pragma unroll
-            for (int b = 0; b < BMAX; ++b) num[b] = (R)0;
-            cnt = 0.0f;
-            slice = a.chunk_slice[c];
+            for (int b = 0; b < BMAX; ++b)
+                if (b < nb) num[b] += v[u][b];
+            cnt += vc[u];
         }
-        const R* src = part_sum + (int64_t)c * B * n_slots + slot;
-#pragma unroll
-        for (int b = 0; b < BMAX; ++b)
-            if (b < nb) num[b] += src[b * n_slots];
-        cnt += a.part_cnt[(int64_t)c * n_slots + slot];
     }
     const int64_t n_cells = (int64_t)a.nru * a.nru * a.n_cells_plane;
     const int64_t cell = (int64_t)lc.i1 * a.ndu - (int64_t)lc.i1 * (lc.i1 - 1) / 2 + (lc.i2 - lc.i1);
