@@ -173,10 +173,7 @@ def serve_bench(args):
         r = reqs[i]
         w = dataclasses.replace(subs[r.key], activity=r.activity)
         ref = run_plain([w], water, precision='fp64')[0]
-        got = kept[i].astype(np.float64)
-        peak = np.abs(ref).max()
-        m = np.abs(ref) > 0.01 * peak
-        errs.append(float((np.abs(got - ref)[m] / np.abs(ref)[m]).max()) if m.any() else 0.0)
+        errs.append(rel_err_stats(kept[i], ref))
     line = {
         'metric': 'served scatter-pt x LOR x TOF-bin evals/s; p50/p99 request latency',
         'value': served_rate, 'unit': 'evals/s', 'n_gpus': 1, 'higher_is_better': True, 'dtype': 'f32',
@@ -192,7 +189,9 @@ def serve_bench(args):
         # below saturation the served rate is the offered load, not a capacity
         'saturated': bool(n_req / t_end < 0.95 * args.rate),
         'speedup': served_rate / plain_rate if n_req / t_end < 0.95 * args.rate else None,
-        'max_rel_err_vs_fp64': max(errs) if errs else None,
+        'max_rel_err_vs_fp64': max(e['max_rel_err_cells_above_1pct_peak'] for e in errs) if errs else None,
+        'mean_rel_err_vs_fp64': float(np.mean([e['mean_rel_err_cells_above_1pct_peak'] for e in errs]))
+        if errs else None,
     }
     print(json.dumps(line))
 
@@ -223,12 +222,13 @@ def make_workload(name):
     return phantoms.config2()
 
 
-def subject(w, precision):
+def subject(w, precision, points=None):
     from opensss_amd.sss import ScatterSubject
     water = np.load(os.path.join(ROOT, 'opensss_amd', 'data', 'attenuation_water.npy'))
     return ScatterSubject(w.attenuation, w.image_size, w.geometry, w.normals, w.det_size, water,
                           w.energy_resolution, w.energy_threshold, w.n_rings_used, w.n_det_used,
-                          w.tof_resolution, w.tof_range, w.n_bins, scatter_points=w.scatter_points,
+                          w.tof_resolution, w.tof_range, w.n_bins,
+                          scatter_points=w.scatter_points if points is None else points,
                           precision=precision)
 
 
@@ -243,20 +243,26 @@ def profile_traffic(workload):
     return (float(m.group(1)), os.path.relpath(path, ROOT)) if m else (None, None)
 
 
-def rel_err_above(got, want, frac=0.01):
+def rel_err_stats(got, want, frac=0.01):
+    """Max and mean relative error on cells above frac of the peak, max
+    absolute error over the peak."""
     got = np.asarray(got, dtype=np.float64)
     want = np.asarray(want, dtype=np.float64)
     peak = np.abs(want).max()
     big = np.abs(want) >= frac * peak
-    return (float((np.abs(got[big] - want[big]) / np.abs(want[big])).max()),
-            float(np.abs(got - want).max() / peak))
+    rel = np.abs(got[big] - want[big]) / np.abs(want[big])
+    return {'max_rel_err_cells_above_1pct_peak': float(rel.max()),
+            'mean_rel_err_cells_above_1pct_peak': float(rel.mean()),
+            'max_abs_err_over_peak': float(np.abs(got - want).max() / peak)}
 
 
 def cpu_baseline(w, threads, target_s=12.0):
     """Oracle (float64 C restatement) on the first k scatter points of each of
     the first `threads` z slices (OpenMP runs one slice per thread); k is
     sized from a one-point-per-slice run so the timed run takes about
-    target_s seconds.  evals/s scaled by the sample's evaluation count."""
+    target_s seconds.  evals/s scaled by the sample's evaluation count.
+    Returns the baseline and the oracle's compact result for the sample, with
+    the sample's points, to check the GPU path against."""
     from oracle import oracle as ora
     from opensss_amd.tables import detectors_used, group_by_slice, rings_used
     ora.build()
@@ -266,24 +272,34 @@ def cpu_baseline(w, threads, target_s=12.0):
     nru, ndu = w.n_rings_used, w.n_det_used
     lors = nru * (ndu * (ndu - 1) // 2) + nru * (nru - 1) * (ndu * (ndu + 1) // 2)
 
+    dets = detectors_used(w.geometry.shape[1], ndu)
+
     def timed(per_slice):
         take = [ptr[i] + j for i in range(n_sl) for j in range(min(per_slice, ptr[i + 1] - ptr[i]))]
         sample = pts[take]
         sp, sptr, _ = group_by_slice(sample)
         t0 = time.perf_counter()
-        ora.sss(w.activity, w.attenuation, w.image_size, w.geometry, w.lor_coordinates, w.normals,
-                w.det_size, water, w.energy_resolution, w.energy_threshold,
-                rings_used(w.geometry.shape[0], nru), detectors_used(w.geometry.shape[1], ndu),
-                w.tof_range, w.tof_resolution, w.n_bins, sp, sptr, n_threads=threads, dense=False)
-        return len(sample), time.perf_counter() - t0
+        out, cell_of = ora.sss(w.activity, w.attenuation, w.image_size, w.geometry, w.lor_coordinates,
+                               w.normals, w.det_size, water, w.energy_resolution, w.energy_threshold,
+                               rings_used(w.geometry.shape[0], nru), dets, w.tof_range, w.tof_resolution,
+                               w.n_bins, sp, sptr, n_threads=threads, dense=False)
+        return sample, time.perf_counter() - t0, out, cell_of
 
-    _, t1 = timed(1)
+    _, t1, _, _ = timed(1)
     per_slice = max(1, int(round(target_s / max(t1, 1e-3))))
-    n, dt = timed(per_slice)
+    sample, dt, out, cell_of = timed(per_slice)
+    n = len(sample)
     evals = n * lors * w.n_bins
-    return {'value': evals / dt, 'unit': 'evals/s', 'cores': threads, 'kind': 'port',
+    # oracle result in the GPU's compact layout (B, NRU^2, NDU(NDU+1)/2)
+    N = w.geometry.shape[1]
+    i1, i2 = np.triu_indices(len(dets))
+    ang = w.lor_coordinates[dets[i1], dets[i2], 0].astype(np.int64)
+    rad = w.lor_coordinates[dets[i1], dets[i2], 1].astype(np.int64)
+    compact = out[:, :, cell_of[rad * (N // 2) + ang]]
+    base = {'value': evals / dt, 'unit': 'evals/s', 'cores': threads, 'kind': 'port',
             'sample': f'{n} scatter points (the first {per_slice} of each of {n_sl} z slices) of {w.name}, '
                       f'{evals:.3e} evals in {dt:.2f} s, float64 C oracle, OpenMP over slices'}
+    return base, sample, compact
 
 
 def main():
@@ -352,15 +368,24 @@ def main():
         s64.run(w.activity)
         out64 = s64.output_compact()
         s64.close()
-        rel, absn = rel_err_above(out32, out64)
-        err = {'max_rel_err_cells_above_1pct_peak': rel, 'max_abs_err_over_peak': absn}
-        print(f'[bench] FP32 vs FP64: max rel err {rel:.3e}', file=sys.stderr, flush=True)
+        err = rel_err_stats(out32, out64)
+        print(f"[bench] FP32 vs FP64: max rel err {err['max_rel_err_cells_above_1pct_peak']:.3e}",
+              file=sys.stderr, flush=True)
 
     cpu = None
+    err_oracle = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(w, threads)
+        cpu, sample, want = cpu_baseline(w, threads)
         print(f"[bench] cpu baseline {cpu['value']:.3e} evals/s", file=sys.stderr, flush=True)
+        # the FP32 GPU path on the same scatter points, checked against the oracle
+        ss = subject(w, 'fp32', points=sample)
+        ss.run(w.activity)
+        err_oracle = rel_err_stats(ss.output_compact(), want)
+        ss.close()
+        err_oracle['sample'] = f'{len(sample)} scatter points of the cpu_baseline sample'
+        print(f"[bench] FP32 vs oracle: max rel err {err_oracle['max_rel_err_cells_above_1pct_peak']:.3e}",
+              file=sys.stderr, flush=True)
 
     lat_ms = np.array(lat) * 1e3
     line = {
@@ -376,6 +401,7 @@ def main():
                    'parallelism': 'single instance, 1 GPU'},
         'latency_ms': {'p50': float(np.percentile(lat_ms, 50)), 'p99': float(np.percentile(lat_ms, 99))},
         'error_vs_fp64': err,
+        'error_vs_oracle': err_oracle,
         'roofline': roofline,
         'cpu_baseline': cpu,
     }
