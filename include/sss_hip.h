@@ -79,6 +79,9 @@ typedef struct {
     int32_t precision;           /* SSS_PRECISION_* */
     int32_t n_chunks;            /* target number of scatter-point chunks (each inside one
                                     slice) for the scatter launch; 0 = auto */
+    int32_t tile_rings;          /* rings per block of a scatter tile's rows and columns
+                                    (1, 2, 4 or 8; the block holds 8 / tile_rings detectors);
+                                    0 = the default (4, or NRU when smaller) */
     int32_t volume_order;        /* layout of mu here and of every activity image (host or
                                     device) given to the subject: SSS_VOLUME_X_FASTEST (index
                                     x + X y + X Y z, the reference's flatten(order='F')) or

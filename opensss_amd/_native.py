@@ -51,7 +51,7 @@ class SubjectDesc(ctypes.Structure):
         ('tof_beta', ctypes.c_double), ('tof_half_window', ctypes.c_int32),
         ('tof_t_lo', ctypes.c_int32), ('tof_t_hi', ctypes.c_int32),
         ('tof_trivial', ctypes.c_int32), ('precision', ctypes.c_int32),
-        ('n_chunks', ctypes.c_int32), ('volume_order', ctypes.c_int32),
+        ('n_chunks', ctypes.c_int32), ('tile_rings', ctypes.c_int32), ('volume_order', ctypes.c_int32),
     ]
 
 

@@ -601,6 +601,8 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
         return fail(SSS_ERR_INVALID, "sss_subject_create: bad precision");
     if (d->volume_order != SSS_VOLUME_X_FASTEST && d->volume_order != SSS_VOLUME_Z_FASTEST)
         return fail(SSS_ERR_INVALID, "sss_subject_create: bad volume_order");
+    if (d->tile_rings != 0 && d->tile_rings != 1 && d->tile_rings != 2 && d->tile_rings != 4 && d->tile_rings != 8)
+        return fail(SSS_ERR_INVALID, "sss_subject_create: tile_rings must be 0, 1, 2, 4 or 8");
     for (int r = 0; r < d->nru; ++r)
         if (d->rings[r] < 0 || d->rings[r] >= d->n_rings)
             return fail(SSS_ERR_INVALID, "sss_subject_create: ring id out of range");
@@ -657,7 +659,11 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
     // (rb * db = 8, rb the largest power of two <= min(4, NRU)); detector
     // blocks cover the upper triangle i1 <= i2
     int rb = 1;
-    while (rb * 2 <= std::min(kTileMaxRings, d->nru)) rb *= 2;
+    if (d->tile_rings > 0) {
+        rb = d->tile_rings;
+    } else {
+        while (rb * 2 <= std::min(kTileMaxRings, d->nru)) rb *= 2;
+    }
     const int db = kTileRows / rb;
     s->tile_db_log2 = 0;
     while ((1 << s->tile_db_log2) < db) ++s->tile_db_log2;

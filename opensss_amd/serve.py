@@ -17,7 +17,9 @@ free up (continuous batching, not a queue drained batch by batch):
   the work of requests for different subjects overlaps on the device;
 - a subject is made resident only when its device-memory footprint fits the
   budget; idle subjects are evicted least recently used first to make room,
-  and a request whose subject cannot fit yet waits.
+  and a request whose subject cannot fit yet waits;
+- with ``autotune``, a subject's first load times the scatter launch for
+  each candidate tile shape (``tune_tile_rings``) and keeps the fastest.
 
 Every request's result is the one-instance path's result for the same
 subject and activity, bit for bit: the same kernels in the same order on
@@ -61,11 +63,38 @@ def footprint_estimate(inst) -> int:
     return 4 * n_vox * 8 + n_hp * 120 + n_hp * walk * 48 + 3 * out
 
 
+def tune_tile_rings(inst, attenuation_table, candidates=(2, 4), precision='fp32', context=None, repeats=2):
+    """Tile shape of the scatter launch for one subject, picked by timing:
+    for each candidate (rings per tile block; 8 / rings detectors) a subject
+    is created, traced and given the instance's activity, and its scatter
+    kernel is timed (HIP events, best of `repeats`).  Results do not depend
+    on the tile shape.  Returns (best, {candidate: ms})."""
+    times = {}
+    for rb in candidates:
+        s = ScatterSubject(inst.attenuation, inst.image_size, inst.geometry, inst.normals, inst.det_size,
+                           attenuation_table, inst.energy_resolution, inst.energy_threshold,
+                           inst.n_rings_used, inst.n_det_used, inst.tof_resolution, inst.tof_range,
+                           inst.n_bins, scatter_points=inst.scatter_points, precision=precision,
+                           context=context, tile_rings=rb)
+        try:
+            s.run(inst.activity)
+            ms = []
+            for _ in range(max(1, repeats)):
+                s.scatter()
+                ms.append(s.scatter_kernel_ms())
+            times[rb] = min(ms)
+        finally:
+            s.close()
+    return min(times, key=times.get), times
+
+
 class ScatterServer:
     def __init__(self, instances: dict, attenuation_table, n_contexts: int = 2,
                  max_inflight: Optional[int] = None, hbm_budget: Optional[int] = None,
-                 precision: str = 'fp32', contexts=None):
+                 precision: str = 'fp32', contexts=None, autotune: bool = False):
         self.instances = dict(instances)
+        self.autotune = autotune
+        self.tile_rings: dict = {}          # key -> tile rings picked by the tuner
         self.table = attenuation_table
         self.precision = precision
         self._own_ctx = contexts is None
@@ -112,11 +141,14 @@ class ScatterServer:
                 return None
         if need > self.hbm_budget:
             raise MemoryError(f'subject {key!r} needs about {need} bytes, budget {self.hbm_budget}')
+        if self.autotune and key not in self.tile_rings:
+            self.tile_rings[key] = tune_tile_rings(inst, self.table, precision=self.precision,
+                                                   context=self._context(key))[0]
         s = ScatterSubject(inst.attenuation, inst.image_size, inst.geometry, inst.normals, inst.det_size,
                            self.table, inst.energy_resolution, inst.energy_threshold, inst.n_rings_used,
                            inst.n_det_used, inst.tof_resolution, inst.tof_range, inst.n_bins,
                            scatter_points=inst.scatter_points, precision=self.precision,
-                           context=self._context(key))
+                           context=self._context(key), tile_rings=self.tile_rings.get(key, 0))
         s.trace()
         self.resident[key] = (s, int(s.info['device_bytes']))
         self.stats['loaded'] += 1

@@ -87,7 +87,7 @@ class ScatterSubject:
                  AttenuationTable, EnergyResolution, EnergyThreshold, NrRingsUsed,
                  NrDetectorsUsed, TOFResolution, TOFRange, NrBins, *, scatter_points=None,
                  SampleStep=(1, 1, 1), precision='fp32', context=None, n_chunks=0,
-                 upload_order='C'):
+                 upload_order='C', tile_rings=0):
         mu = np.asarray(AttenuationMap)
         if mu.ndim != 3:
             raise ValueError('AttenuationMap must be 3-D (X, Y, Z)')
@@ -151,6 +151,7 @@ class ScatterSubject:
         d.tof_trivial = int(self.tof.trivial)
         d.precision = PRECISIONS[precision]
         d.n_chunks = int(n_chunks)
+        d.tile_rings = int(tile_rings)
         d.volume_order = (_native.SSS_VOLUME_Z_FASTEST if upload_order == 'C'
                           else _native.SSS_VOLUME_X_FASTEST)
         h = _native.ctypes.c_void_p()

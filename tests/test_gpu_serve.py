@@ -77,3 +77,37 @@ def test_admission_by_footprint_evicts_and_reloads():
     assert stats['evicted'] > 0 and stats['loaded'] > 3
     for r in reqs:
         assert np.array_equal(r.output, _alone(subs, r)), f'request {r.tag} (subject {r.key})'
+
+
+def test_tile_shapes_give_identical_results_and_autotune_serves_them():
+    """Every tile shape gives the same bits (each cell keeps its order of
+    sums); the server with autotune picks one by timing and serves the
+    one-instance path's results."""
+    from opensss_amd.batch import run_plain
+    from opensss_amd.serve import ScatterServer, serve, tune_tile_rings
+    from opensss_amd.sss import ScatterSubject
+    w = phantoms.small_tof(seed=21, n_sp=30, n_bins=7, n_rings=8, nru=8, ndu=16)
+    outs = []
+    for rb in (1, 2, 4, 8):
+        s = ScatterSubject(w.attenuation, w.image_size, w.geometry, w.normals, w.det_size, WATER,
+                           w.energy_resolution, w.energy_threshold, w.n_rings_used, w.n_det_used,
+                           w.tof_resolution, w.tof_range, w.n_bins, scatter_points=w.scatter_points,
+                           tile_rings=rb)
+        try:
+            s.run(w.activity)
+            outs.append(s.output_compact())
+        finally:
+            s.close()
+    assert all(np.array_equal(outs[0], o) for o in outs[1:])
+    best, times = tune_tile_rings(w, WATER)
+    assert best in (2, 4) and set(times) == {2, 4} and all(t > 0 for t in times.values())
+    subs = {'a': w}
+    reqs = _requests(subs, 3, seed=5)
+    server = ScatterServer(subs, WATER, n_contexts=1, autotune=True)
+    try:
+        serve(server, reqs)
+        assert server.tile_rings['a'] in (2, 4)
+    finally:
+        server.close()
+    for r in reqs:
+        assert np.array_equal(r.output, _alone(subs, r))

@@ -43,9 +43,11 @@ class FakeSubject:
     log = []
     sizes = {}   # device bytes per subject key (default 1000)
 
-    def __init__(self, attenuation, *args, context=None, scatter_points=None, precision='fp32', **kw):
+    def __init__(self, attenuation, *args, context=None, scatter_points=None, precision='fp32',
+                 tile_rings=0, **kw):
         self.ctx = context
         self.key = int(attenuation.flat[0])
+        self.tile_rings = tile_rings
         self.output_shape = (1, 2, 3)
         self.output_dtype = np.float32
         self.output_nbytes = 24
@@ -61,15 +63,23 @@ class FakeSubject:
     def trace(self):
         pass
 
+    def run(self, lam):
+        self.set_activity(lam)
+
+    def scatter_kernel_ms(self):
+        # the stand-in's fastest tile shape: 2 rings for even keys, 4 for odd
+        best = 2 if self.key % 2 == 0 else 4
+        return 1.0 if self.tile_rings == best else 2.0
+
     def set_activity(self, lam):
         assert not self.busy, 'activity changed while a request is in flight'
         self.lam = np.asarray(lam)
 
     def scatter(self):
-        self.busy = True
         self.value = np.float32((self.key + 1) * float(self.lam.sum()))
 
     def output_compact_async(self, addr):
+        self.busy = True
         self.pending = addr
         self.polls = 0
 
@@ -177,4 +187,18 @@ def test_subjects_spread_over_contexts(fake):
     server = serve.ScatterServer(subs, None, contexts=fake)
     ctxs = [server.load(k).ctx for k in range(4)]
     assert ctxs == [fake[0], fake[1], fake[0], fake[1]]
+    server.close()
+
+
+def test_autotune_keeps_the_fastest_tile_shape(fake):
+    subs = {k: _instance(k) for k in range(3)}
+    for w in subs.values():
+        w.activity = np.ones((4, 4, 2))
+    reqs = _requests([0, 1, 2, 0])
+    server = serve.ScatterServer(subs, None, contexts=fake, autotune=True)
+    serve.serve(server, reqs)
+    assert server.tile_rings == {0: 2, 1: 4, 2: 2}
+    assert [server.resident[k][0].tile_rings for k in range(3)] == [2, 4, 2]
+    for r in reqs:
+        assert np.all(r.output == np.float32((r.key + 1) * float(r.activity.sum())))
     server.close()
