@@ -51,6 +51,8 @@ def parse():
     p.add_argument('--subjects', type=int, default=16, help='subjects of --serve (uneven random instances)')
     p.add_argument('--subject-kind', default='random', choices=['random', 'config1'],
                    help='subjects of --serve: uneven random instances or config 1 instances')
+    p.add_argument('--autotune', action='store_true',
+                   help='--serve: pick each subject\'s tile shape by timing when it is loaded')
     p.add_argument('--noise-fields', type=int, default=3,
                    help='precomputed 5%% noise fields per subject that the requests cycle through')
     return p.parse_args()
@@ -132,7 +134,7 @@ def serve_bench(args):
     print(f'[bench] serve: {n_req} requests over {args.serve:.0f} s, {args.subjects} subjects', file=sys.stderr,
           flush=True)
 
-    server = ScatterServer(subs, water, n_contexts=args.streams)
+    server = ScatterServer(subs, water, n_contexts=args.streams, autotune=args.autotune)
     t0 = time.perf_counter()
     for k in subs:
         server.load(k)
@@ -180,6 +182,7 @@ def serve_bench(args):
         'data': 'synthetic',
         'config': {'workload': f'{args.subject_kind}-subjects-0..{args.subjects - 1}', 'requests': n_req,
                    'offered_rate_per_s': args.rate, 'seconds': args.serve, 'contexts': args.streams,
+                   'autotune': bool(args.autotune),
                    'noise': '5% multiplicative per voxel'},
         'completed_per_s': n_req / t_end,
         'latency_ms': {'p50': float(np.percentile(lat, 50) * 1e3), 'p99': float(np.percentile(lat, 99) * 1e3),

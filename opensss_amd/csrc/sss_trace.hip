@@ -180,9 +180,20 @@ __device__ __forceinline__ void fold_store(SampleTable s, int64_t r, int64_t nq,
 // half path to a whole group with copies of its last kept record of weight
 // 0.  One group of kCompactLanes lanes per half path; a lane's place among
 // the kept samples of one pass comes from the wave's ballot.
-__global__ void k_fold_compact(HalfPathTable t, SampleTable s, int64_t n_hp)
+// Workgroups of 256 threads take 16 half paths, so the 64 paths of one
+// interleaved block (which share cache lines) take 4 workgroups.  Those 4 go
+// to the same XCD (workgroup g runs on XCD g % 8), so the block's lines are
+// fetched into one L2 once rather than into 4.
+constexpr int kXcds = 8;
+constexpr int kCompactParts = kWave * kCompactLanes / 256;
+
+__global__ __launch_bounds__(256) void k_fold_compact(HalfPathTable t, SampleTable s, int64_t n_hp)
 {
-    const int64_t h = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kCompactLanes;
+    const int64_t g = blockIdx.x;
+    const int64_t slot = g / kXcds;
+    const int64_t path_block = (slot / kCompactParts) * kXcds + g % kXcds;
+    const int64_t h = path_block * kWave + (slot % kCompactParts) * (256 / kCompactLanes) +
+                      threadIdx.x / kCompactLanes;
     const int j = threadIdx.x % kCompactLanes;
     const int g0 = (threadIdx.x % kWave) - j;   // the group's first lane in the wave
     if (h >= n_hp) return;
@@ -361,7 +372,9 @@ hipError_t launch_activity_sum(const HalfPathTable t, SampleTable s, int64_t n_h
 hipError_t launch_fold_compact(const HalfPathTable t, SampleTable s, int64_t n_hp, hipStream_t st)
 {
     if (n_hp == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fold_compact, dim3(blocks_for(n_hp * kCompactLanes, 256)), dim3(256), 0, st,
+    const int64_t path_blocks = (n_hp + kWave - 1) / kWave;
+    const int64_t groups = (path_blocks + kXcds - 1) / kXcds;   // of kXcds path blocks
+    hipLaunchKernelGGL(k_fold_compact, dim3((unsigned)(groups * kXcds * kCompactParts)), dim3(256), 0, st,
                        t, s, n_hp);
     return hipGetLastError();
 }
