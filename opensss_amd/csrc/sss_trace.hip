@@ -80,14 +80,21 @@ __global__ void k_halfpath_fill(Grid g, const double* __restrict__ sp_xyz, int n
     if (walk_setup(g, lc, w)) {
         const double two_over_c = 2.0 / kLightCmPerPs;
         bool truncated = false;
+        // the mu gather of a step is added one step later, so its latency
+        // overlaps the next step (same terms, same order: the first added
+        // term is 0 * 0)
+        double mu_prev = 0, len_prev = 0;
         const int got = walk_run(g, w, [&](int k, int vox, double L, double smp) {
             const int64_t i = base + (int64_t)k * kWave;
             // q * 2^32 is exact in float64; rint drops bits below 2^-32 ps
             s.nq[i] = -(int64_t)rint(smp * two_over_c * kFixOne);
             s.vox[i] = vox;
             s.len[i] = L;
-            mu_int += mu[vox] * L;
+            mu_int += mu_prev * len_prev;
+            mu_prev = mu[vox];
+            len_prev = L;
         }, &truncated);
+        mu_int += mu_prev * len_prev;
         if (truncated) status = 1;
         t.n[h] = got;   // <= the count the offsets were built from
     }
