@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cstdint>
 #include <string>
 
@@ -168,18 +169,13 @@ struct ScatterArgs {
     int range_check;                               // offsets may leave [t_lo, t_hi]: test each sample
     // the same constants rounded to float once on the host (uniform registers)
     float f_inv_w, f_kap, f_mid, f_jmax, f_wk, f_mid_wk, f_c, f_inv_c;
-    // sliding window (kTofSlide): LDS row of bin b is b + slide_row0; per
-    // centre bin b in [1, B-2] with L(b) = ceil((b - mid - 1/2) W): the
-    // threshold of the high word below which the centre moves on the path to
-    // D1 (L(b), INT_MIN for b = 1) and to D2 (1 - L(b+1), INT_MIN for
-    // b = B-2), and kappa2 (mid - b) W
+    // sliding windows (kTofSlide, kTofSlideRel): LDS row of bin b is
+    // b + slide_row0; centre thresholds L(b) = ceil((b - mid - 1/2) W) and
+    // ku offsets kappa2 (mid - b) W are computed from these float constants
     int slide_row0;
-    int slide_thr_d1[kMaxBins], slide_thr_d2[kMaxBins];
-    float slide_kc[kMaxBins];
-    // kTofSlideRel: centre b in [0, B-1] (thresholds INT_MIN at the ends);
-    // slide_kc[b] = kappa2 (mid - b) W - beta2, slide_km[b] = -kappa2 (mid - b) W - beta2
-    float slide_km[kMaxBins];
     float f_beta2, f_c2;         // beta2, 2^(-2 beta2)
+    float f_w, f_l0, f_l1;       // W, (-mid - 1/2) W, (1/2 - mid) W: L(b) = ceil(b W + f_l0)
+    float f_wk_lo, f_mid_wk_lo;  // W kappa2 - f_wk, mid W kappa2 - f_mid_wk (the rounding parts)
     double g_const;              // (Dx Dy 1e-2)^2 / (4 pi)
     double kn_gamma;
     // output: per chunk, the un-normalised sums and the pair counts of every

@@ -1033,17 +1033,12 @@ static ScatterArgs scatter_args(const sss_subject* s)
     a.f_inv_c = (float)std::exp2(a.beta2);
     a.slide_row0 = s->half_window;
     a.f_beta2 = (float)a.beta2;
+    a.f_w = (float)s->bin_w;
+    a.f_l0 = (float)((-a.mid - 0.5) * s->bin_w);
+    a.f_l1 = (float)((0.5 - a.mid) * s->bin_w);
     a.f_c2 = (float)std::exp2(-2.0 * a.beta2);
-    // centre range: [1, B-2] for the inner window, [0, B-1] for the relative one
-    const int c_lo = rel ? 0 : 1, c_hi = rel ? s->n_bins - 1 : s->n_bins - 2;
-    for (int b = 0; b < kMaxBins; ++b) {
-        auto L = [&](int j) { return (int)std::ceil((j - a.mid - 0.5) * s->bin_w); };
-        a.slide_thr_d1[b] = (b > c_lo && b <= c_hi) ? L(b) : INT_MIN;
-        a.slide_thr_d2[b] = (b >= c_lo && b < c_hi) ? 1 - L(b + 1) : INT_MIN;
-        const double kc = (a.mid - b) * s->bin_w * a.kappa2;
-        a.slide_kc[b] = (float)(rel ? kc - a.beta2 : kc);
-        a.slide_km[b] = (float)(-kc - a.beta2);
-    }
+    a.f_wk_lo = (float)(a.bin_w * a.kappa2 - (double)a.f_wk);
+    a.f_mid_wk_lo = (float)(a.mid * a.bin_w * a.kappa2 - (double)a.f_mid_wk);
     a.g_const = s->g_const;
     a.kn_gamma = (511.0e3 * 1.6e-19) / (9.10938356e-31 * (299792458.0 * 299792458.0));
     a.part_sum = s->d_part_sum;

@@ -437,8 +437,10 @@ __device__ __forceinline__ void fold_path(const ScatterArgs& a, const TofConst<R
 // go either way: both nearest bins are in the window).  Along a half path t
 // only moves one way (s_k never decreases), so the centre only changes once
 // the high word of the fixed-point offset drops below one integer
-// threshold: one integer compare per sample.  The thresholds and ku offsets
-// of every centre are tables in LDS (SlideTables, filled from ScatterArgs).
+// threshold: one integer compare per sample.  A centre's thresholds and ku
+// offset are computed in registers when the centre moves (slide_thr_f,
+// slide_kc_f): no LDS read, so a move waits for nothing (tables in LDS
+// measured 2% slower).
 struct SlideWindow {
     float A, M, P;
     int cur;       // centre bin
@@ -447,12 +449,6 @@ struct SlideWindow {
     float* wp;     // LDS row of bin cur - 1 in the lane's column
 };
 
-struct SlideTables {
-    int thr_d1[kMaxBins];   // path to D1 (t = hi falls): L(b) for b > 1, else INT_MIN
-    int thr_d2[kMaxBins];   // path to D2 (t = -hi rises): 1 - L(b+1) for b < B-2, else INT_MIN
-    float kc[kMaxBins];     // kappa2 (mid - b) W  (kTofSlideRel: minus beta2)
-    float km[kMaxBins];     // kTofSlideRel: -kappa2 (mid - b) W - beta2
-};
 
 // Adds the window, times the path coefficient (ccoef = c coef), into its
 // three LDS rows and clears it.
@@ -464,10 +460,30 @@ __device__ __forceinline__ void slide_flush(SlideWindow& win, float coef, float 
     win.A = win.M = win.P = 0.0f;
 }
 
-template <int SIGN>
-__device__ __forceinline__ int slide_thr(const SlideTables* tb, int b)
+
+// The inner window's thresholds and ku offsets from registers: L(b) =
+// ceil((b - mid - 1/2) W) in float (within a float rounding of a tie the
+// centre may go either way: both nearest bins are in the window), kc(b) =
+// kappa2 (mid - b) W.
+template <int SIGN, bool REL = false>
+__device__ __forceinline__ int slide_thr_f(const ScatterArgs& a, int b)
 {
-    return SIGN < 0 ? tb->thr_d1[b] : tb->thr_d2[b];
+    // centres: [1, B-2] for the inner window, [0, B-1] for the relative one
+    constexpr int kLo = REL ? 0 : 1, kHiOff = REL ? 1 : 2;
+    if constexpr (SIGN < 0) {
+        return b <= kLo ? INT_MIN : (int)__builtin_ceilf(__builtin_fmaf((float)b, a.f_w, a.f_l0));
+    } else {
+        return b >= a.n_bins - kHiOff ? INT_MIN
+                                      : 1 - (int)__builtin_ceilf(__builtin_fmaf((float)b, a.f_w, a.f_l1));
+    }
+}
+
+__device__ __forceinline__ float slide_kc_f(const ScatterArgs& a, int b)
+{
+    // the constants' rounding parts added back: as accurate as rounding the
+    // exact value once
+    const float fb = (float)b;
+    return __builtin_fmaf(-fb, a.f_wk, a.f_mid_wk) + __builtin_fmaf(-fb, a.f_wk_lo, a.f_mid_wk_lo);
 }
 
 // One sample.  ku = kappa2 (t - s_cur), E = 2^ku, D = c E^2 + E + c; the
@@ -478,13 +494,13 @@ __device__ __forceinline__ int slide_thr(const SlideTables* tb, int b)
 // zero terms) flushes the window.
 template <int SIGN>
 __device__ __forceinline__ void slide_sample(const ScatterArgs& a, const TofConst<float>& d,
-                                             const SlideTables* tb, float* col, SlideWindow& win, int64_t x,
+                                             float* col, SlideWindow& win, int64_t x,
                                              int64_t nq, float av, float coef, float ccoef, float skap)
 {
     const int hi = (int)((x + nq) >> 32);
     if (hi < win.thr) {
         int b = win.cur + SIGN;
-        int thr = slide_thr<SIGN>(tb, b);
+        int thr = slide_thr_f<SIGN>(a, b);
         if (hi >= thr) {
             if constexpr (SIGN < 0) {
                 lds_add(win.wp + 2 * kWave, win.P * ccoef);
@@ -502,14 +518,14 @@ __device__ __forceinline__ void slide_sample(const ScatterArgs& a, const TofCons
         } else {
             do {
                 b += SIGN;
-                thr = slide_thr<SIGN>(tb, b);
+                thr = slide_thr_f<SIGN>(a, b);
             } while (hi < thr);
             slide_flush(win, coef, ccoef);
             win.wp = col + (b - 1 + a.slide_row0) * kWave;
         }
         win.cur = b;
         win.thr = thr;
-        win.kc = tb->kc[b];
+        win.kc = slide_kc_f(a, b);
     }
     const float ku = __builtin_fmaf((float)hi, skap, win.kc);
     const float E = fast_exp2(ku);
@@ -524,15 +540,15 @@ __device__ __forceinline__ void slide_sample(const ScatterArgs& a, const TofCons
 // Centre bin of the path's first sample (offset t, high word hi): the float
 // estimate, then the thresholds.
 template <int SIGN>
-__device__ __forceinline__ int slide_first(const ScatterArgs& a, const SlideTables* tb, int hi)
+__device__ __forceinline__ int slide_first(const ScatterArgs& a, int hi)
 {
     const int t = SIGN < 0 ? hi : -hi;
     int b = (int)rintf((float)t * a.f_inv_w + a.f_mid);
     b = min(max(b, 1), a.n_bins - 2);
     // move the centre towards the path's direction of travel while t is past
     // its threshold, and back while the previous centre's threshold holds
-    while (hi < slide_thr<SIGN>(tb, b)) b += SIGN;
-    while (b - SIGN >= 1 && b - SIGN <= a.n_bins - 2 && hi >= slide_thr<SIGN>(tb, b - SIGN)) b -= SIGN;
+    while (hi < slide_thr_f<SIGN>(a, b)) b += SIGN;
+    while (b - SIGN >= 1 && b - SIGN <= a.n_bins - 2 && hi >= slide_thr_f<SIGN>(a, b - SIGN)) b -= SIGN;
     return b;
 }
 
@@ -543,18 +559,18 @@ __device__ __forceinline__ int slide_first(const ScatterArgs& a, const SlideTabl
 // The window is placed at the path's first sample and added into LDS at its
 // end.
 template <int SIGN>
-__device__ __forceinline__ void slide_group(const ScatterArgs& a, const TofConst<float>& d, const SlideTables* tb,
+__device__ __forceinline__ void slide_group(const ScatterArgs& a, const TofConst<float>& d,
                                             float* col, SlideWindow& win, int64_t x, const uint4& q01,
                                             const uint4& q23, const uint4& av, float coef, float ccoef, float skap)
 {
-    slide_sample<SIGN>(a, d, tb, col, win, x, nq_lo(q01), __uint_as_float(av.x), coef, ccoef, skap);
-    slide_sample<SIGN>(a, d, tb, col, win, x, nq_hi(q01), __uint_as_float(av.y), coef, ccoef, skap);
-    slide_sample<SIGN>(a, d, tb, col, win, x, nq_lo(q23), __uint_as_float(av.z), coef, ccoef, skap);
-    slide_sample<SIGN>(a, d, tb, col, win, x, nq_hi(q23), __uint_as_float(av.w), coef, ccoef, skap);
+    slide_sample<SIGN>(a, d, col, win, x, nq_lo(q01), __uint_as_float(av.x), coef, ccoef, skap);
+    slide_sample<SIGN>(a, d, col, win, x, nq_hi(q01), __uint_as_float(av.y), coef, ccoef, skap);
+    slide_sample<SIGN>(a, d, col, win, x, nq_lo(q23), __uint_as_float(av.z), coef, ccoef, skap);
+    slide_sample<SIGN>(a, d, col, win, x, nq_hi(q23), __uint_as_float(av.w), coef, ccoef, skap);
 }
 
 template <int SIGN>
-__device__ __forceinline__ void slide_path(const ScatterArgs& a, const TofConst<float>& d, const SlideTables* tb,
+__device__ __forceinline__ void slide_path(const ScatterArgs& a, const TofConst<float>& d,
                                            float* __restrict__ col, SlideWindow& win, int64_t base, int n,
                                            int64_t shift, float coef)
 {
@@ -567,18 +583,18 @@ __device__ __forceinline__ void slide_path(const ScatterArgs& a, const TofConst<
     uint4 n0 = g[3], n1 = g[4], n2 = g[5];
     {
         const int hi = (int)((x + nq_lo(c0)) >> 32);
-        win.cur = slide_first<SIGN>(a, tb, hi);
-        win.thr = slide_thr<SIGN>(tb, win.cur);
-        win.kc = tb->kc[win.cur];
+        win.cur = slide_first<SIGN>(a, hi);
+        win.thr = slide_thr_f<SIGN>(a, win.cur);
+        win.kc = slide_kc_f(a, win.cur);
         win.wp = col + (win.cur - 1 + a.slide_row0) * kWave;
     }
     for (int k = n;; ) {
-        slide_group<SIGN>(a, d, tb, col, win, x, c0, c1, c2, coef, ccoef, skap);
+        slide_group<SIGN>(a, d, col, win, x, c0, c1, c2, coef, ccoef, skap);
         c0 = g[6]; c1 = g[7]; c2 = g[8];
         k -= kSampleGroup;
         g += 3;
         if (k <= 0) break;
-        slide_group<SIGN>(a, d, tb, col, win, x, n0, n1, n2, coef, ccoef, skap);
+        slide_group<SIGN>(a, d, col, win, x, n0, n1, n2, coef, ccoef, skap);
         n0 = g[6]; n1 = g[7]; n2 = g[8];
         k -= kSampleGroup;
         g += 3;
@@ -588,8 +604,8 @@ __device__ __forceinline__ void slide_path(const ScatterArgs& a, const TofConst<
 }
 
 // Sliding window in relative form (kTofSlideRel, float, h = H in {1, 2}).
-// The centre is the nearest bin, b in [0, B-1] (thresholds as above, with
-// INT_MIN only at the two ends).  Relative to the centre the weights of bins
+// The centre is the nearest bin, b in [0, B-1] (thresholds as above, INT_MIN
+// only at the two ends).  Relative to the centre the weights of bins
 // cur + d are 2^(d ku - d^2 beta2), ku = kappa2 (t - s_cur): for d = +-1
 // 2^(+-ku - beta2), one exp2 each, and for d = +-2 the square of those times
 // 2^(-2 beta2).  Inside the nearest bin |ku| <= beta2, so no term exceeds 1
@@ -610,11 +626,12 @@ struct RelWindow {
 };
 
 template <int H>
-__device__ __forceinline__ void rel_centre(const ScatterArgs& a, const SlideTables* tb, RelWindow<H>& w, int b)
+__device__ __forceinline__ void rel_centre(const ScatterArgs& a, RelWindow<H>& w, int b)
 {
     w.cur = b;
-    w.kp = tb->kc[b];
-    w.km = tb->km[b];
+    const float kc = slide_kc_f(a, b);
+    w.kp = kc - a.f_beta2;
+    w.km = -kc - a.f_beta2;
     w.mp1 = b + 1 < a.n_bins ? 1.0f : 0.0f;
     w.mm1 = b > 0 ? 1.0f : 0.0f;
     if constexpr (H == 2) {
@@ -634,14 +651,14 @@ __device__ __forceinline__ void rel_flush(RelWindow<H>& w, float coef)
 }
 
 template <int H, int SIGN>
-__device__ __forceinline__ void rel_sample(const ScatterArgs& a, const SlideTables* tb, float* col,
+__device__ __forceinline__ void rel_sample(const ScatterArgs& a, float* col,
                                            RelWindow<H>& w, int64_t x, int64_t nq, float av, float coef,
                                            float skap)
 {
     const int hi = (int)((x + nq) >> 32);
     if (hi < w.thr) {
         int b = w.cur + SIGN;
-        int thr = slide_thr<SIGN>(tb, b);
+        int thr = slide_thr_f<SIGN, true>(a, b);
         if (hi >= thr) {
             // a move by one bin: the bin leaving the window goes into LDS
             if constexpr (SIGN < 0) {
@@ -660,13 +677,13 @@ __device__ __forceinline__ void rel_sample(const ScatterArgs& a, const SlideTabl
         } else {
             do {
                 b += SIGN;
-                thr = slide_thr<SIGN>(tb, b);
+                thr = slide_thr_f<SIGN, true>(a, b);
             } while (hi < thr);
             rel_flush(w, coef);
             w.wp = col + (b - H + a.slide_row0) * kWave;
         }
         w.thr = thr;
-        rel_centre(a, tb, w, b);
+        rel_centre(a, w, b);
     }
     const float fh = (float)hi;
     const float tp = fast_exp2(fminf(__builtin_fmaf(fh, skap, w.kp), 30.0f)) * w.mp1;
@@ -691,31 +708,31 @@ __device__ __forceinline__ void rel_sample(const ScatterArgs& a, const SlideTabl
 // Nearest bin of the path's first sample: the float estimate, then the
 // thresholds (INT_MIN at the ends stops both loops).
 template <int SIGN>
-__device__ __forceinline__ int rel_first(const ScatterArgs& a, const SlideTables* tb, int hi)
+__device__ __forceinline__ int rel_first(const ScatterArgs& a, int hi)
 {
     const int t = SIGN < 0 ? hi : -hi;
     int b = (int)rintf((float)t * a.f_inv_w + a.f_mid);
     b = min(max(b, 0), a.n_bins - 1);
-    while (hi < slide_thr<SIGN>(tb, b)) b += SIGN;
-    while (b - SIGN >= 0 && b - SIGN <= a.n_bins - 1 && hi >= slide_thr<SIGN>(tb, b - SIGN)) b -= SIGN;
+    while (hi < slide_thr_f<SIGN, true>(a, b)) b += SIGN;
+    while (b - SIGN >= 0 && b - SIGN <= a.n_bins - 1 && hi >= slide_thr_f<SIGN, true>(a, b - SIGN)) b -= SIGN;
     return b;
 }
 
 template <int H, int SIGN>
-__device__ __forceinline__ void rel_group(const ScatterArgs& a, const SlideTables* tb, float* col,
+__device__ __forceinline__ void rel_group(const ScatterArgs& a, float* col,
                                           RelWindow<H>& w, int64_t x, const uint4& q01, const uint4& q23,
                                           const uint4& av, float coef, float skap)
 {
-    rel_sample<H, SIGN>(a, tb, col, w, x, nq_lo(q01), __uint_as_float(av.x), coef, skap);
-    rel_sample<H, SIGN>(a, tb, col, w, x, nq_hi(q01), __uint_as_float(av.y), coef, skap);
-    rel_sample<H, SIGN>(a, tb, col, w, x, nq_lo(q23), __uint_as_float(av.z), coef, skap);
-    rel_sample<H, SIGN>(a, tb, col, w, x, nq_hi(q23), __uint_as_float(av.w), coef, skap);
+    rel_sample<H, SIGN>(a, col, w, x, nq_lo(q01), __uint_as_float(av.x), coef, skap);
+    rel_sample<H, SIGN>(a, col, w, x, nq_hi(q01), __uint_as_float(av.y), coef, skap);
+    rel_sample<H, SIGN>(a, col, w, x, nq_lo(q23), __uint_as_float(av.z), coef, skap);
+    rel_sample<H, SIGN>(a, col, w, x, nq_hi(q23), __uint_as_float(av.w), coef, skap);
 }
 
 // fold_path for kTofSlideRel: groups of 4 records, two groups in registers
 // (as slide_path).
 template <int H, int SIGN>
-__device__ __forceinline__ void rel_path(const ScatterArgs& a, const SlideTables* tb, float* __restrict__ col,
+__device__ __forceinline__ void rel_path(const ScatterArgs& a, float* __restrict__ col,
                                          RelWindow<H>& w, int64_t base, int n, int64_t shift, float coef)
 {
     const uint4* __restrict__ g = fold_groups(a, base);
@@ -726,20 +743,20 @@ __device__ __forceinline__ void rel_path(const ScatterArgs& a, const SlideTables
     uint4 n0 = g[3], n1 = g[4], n2 = g[5];
     {
         const int hi = (int)((x + nq_lo(c0)) >> 32);
-        const int b = rel_first<SIGN>(a, tb, hi);
-        w.thr = slide_thr<SIGN>(tb, b);
+        const int b = rel_first<SIGN>(a, hi);
+        w.thr = slide_thr_f<SIGN, true>(a, b);
         w.wp = col + (b - H + a.slide_row0) * kWave;
-        rel_centre(a, tb, w, b);
+        rel_centre(a, w, b);
 #pragma unroll
         for (int i = 0; i < 2 * H + 1; ++i) w.A[i] = 0.0f;
     }
     for (int k = n;; ) {
-        rel_group<H, SIGN>(a, tb, col, w, x, c0, c1, c2, coef, skap);
+        rel_group<H, SIGN>(a, col, w, x, c0, c1, c2, coef, skap);
         c0 = g[6]; c1 = g[7]; c2 = g[8];
         k -= kSampleGroup;
         g += 3;
         if (k <= 0) break;
-        rel_group<H, SIGN>(a, tb, col, w, x, n0, n1, n2, coef, skap);
+        rel_group<H, SIGN>(a, col, w, x, n0, n1, n2, coef, skap);
         n0 = g[6]; n1 = g[7]; n2 = g[8];
         k -= kSampleGroup;
         g += 3;
@@ -762,23 +779,11 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
                   "relative sliding window: float, h = 1 or 2, no range test");
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
-    // LDS: the sliding window's tables (shared), then one column per lane;
-    // the one-bin FP32 path: the energy table
+    // LDS: one column per lane; the one-bin FP32 path: the energy table
     constexpr bool ONE_BIN_REC = TRIVIAL && sizeof(R) == 4;
-    SlideTables* tb = reinterpret_cast<SlideTables*>(smem_raw);
     float2* et = reinterpret_cast<float2*>(smem_raw);
     if constexpr (ONE_BIN_REC) {
         for (int i = threadIdx.x; i < kEnergyRows; i += blockDim.x) et[i] = a.etab[i];
-        __syncthreads();   // every wave of the workgroup, before any exits
-    }
-    if constexpr (SLIDE || REL) {
-        const int i = threadIdx.x;
-        if (i < kMaxBins) {
-            tb->thr_d1[i] = a.slide_thr_d1[i];
-            tb->thr_d2[i] = a.slide_thr_d2[i];
-            tb->kc[i] = a.slide_kc[i];
-            tb->km[i] = a.slide_km[i];
-        }
         __syncthreads();   // every wave of the workgroup, before any exits
     }
     // (tile, chunk) of this wave; padding items (tile -1) exit at once
@@ -788,7 +793,7 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
     const int chunk = item.y;
     const int B = a.n_bins;
     const int rows = B + 2 * HM;   // bins -HM .. B-1+HM (the window's reach)
-    R* col = reinterpret_cast<R*>(smem_raw + ((SLIDE || REL) ? sizeof(SlideTables) : 0)) +
+    R* col = reinterpret_cast<R*>(smem_raw) +
              (size_t)wave * rows * kWave + lane;
     if constexpr (!TRIVIAL) {
         for (int r = 0; r < rows; ++r) col[r * kWave] = (R)0;
@@ -871,11 +876,11 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
                 if constexpr (TRIVIAL) {
                     num += p.coef1 * p.act1 + p.coef2 * p.act2;
                 } else if constexpr (SLIDE) {
-                    if (p.n1 > 0) slide_path<-1>(a, tc, tb, col, sl, p.b1, p.n1, p.shift, p.coef1);
-                    if (p.n2 > 0) slide_path<+1>(a, tc, tb, col, sl, p.b2, p.n2, p.shift, p.coef2);
+                    if (p.n1 > 0) slide_path<-1>(a, tc, col, sl, p.b1, p.n1, p.shift, p.coef1);
+                    if (p.n2 > 0) slide_path<+1>(a, tc, col, sl, p.b2, p.n2, p.shift, p.coef2);
                 } else if constexpr (REL) {
-                    if (p.n1 > 0) rel_path<HM, -1>(a, tb, col, rw, p.b1, p.n1, p.shift, p.coef1);
-                    if (p.n2 > 0) rel_path<HM, +1>(a, tb, col, rw, p.b2, p.n2, p.shift, p.coef2);
+                    if (p.n1 > 0) rel_path<HM, -1>(a, col, rw, p.b1, p.n1, p.shift, p.coef1);
+                    if (p.n2 > 0) rel_path<HM, +1>(a, col, rw, p.b2, p.n2, p.shift, p.coef2);
                 } else if (a.t_hi >= a.t_lo) {   // else every TOF weight is zero
                     fold_path<R, HM, TM, RANGE, -1>(a, tc, col, win, p.b1, p.n1, p.shift, p.coef1);
                     fold_path<R, HM, TM, RANGE, +1>(a, tc, col, win, p.b2, p.n2, p.shift, p.coef2);
@@ -1064,7 +1069,6 @@ hipError_t launch_variant(const ScatterArgs& a, hipStream_t st)
     dim3 grid((unsigned)(a.n_items / kScatterWaves));
     size_t lds = TRIVIAL ? (sizeof(R) == 4 ? kEnergyRows * sizeof(float2) : 0)
                          : (size_t)kScatterWaves * (a.n_bins + 2 * HM) * kWave * sizeof(R);
-    if (TM == kTofSlide || TM == kTofSlideRel) lds += sizeof(SlideTables);
     hipLaunchKernelGGL((k_scatter<R, HM, TM, TRIVIAL, RANGE>), grid,
                        dim3(kScatterWaves * kWave), lds, st, a);
     return hipGetLastError();
