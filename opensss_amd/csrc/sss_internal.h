@@ -129,6 +129,9 @@ struct SampleTable {
     double* a64;       // lam * len in float64 (FP64 validation path only, else null)
     uint32_t* fold;    // kFoldGroupWords per group of kSampleGroup records
     double* fold_a64;  // FP64 path only
+    int32_t* fold_src; // per fold record: its sample entry (-1: padding), so a
+                       // new activity image with the same zero terms only
+                       // rewrites the weights
 };
 constexpr int kFoldGroupWords = 3 * kSampleGroup;
 
@@ -204,12 +207,19 @@ hipError_t launch_halfpath_fill(const Grid& g, const double* sp_xyz, int n_sp, c
 // Emission terms lam L of every sample, their sums and the fold-table counts
 // (then the caller scans the padded counts into off_fold), and the copy of
 // the non-zero terms into the fold table.
+hipError_t launch_activity_check(const HalfPathTable t, SampleTable s, int64_t n_hp, const double* lam,
+                                 int32_t* changed, hipStream_t st);
+hipError_t launch_fold_weights(SampleTable s, const int64_t* n_records, int64_t cap, const double* lam,
+                               int32_t* changed, hipStream_t st);
+// run_if: when not null, the kernel does its work only if *run_if != 0 (the
+// fallback of a weights-only refresh)
 hipError_t launch_activity_refresh(const HalfPathTable t, SampleTable s, int64_t n_hp,
-                                   const double* lam, hipStream_t st);
+                                   const double* lam, hipStream_t st, const int32_t* run_if = nullptr);
 hipError_t launch_volume_x_fastest(const double* src, double* dst, int X, int Y, int Z, hipStream_t st);
 hipError_t launch_activity_sum(const HalfPathTable t, SampleTable s, int64_t n_hp,
                                const double* lam, hipStream_t st);
-hipError_t launch_fold_compact(const HalfPathTable t, SampleTable s, int64_t n_hp, hipStream_t st);
+hipError_t launch_fold_compact(const HalfPathTable t, SampleTable s, int64_t n_hp, hipStream_t st,
+                               const int32_t* run_if = nullptr);
 hipError_t launch_pair_half(const HalfPathTable t, int64_t n_hp, bool one_bin, hipStream_t st);
 hipError_t launch_trace_segments(const Grid& g, const double* segs, int64_t n, int cap,
                                  int32_t* counts, double* L, int32_t* idx, double* s,

@@ -340,6 +340,9 @@ struct sss_subject {
     int32_t* d_chunk_slice = nullptr;     // slice of each chunk
     int32_t* d_dets = nullptr;
     int32_t* d_flag = nullptr;
+    int32_t* d_fold_changed = nullptr;    // set when a new activity changes which terms are zero
+    int64_t fold_cap = 0;                 // fold records allocated
+    bool fold_valid = false;              // the fold table's layout matches the current tables
     void* d_scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
     HalfPathTable hp{};
@@ -376,13 +379,13 @@ struct sss_subject {
         sss_ctx* c = ctx;
         pool_free(c, d_upload);   // d_mu (x fastest) .. d_dets live in it
         pool_free(c, d_mu_own);
-        pool_free(c, d_lam); pool_free(c, d_flag); pool_free(c, d_scan_tmp);
+        pool_free(c, d_lam); pool_free(c, d_flag); pool_free(c, d_fold_changed); pool_free(c, d_scan_tmp);
         pool_free(c, hp.off); pool_free(c, hp.n); pool_free(c, hp.off_fold); pool_free(c, hp.n_fold);
         pool_free(c, hp.rec); pool_free(c, hp.blk); pool_free(c, hp.blk_off); pool_free(c, hp.dist);
         pool_free(c, hp.att); pool_free(c, hp.lnatt); pool_free(c, hp.act); pool_free(c, hp.cosabs);
         pool_free(c, hp.status);
         pool_free(c, smp.nq); pool_free(c, smp.vox); pool_free(c, smp.len); pool_free(c, smp.a);
-        pool_free(c, smp.a64); pool_free(c, smp.fold); pool_free(c, smp.fold_a64);
+        pool_free(c, smp.a64); pool_free(c, smp.fold); pool_free(c, smp.fold_a64); pool_free(c, smp.fold_src);
         pool_free(c, d_out); pool_free(c, d_part_sum); pool_free(c, d_part_cnt);
     }
 };
@@ -744,7 +747,7 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
     int64_t& t = s->device_bytes;
     if ((s->z_fastest && (rc = pool_alloc(ctx, &s->d_mu_own, (size_t)s->n_vox, &t))) ||
         (rc = pool_alloc(ctx, &s->d_upload, up_bytes, &t)) || (rc = pool_alloc(ctx, &s->d_lam, (size_t)s->n_vox, &t)) ||
-        (rc = pool_alloc(ctx, &s->d_flag, 1, &t)) ||
+        (rc = pool_alloc(ctx, &s->d_flag, 1, &t)) || (rc = pool_alloc(ctx, &s->d_fold_changed, 1, &t)) ||
         (rc = pool_alloc(ctx, &s->hp.off, (size_t)s->n_hp + 1, &t)) || (rc = pool_alloc(ctx, &s->hp.n, (size_t)s->n_hp + 1, &t)) ||
         (rc = pool_alloc(ctx, &s->hp.off_fold, (size_t)s->n_hp + 1, &t)) ||
         (rc = pool_alloc(ctx, &s->hp.n_fold, (size_t)s->n_hp + 1, &t)) ||
@@ -853,11 +856,25 @@ static int refresh_fold(sss_subject* s)
         SSS_HIP(launch_pair_half(s->hp, s->n_hp, true, st));
         return SSS_OK;
     }
-    SSS_HIP(launch_activity_refresh(s->hp, s->smp, s->n_hp, s->d_lam, st));
+    // When the fold table was built for these walks, the new image most often
+    // has the same zero terms (served requests rescale the activity): then
+    // only its weights are rewritten.  The checks set d_fold_changed on any
+    // difference, and the full refresh and compaction below run only then
+    // (they test the flag on the device: no wait here).
+    const int32_t* run_if = nullptr;
+    if (s->fold_valid && s->smp.fold_src) {
+        SSS_HIP(hipMemsetAsync(s->d_fold_changed, 0, sizeof(int32_t), st));
+        SSS_HIP(launch_activity_check(s->hp, s->smp, s->n_hp, s->d_lam, s->d_fold_changed, st));
+        SSS_HIP(launch_fold_weights(s->smp, s->hp.off_fold + s->n_hp, s->fold_cap, s->d_lam, s->d_fold_changed,
+                                    st));
+        run_if = s->d_fold_changed;
+    }
+    SSS_HIP(launch_activity_refresh(s->hp, s->smp, s->n_hp, s->d_lam, st, run_if));
     const auto padded = rocprim::make_transform_iterator(s->hp.n_fold, PaddedRecords{});
     if (int rc = scan_counts(s, padded, s->hp.off_fold, s->n_hp, st)) return rc;
-    SSS_HIP(launch_fold_compact(s->hp, s->smp, s->n_hp, st));
+    SSS_HIP(launch_fold_compact(s->hp, s->smp, s->n_hp, st, run_if));
     SSS_HIP(launch_pair_half(s->hp, s->n_hp, false, st));
+    s->fold_valid = true;
     return SSS_OK;
 }
 
@@ -900,7 +917,9 @@ int sss_subject_trace_end(sss_subject* s)
         sss_ctx* c = s->ctx;
         pool_free(c, s->smp.nq); pool_free(c, s->smp.vox); pool_free(c, s->smp.len); pool_free(c, s->smp.a);
         pool_free(c, s->smp.a64); pool_free(c, s->smp.fold); pool_free(c, s->smp.fold_a64);
+        pool_free(c, s->smp.fold_src);
         s->samples_cap = 0;
+        s->fold_cap = 0;
         // the fold table holds at most every sample plus the group padding,
         // and spare records after the last one (the fold prefetches ahead)
         size_t fold_cap = (size_t)total + (size_t)(kSampleGroup - 1) * s->n_hp + kSamplePad;
@@ -915,11 +934,16 @@ int sss_subject_trace_end(sss_subject* s)
             (fold && (rc = pool_alloc(c, &s->smp.a, (size_t)total, tally))) ||
             (fold && (rc = pool_alloc(c, &s->smp.fold, fold_cap / kSampleGroup * kFoldGroupWords, tally))) ||
             (fold && fp64 && ((rc = pool_alloc(c, &s->smp.a64, (size_t)total, tally)) ||
-                              (rc = pool_alloc(c, &s->smp.fold_a64, fold_cap, tally)))))
+                              (rc = pool_alloc(c, &s->smp.fold_a64, fold_cap, tally)))) ||
+            // sample entries of the fold records (int32 entries), for
+            // weights-only refreshes
+            (fold && total < INT32_MAX && (rc = pool_alloc(c, &s->smp.fold_src, fold_cap, tally))))
             return rc;
+        s->fold_cap = fold ? (int64_t)fold_cap : 0;
         s->samples_cap = total;
     }
     s->n_samples = total;
+    s->fold_valid = false;   // new walks: the next activity builds the fold table in full
     SSS_HIP(launch_halfpath_fill(s->grid, s->d_sp, s->n_sp, s->d_detpos, s->nru, s->ndu, s->d_mu, s->hp,
                                  s->smp, st));
     // truncated walks are counted (read by sss_subject_info_get), not an error

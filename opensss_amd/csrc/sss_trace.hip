@@ -113,10 +113,18 @@ __global__ void k_halfpath_fill(Grid g, const double* __restrict__ sp_xyz, int n
 // sum do not depend on the order.
 constexpr int kRefreshWaves = 4;
 
-template <bool FOLD>
+// kActSum: ActPath only (one-bin subjects); kActRefresh: also the FP32 (and
+// FP64) terms per sample and the non-zero counts; kActCheck: ActPath and the
+// non-zero counts, compared with the counts the fold table was built from
+// (any difference sets *changed).
+enum ActMode { kActSum, kActRefresh, kActCheck };
+
+template <int MODE>
 __device__ __forceinline__ void activity_block(HalfPathTable t, SampleTable s, int64_t n_hp,
-                                               const double* __restrict__ lam)
+                                               const double* __restrict__ lam, int32_t* changed = nullptr)
 {
+    constexpr bool FOLD = MODE == kActRefresh;
+    constexpr bool COUNT = MODE != kActSum;
     __shared__ double part_act[kRefreshWaves][kWave];
     __shared__ int part_nz[kRefreshWaves][kWave];
     const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
@@ -133,9 +141,9 @@ __device__ __forceinline__ void activity_block(HalfPathTable t, SampleTable s, i
             if constexpr (FOLD) {
                 s.a[i] = (float)ar;
                 if (s.a64) s.a64[i] = ar;
-                // the fold's own precision decides which terms are zero
-                nz += s.a64 ? (ar != 0.0) : ((float)ar != 0.0f);
             }
+            // the fold's own precision decides which terms are zero
+            if constexpr (COUNT) nz += s.fold_a64 ? (ar != 0.0) : ((float)ar != 0.0f);
         }
     }
     part_act[w][l] = act;
@@ -151,14 +159,47 @@ __device__ __forceinline__ void activity_block(HalfPathTable t, SampleTable s, i
         }
         t.act[h] = sum;
         if constexpr (FOLD) t.n_fold[h] = cnt;
+        if constexpr (MODE == kActCheck) {
+            if (cnt != t.n_fold[h]) atomicOr(changed, 1);
+        }
     }
 }
 
 __global__ __launch_bounds__(kRefreshWaves * kWave) void k_activity_refresh(HalfPathTable t, SampleTable s,
                                                                             int64_t n_hp,
-                                                                            const double* __restrict__ lam)
+                                                                            const double* __restrict__ lam,
+                                                                            const int32_t* run_if)
 {
-    activity_block<true>(t, s, n_hp, lam);
+    if (run_if && *run_if == 0) return;   // the same value for every thread
+    activity_block<kActRefresh>(t, s, n_hp, lam);
+}
+
+__global__ __launch_bounds__(kRefreshWaves * kWave) void k_activity_check(HalfPathTable t, SampleTable s,
+                                                                          int64_t n_hp,
+                                                                          const double* __restrict__ lam,
+                                                                          int32_t* changed)
+{
+    activity_block<kActCheck>(t, s, n_hp, lam, changed);
+}
+
+// Weights-only refresh of the fold table (same layout: the same terms are
+// non-zero): record r gets the FP32 (and FP64) term of the sample it was
+// copied from.  A record whose term became zero sets *changed (with the
+// count check in k_activity_check, equal counts and all records non-zero
+// mean the same set of terms).
+__global__ __launch_bounds__(256) void k_fold_weights(SampleTable s, const int64_t* __restrict__ n_records,
+                                                      int64_t cap, const double* __restrict__ lam,
+                                                      int32_t* changed)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= cap || r >= *n_records) return;
+    const int32_t src = s.fold_src[r];
+    if (src < 0) return;   // padding record: weight 0 already
+    const double ar = lam[s.vox[src]] * s.len[src];
+    const bool keep = s.fold_a64 ? ar != 0.0 : (float)ar != 0.0f;
+    if (!keep) atomicOr(changed, 1);
+    s.fold[(r / kSampleGroup) * kFoldGroupWords + 2 * kSampleGroup + r % kSampleGroup] = __float_as_uint((float)ar);
+    if (s.fold_a64) s.fold_a64[r] = ar;
 }
 
 // k_activity_refresh for one-bin subjects, which have no fold table: only
@@ -167,7 +208,7 @@ __global__ __launch_bounds__(kRefreshWaves * kWave) void k_activity_sum(HalfPath
                                                                         int64_t n_hp,
                                                                         const double* __restrict__ lam)
 {
-    activity_block<false>(t, s, n_hp, lam);
+    activity_block<kActSum>(t, s, n_hp, lam);
 }
 
 constexpr int kCompactLanes = 16;   // lanes per half path in k_fold_compact
@@ -194,8 +235,10 @@ __device__ __forceinline__ void fold_store(SampleTable s, int64_t r, int64_t nq,
 constexpr int kXcds = 8;
 constexpr int kCompactParts = kWave * kCompactLanes / 256;
 
-__global__ __launch_bounds__(256) void k_fold_compact(HalfPathTable t, SampleTable s, int64_t n_hp)
+__global__ __launch_bounds__(256) void k_fold_compact(HalfPathTable t, SampleTable s, int64_t n_hp,
+                                                      const int32_t* run_if)
 {
+    if (run_if && *run_if == 0) return;   // the same value for every thread
     const int64_t g = blockIdx.x;
     const int64_t slot = g / kXcds;
     const int64_t path_block = (slot / kCompactParts) * kXcds + g % kXcds;
@@ -228,6 +271,7 @@ __global__ __launch_bounds__(256) void k_fold_compact(HalfPathTable t, SampleTab
             const int at = kept + __popc(gm & ((1u << j) - 1));
             fold_store(s, dst + at, nq, a);
             if (s.fold_a64) s.fold_a64[dst + at] = a64;
+            if (s.fold_src) s.fold_src[dst + at] = (int32_t)(base + (int64_t)k * kWave);
         }
         if (gm) last_nq = __shfl(nq, g0 + 31 - __clz(gm), kWave);
         kept += __popc(gm);
@@ -236,6 +280,7 @@ __global__ __launch_bounds__(256) void k_fold_compact(HalfPathTable t, SampleTab
     if (kept + j < padded) {
         fold_store(s, dst + kept + j, last_nq, 0.0f);
         if (s.fold_a64) s.fold_a64[dst + kept + j] = 0.0;
+        if (s.fold_src) s.fold_src[dst + kept + j] = -1;
     }
 }
 
@@ -305,11 +350,29 @@ hipError_t launch_halfpath_fill(const Grid& g, const double* sp_xyz, int n_sp, c
 }
 
 hipError_t launch_activity_refresh(const HalfPathTable t, SampleTable s, int64_t n_hp,
-                                   const double* lam, hipStream_t st)
+                                   const double* lam, hipStream_t st, const int32_t* run_if)
 {
     if (n_hp == 0) return hipSuccess;
     hipLaunchKernelGGL(k_activity_refresh, dim3(blocks_for(n_hp, kWave)), dim3(kRefreshWaves * kWave), 0, st, t, s,
-                       n_hp, lam);
+                       n_hp, lam, run_if);
+    return hipGetLastError();
+}
+
+hipError_t launch_activity_check(const HalfPathTable t, SampleTable s, int64_t n_hp, const double* lam,
+                                 int32_t* changed, hipStream_t st)
+{
+    if (n_hp == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_activity_check, dim3(blocks_for(n_hp, kWave)), dim3(kRefreshWaves * kWave), 0, st, t, s,
+                       n_hp, lam, changed);
+    return hipGetLastError();
+}
+
+hipError_t launch_fold_weights(SampleTable s, const int64_t* n_records, int64_t cap, const double* lam,
+                               int32_t* changed, hipStream_t st)
+{
+    if (cap == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fold_weights, dim3(blocks_for(cap, 256)), dim3(256), 0, st, s, n_records, cap, lam,
+                       changed);
     return hipGetLastError();
 }
 
@@ -376,13 +439,14 @@ hipError_t launch_activity_sum(const HalfPathTable t, SampleTable s, int64_t n_h
     return hipGetLastError();
 }
 
-hipError_t launch_fold_compact(const HalfPathTable t, SampleTable s, int64_t n_hp, hipStream_t st)
+hipError_t launch_fold_compact(const HalfPathTable t, SampleTable s, int64_t n_hp, hipStream_t st,
+                               const int32_t* run_if)
 {
     if (n_hp == 0) return hipSuccess;
     const int64_t path_blocks = (n_hp + kWave - 1) / kWave;
     const int64_t groups = (path_blocks + kXcds - 1) / kXcds;   // of kXcds path blocks
     hipLaunchKernelGGL(k_fold_compact, dim3((unsigned)(groups * kXcds * kCompactParts)), dim3(256), 0, st,
-                       t, s, n_hp);
+                       t, s, n_hp, run_if);
     return hipGetLastError();
 }
 

@@ -213,6 +213,33 @@ def test_resident_subject_activity_refresh(oracle):
     assert rel <= FP32_TOL and absn <= 1e-6
 
 
+@pytest.mark.parametrize('precision', ['fp32', 'fp64'])
+def test_weights_only_refresh_and_its_fallback(precision):
+    """A resident subject given a sequence of activity images: rescaled ones
+    keep the zero terms (the fold table's weights are rewritten in place),
+    others zero some voxels or bring them back (the table is rebuilt).  Every
+    result equals a fresh subject's, bit for bit."""
+    w = phantoms.small_tof(seed=9, n_sp=30)
+    rng = np.random.default_rng(3)
+    lam = w.activity
+    holes = lam.copy()
+    holes[::3, :, :] = 0.0                      # zero terms appear
+    images = [lam, lam * rng.uniform(0.9, 1.1, size=lam.shape), holes,
+              holes * 1.7, lam * 0.5, lam]
+    s = gpu_subject(w, precision)
+    try:
+        s.run(images[0])
+        for k, img in enumerate(images):
+            if k:
+                s.set_activity(img)
+                s.scatter()
+            got = s.output_compact()
+            want = gpu_compact(w, precision, lam=img)
+            assert np.array_equal(got, want), f'image {k}'
+    finally:
+        s.close()
+
+
 @pytest.mark.parametrize('n_bins', [1, 7])
 def test_volume_upload_orders_agree(n_bins):
     """Volumes sent in C order (transposed on the device, through a 32 x 32
