@@ -124,7 +124,8 @@ constexpr int kSamplePad = 8;              // fold records after the last one (p
 struct SampleTable {
     int64_t* nq;       // -round(2^32 q), q = 2 s_k / c in ps
     int32_t* vox;      // 0-based voxel (reference linear index - 1)
-    double* len;       // intersection length, cm
+    double* len;       // intersection length, cm (FP64 subjects)
+    float* len32;      // the same in FP32 (FP32 subjects, which read 8 bytes a sample instead of 12)
     float* a;          // lam * len (FP32 fold input)
     double* a64;       // lam * len in float64 (FP64 validation path only, else null)
     uint32_t* fold;    // kFoldGroupWords per group of kSampleGroup records

@@ -343,6 +343,7 @@ struct sss_subject {
     int32_t* d_fold_changed = nullptr;    // set when a new activity changes which terms are zero
     int64_t fold_cap = 0;                 // fold records allocated
     bool fold_valid = false;              // the fold table's layout matches the current tables
+    int refreshes = 0;                    // activity images since the last walks
     void* d_scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
     HalfPathTable hp{};
@@ -384,7 +385,7 @@ struct sss_subject {
         pool_free(c, hp.rec); pool_free(c, hp.blk); pool_free(c, hp.blk_off); pool_free(c, hp.dist);
         pool_free(c, hp.att); pool_free(c, hp.lnatt); pool_free(c, hp.act); pool_free(c, hp.cosabs);
         pool_free(c, hp.status);
-        pool_free(c, smp.nq); pool_free(c, smp.vox); pool_free(c, smp.len); pool_free(c, smp.a);
+        pool_free(c, smp.nq); pool_free(c, smp.vox); pool_free(c, smp.len); pool_free(c, smp.len32); pool_free(c, smp.a);
         pool_free(c, smp.a64); pool_free(c, smp.fold); pool_free(c, smp.fold_a64); pool_free(c, smp.fold_src);
         pool_free(c, d_out); pool_free(c, d_part_sum); pool_free(c, d_part_cnt);
     }
@@ -862,19 +863,23 @@ static int refresh_fold(sss_subject* s)
     // difference, and the full refresh and compaction below run only then
     // (they test the flag on the device: no wait here).
     const int32_t* run_if = nullptr;
-    if (s->fold_valid && s->smp.fold_src) {
+    // the record -> sample map costs the compaction 4 bytes a record: it is
+    // kept from the second image after the walks on (a subject that takes
+    // one image per trace, as one reference call does, never needs it)
+    SampleTable smp = s->smp;
+    if (s->refreshes++ == 0) smp.fold_src = nullptr;
+    if (s->fold_valid && smp.fold_src) {
         SSS_HIP(hipMemsetAsync(s->d_fold_changed, 0, sizeof(int32_t), st));
-        SSS_HIP(launch_activity_check(s->hp, s->smp, s->n_hp, s->d_lam, s->d_fold_changed, st));
-        SSS_HIP(launch_fold_weights(s->smp, s->hp.off_fold + s->n_hp, s->fold_cap, s->d_lam, s->d_fold_changed,
-                                    st));
+        SSS_HIP(launch_activity_check(s->hp, smp, s->n_hp, s->d_lam, s->d_fold_changed, st));
+        SSS_HIP(launch_fold_weights(smp, s->hp.off_fold + s->n_hp, s->fold_cap, s->d_lam, s->d_fold_changed, st));
         run_if = s->d_fold_changed;
     }
-    SSS_HIP(launch_activity_refresh(s->hp, s->smp, s->n_hp, s->d_lam, st, run_if));
+    SSS_HIP(launch_activity_refresh(s->hp, smp, s->n_hp, s->d_lam, st, run_if));
     const auto padded = rocprim::make_transform_iterator(s->hp.n_fold, PaddedRecords{});
     if (int rc = scan_counts(s, padded, s->hp.off_fold, s->n_hp, st)) return rc;
-    SSS_HIP(launch_fold_compact(s->hp, s->smp, s->n_hp, st, run_if));
+    SSS_HIP(launch_fold_compact(s->hp, smp, s->n_hp, st, run_if));
     SSS_HIP(launch_pair_half(s->hp, s->n_hp, false, st));
-    s->fold_valid = true;
+    s->fold_valid = smp.fold_src != nullptr;
     return SSS_OK;
 }
 
@@ -915,7 +920,8 @@ int sss_subject_trace_end(sss_subject* s)
     const int64_t total = *s->h_total;
     if (total > s->samples_cap || !s->smp.nq) {
         sss_ctx* c = s->ctx;
-        pool_free(c, s->smp.nq); pool_free(c, s->smp.vox); pool_free(c, s->smp.len); pool_free(c, s->smp.a);
+        pool_free(c, s->smp.nq); pool_free(c, s->smp.vox); pool_free(c, s->smp.len); pool_free(c, s->smp.len32);
+        pool_free(c, s->smp.a);
         pool_free(c, s->smp.a64); pool_free(c, s->smp.fold); pool_free(c, s->smp.fold_a64);
         pool_free(c, s->smp.fold_src);
         s->samples_cap = 0;
@@ -930,7 +936,8 @@ int sss_subject_trace_end(sss_subject* s)
         int rc;
         if ((rc = pool_alloc(c, &s->smp.nq, (size_t)total, tally)) ||
             (rc = pool_alloc(c, &s->smp.vox, (size_t)total, tally)) ||
-            (rc = pool_alloc(c, &s->smp.len, (size_t)total, tally)) ||
+            (fp64 && (rc = pool_alloc(c, &s->smp.len, (size_t)total, tally))) ||
+            (!fp64 && (rc = pool_alloc(c, &s->smp.len32, (size_t)total, tally))) ||
             (fold && (rc = pool_alloc(c, &s->smp.a, (size_t)total, tally))) ||
             (fold && (rc = pool_alloc(c, &s->smp.fold, fold_cap / kSampleGroup * kFoldGroupWords, tally))) ||
             (fold && fp64 && ((rc = pool_alloc(c, &s->smp.a64, (size_t)total, tally)) ||
@@ -944,6 +951,7 @@ int sss_subject_trace_end(sss_subject* s)
     }
     s->n_samples = total;
     s->fold_valid = false;   // new walks: the next activity builds the fold table in full
+    s->refreshes = 0;
     SSS_HIP(launch_halfpath_fill(s->grid, s->d_sp, s->n_sp, s->d_detpos, s->nru, s->ndu, s->d_mu, s->hp,
                                  s->smp, st));
     // truncated walks are counted (read by sss_subject_info_get), not an error

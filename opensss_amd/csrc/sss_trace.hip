@@ -89,7 +89,8 @@ __global__ void k_halfpath_fill(Grid g, const double* __restrict__ sp_xyz, int n
             // q * 2^32 is exact in float64; rint drops bits below 2^-32 ps
             s.nq[i] = -(int64_t)rint(smp * two_over_c * kFixOne);
             s.vox[i] = vox;
-            s.len[i] = L;
+            if (s.len32) s.len32[i] = (float)L;
+            else s.len[i] = L;
             mu_int += mu_prev * len_prev;
             mu_prev = mu[vox];
             len_prev = L;
@@ -112,6 +113,13 @@ __global__ void k_halfpath_fill(Grid g, const double* __restrict__ sp_xyz, int n
 // in wave order.  The terms are non-negative, so the exact-zero tests on the
 // sum do not depend on the order.
 constexpr int kRefreshWaves = 4;
+
+// Intersection length of sample entry i in the precision the subject keeps
+// (the same for every thread: a uniform branch).
+__device__ __forceinline__ double sample_len(const SampleTable& s, int64_t i)
+{
+    return s.len32 ? (double)s.len32[i] : s.len[i];
+}
 
 // kActSum: ActPath only (one-bin subjects); kActRefresh: also the FP32 (and
 // FP64) terms per sample and the non-zero counts; kActCheck: ActPath and the
@@ -136,7 +144,7 @@ __device__ __forceinline__ void activity_block(HalfPathTable t, SampleTable s, i
         const int n = t.n[h];
         for (int k = w; k < n; k += kRefreshWaves) {
             const int64_t i = base + (int64_t)k * kWave;
-            const double ar = lam[s.vox[i]] * s.len[i];
+            const double ar = lam[s.vox[i]] * sample_len(s, i);
             act += ar;
             if constexpr (FOLD) {
                 s.a[i] = (float)ar;
@@ -195,7 +203,7 @@ __global__ __launch_bounds__(256) void k_fold_weights(SampleTable s, const int64
     if (r >= cap || r >= *n_records) return;
     const int32_t src = s.fold_src[r];
     if (src < 0) return;   // padding record: weight 0 already
-    const double ar = lam[s.vox[src]] * s.len[src];
+    const double ar = lam[s.vox[src]] * sample_len(s, src);
     const bool keep = s.fold_a64 ? ar != 0.0 : (float)ar != 0.0f;
     if (!keep) atomicOr(changed, 1);
     s.fold[(r / kSampleGroup) * kFoldGroupWords + 2 * kSampleGroup + r % kSampleGroup] = __float_as_uint((float)ar);
