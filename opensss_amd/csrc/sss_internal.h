@@ -177,9 +177,10 @@ struct ScatterArgs {
     // b + slide_row0; centre thresholds L(b) = ceil((b - mid - 1/2) W) and
     // ku offsets kappa2 (mid - b) W are computed from these float constants
     int slide_row0;
-    float f_beta2, f_c2;         // beta2, 2^(-2 beta2)
+    float f_beta2;               // beta2
     float f_w, f_l0, f_l1;       // W, (-mid - 1/2) W, (1/2 - mid) W: L(b) = ceil(b W + f_l0)
     float f_wk_lo, f_mid_wk_lo;  // W kappa2 - f_wk, mid W kappa2 - f_mid_wk (the rounding parts)
+    float f_c4, f_kclamp;        // 2^(-4 beta2); the h = 2 relative window's ku bound, beta2 + 10
     double g_const;              // (Dx Dy 1e-2)^2 / (4 pi)
     double kn_gamma;
     // output: per chunk, the un-normalised sums and the pair counts of every

@@ -1049,10 +1049,10 @@ static ScatterArgs scatter_args(const sss_subject* s)
         }
     }
     // float subjects whose fold is not the inner sliding window: the relative
-    // sliding window (h = 1 at any beta2; h = 2 while 2^(-2 beta2) is a
+    // sliding window (h = 1 at any beta2; h = 2 while 2^(-4 beta2) is a
     // normal float)
     const bool rel = s->precision != SSS_PRECISION_FP64 && !s->range_check && a.tof_mode != kTofOneExpInner &&
-                     (s->half_window == 1 || (s->half_window == 2 && a.beta2 < 60.0));
+                     (s->half_window == 1 || (s->half_window == 2 && a.beta2 < 30.0));
     if (rel) a.tof_mode = kTofSlideRel;
     a.range_check = s->range_check ? 1 : 0;
     a.f_inv_w = (float)a.inv_w_bins;
@@ -1068,7 +1068,8 @@ static ScatterArgs scatter_args(const sss_subject* s)
     a.f_w = (float)s->bin_w;
     a.f_l0 = (float)((-a.mid - 0.5) * s->bin_w);
     a.f_l1 = (float)((0.5 - a.mid) * s->bin_w);
-    a.f_c2 = (float)std::exp2(-2.0 * a.beta2);
+    a.f_c4 = (float)std::exp2(-4.0 * a.beta2);
+    a.f_kclamp = (float)(a.beta2 + 10.0);
     a.f_wk_lo = (float)(a.bin_w * a.kappa2 - (double)a.f_wk);
     a.f_mid_wk_lo = (float)(a.mid * a.bin_w * a.kappa2 - (double)a.f_mid_wk);
     a.g_const = s->g_const;

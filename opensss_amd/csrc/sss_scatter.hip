@@ -606,12 +606,13 @@ __device__ __forceinline__ void slide_path(const ScatterArgs& a, const TofConst<
 // Sliding window in relative form (kTofSlideRel, float, h = H in {1, 2}).
 // The centre is the nearest bin, b in [0, B-1] (thresholds as above, INT_MIN
 // only at the two ends).  Relative to the centre the weights of bins
-// cur + d are 2^(d ku - d^2 beta2), ku = kappa2 (t - s_cur): for d = +-1
-// 2^(+-ku - beta2), one exp2 each, and for d = +-2 the square of those times
-// 2^(-2 beta2).  Inside the nearest bin |ku| <= beta2, so no term exceeds 1
-// whatever beta2 is (terms that underflow weigh under 2^-126 of the centre);
-// bins past the ends are masked by per-centre factors (their exponents are
-// capped first, so a masked term is 0, never inf * 0).  The window holds
+// cur + d are 2^(d ku - d^2 beta2), ku = kappa2 (t - s_cur).  h = 1: for
+// d = +-1 2^(+-ku - beta2), one exp2 each; inside the nearest bin
+// |ku| <= beta2, so no term exceeds 1 whatever beta2 is (terms that
+// underflow weigh under 2^-126 of the centre), and bins past the ends are
+// masked by per-centre factors (their exponents are capped first, so a
+// masked term is 0, never inf * 0).  h = 2 (beta2 < 30): one exp2, see
+// rel_sample.  The window holds
 // the weights without the path coefficient, which is applied when a bin
 // leaves the window.
 template <int H>
@@ -619,9 +620,13 @@ struct RelWindow {
     float A[2 * H + 1];   // bins cur - H .. cur + H
     int cur;
     int thr;              // the centre changes once hi < thr
+    // H = 1
     float kp, km;         // ku - beta2 = skap hi + kp, -ku - beta2 = -skap hi + km
     float mp1, mm1;       // 1 where bin cur +- 1 exists, else 0
-    float c2p, c2m;       // 2^(-2 beta2) where bin cur +- 2 exists, else 0 (H = 2)
+    // H = 2
+    float kc;             // ku = skap hi + kc
+    float cp1, cm1;       // 2^-beta2 where bin cur +- 1 exists, else 0
+    float c4p, c4m;       // 2^(-4 beta2) where bin cur +- 2 exists, else 0
     float* wp;            // LDS row of bin cur - H
 };
 
@@ -630,13 +635,17 @@ __device__ __forceinline__ void rel_centre(const ScatterArgs& a, RelWindow<H>& w
 {
     w.cur = b;
     const float kc = slide_kc_f(a, b);
-    w.kp = kc - a.f_beta2;
-    w.km = -kc - a.f_beta2;
-    w.mp1 = b + 1 < a.n_bins ? 1.0f : 0.0f;
-    w.mm1 = b > 0 ? 1.0f : 0.0f;
-    if constexpr (H == 2) {
-        w.c2p = b + 2 < a.n_bins ? a.f_c2 : 0.0f;
-        w.c2m = b > 1 ? a.f_c2 : 0.0f;
+    if constexpr (H == 1) {
+        w.kp = kc - a.f_beta2;
+        w.km = -kc - a.f_beta2;
+        w.mp1 = b + 1 < a.n_bins ? 1.0f : 0.0f;
+        w.mm1 = b > 0 ? 1.0f : 0.0f;
+    } else {
+        w.kc = kc;
+        w.cp1 = b + 1 < a.n_bins ? a.f_c : 0.0f;
+        w.cm1 = b > 0 ? a.f_c : 0.0f;
+        w.c4p = b + 2 < a.n_bins ? a.f_c4 : 0.0f;
+        w.c4m = b > 1 ? a.f_c4 : 0.0f;
     }
 }
 
@@ -686,22 +695,33 @@ __device__ __forceinline__ void rel_sample(const ScatterArgs& a, float* col,
         rel_centre(a, w, b);
     }
     const float fh = (float)hi;
-    const float tp = fast_exp2(fminf(__builtin_fmaf(fh, skap, w.kp), 30.0f)) * w.mp1;
-    const float tm = fast_exp2(fminf(__builtin_fmaf(fh, -skap, w.km), 30.0f)) * w.mm1;
-    float D = 1.0f + (tp + tm);
-    float tp2 = 0.0f, tm2 = 0.0f;
-    if constexpr (H == 2) {
-        tp2 = tp * tp * w.c2p;
-        tm2 = tm * tm * w.c2m;
-        D += tp2 + tm2;
-    }
-    const float g = av * fast_rcp(D);
-    w.A[H] += g;
-    w.A[H + 1] = __builtin_fmaf(g, tp, w.A[H + 1]);
-    w.A[H - 1] = __builtin_fmaf(g, tm, w.A[H - 1]);
-    if constexpr (H == 2) {
+    if constexpr (H == 1) {
+        const float tp = fast_exp2(fminf(__builtin_fmaf(fh, skap, w.kp), 30.0f)) * w.mp1;
+        const float tm = fast_exp2(fminf(__builtin_fmaf(fh, -skap, w.km), 30.0f)) * w.mm1;
+        const float D = 1.0f + (tp + tm);
+        const float g = av * fast_rcp(D);
+        w.A[1] += g;
+        w.A[2] = __builtin_fmaf(g, tp, w.A[2]);
+        w.A[0] = __builtin_fmaf(g, tm, w.A[0]);
+    } else {
+        // h = 2 with one exp2: the weights 2^(d ku - d^2 beta2) times E^2,
+        // E = 2^ku: c^4, c E, E^2, c E^3, c^4 E^4 (c = 2^-beta2).  With the
+        // nearest-bin centre |ku| <= beta2, and beta2 < 30 keeps every term
+        // a normal float; past the end bins ku is held to +-(beta2 + 10),
+        // where the far side's terms weigh under 2^-(2 beta2 + 10).
+        const float ku = __builtin_amdgcn_fmed3f(__builtin_fmaf(fh, skap, w.kc), -a.f_kclamp, a.f_kclamp);
+        const float E = fast_exp2(ku);
+        const float E2 = E * E;
+        const float tm1 = w.cm1 * E;
+        const float tp1 = (w.cp1 * E2) * E;
+        const float tp2 = (w.c4p * E2) * E2;
+        const float D = ((w.c4m + tm1) + E2) + (tp1 + tp2);
+        const float g = av * fast_rcp(D);
+        w.A[0] = __builtin_fmaf(g, w.c4m, w.A[0]);
+        w.A[1] = __builtin_fmaf(g, tm1, w.A[1]);
+        w.A[2] = __builtin_fmaf(g, E2, w.A[2]);
+        w.A[3] = __builtin_fmaf(g, tp1, w.A[3]);
         w.A[4] = __builtin_fmaf(g, tp2, w.A[4]);
-        w.A[0] = __builtin_fmaf(g, tm2, w.A[0]);
     }
 }
 
