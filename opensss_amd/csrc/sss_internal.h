@@ -44,6 +44,10 @@ constexpr int kTofSlide = 3;
 // the side weights in relative form, 2^(+-ku - beta2) (two exp2 per sample):
 // no float range limit on beta2 (h = 1) or on the offsets' reach
 constexpr int kTofSlideRel = 4;
+// float only, h = 1 with beta2 above ~95 (bins far wider than the kernel):
+// kTofSlideRel's window with the side weights 2^(+-ku - beta2) by one exp2
+// each (any beta2)
+constexpr int kTofSlideRelWide = 5;
 constexpr int kMaxHalfWindow = 8;
 
 void set_error(const std::string& msg);
@@ -181,6 +185,7 @@ struct ScatterArgs {
     float f_w, f_l0, f_l1;       // W, (-mid - 1/2) W, (1/2 - mid) W: L(b) = ceil(b W + f_l0)
     float f_wk_lo, f_mid_wk_lo;  // W kappa2 - f_wk, mid W kappa2 - f_mid_wk (the rounding parts)
     float f_c4, f_kclamp;        // 2^(-4 beta2); the h = 2 relative window's ku bound, beta2 + 10
+    float f_kclamp1;             // the h = 1 relative window's ku bound, beta2 + 30
     double g_const;              // (Dx Dy 1e-2)^2 / (4 pi)
     double kn_gamma;
     // output: per chunk, the un-normalised sums and the pair counts of every

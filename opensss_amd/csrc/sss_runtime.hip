@@ -1053,7 +1053,9 @@ static ScatterArgs scatter_args(const sss_subject* s)
     // normal float)
     const bool rel = s->precision != SSS_PRECISION_FP64 && !s->range_check && a.tof_mode != kTofOneExpInner &&
                      (s->half_window == 1 || (s->half_window == 2 && a.beta2 < 30.0));
-    if (rel) a.tof_mode = kTofSlideRel;
+    // one exp2 per sample while every term stays a normal float (h = 1:
+    // beta2 + 30 <= 125), else the two-exp2 form
+    if (rel) a.tof_mode = (s->half_window == 1 && a.beta2 > 95.0) ? kTofSlideRelWide : kTofSlideRel;
     a.range_check = s->range_check ? 1 : 0;
     a.f_inv_w = (float)a.inv_w_bins;
     a.f_kap = (float)a.kappa2;
@@ -1070,6 +1072,7 @@ static ScatterArgs scatter_args(const sss_subject* s)
     a.f_l1 = (float)((0.5 - a.mid) * s->bin_w);
     a.f_c4 = (float)std::exp2(-4.0 * a.beta2);
     a.f_kclamp = (float)(a.beta2 + 10.0);
+    a.f_kclamp1 = (float)(a.beta2 + 30.0);
     a.f_wk_lo = (float)(a.bin_w * a.kappa2 - (double)a.f_wk);
     a.f_mid_wk_lo = (float)(a.mid * a.bin_w * a.kappa2 - (double)a.f_mid_wk);
     a.g_const = s->g_const;

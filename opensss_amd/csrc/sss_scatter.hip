@@ -620,22 +620,22 @@ struct RelWindow {
     float A[2 * H + 1];   // bins cur - H .. cur + H
     int cur;
     int thr;              // the centre changes once hi < thr
-    // H = 1
+    // two-exp2 form (h = 1, wide bins)
     float kp, km;         // ku - beta2 = skap hi + kp, -ku - beta2 = -skap hi + km
     float mp1, mm1;       // 1 where bin cur +- 1 exists, else 0
-    // H = 2
+    // one-exp2 forms
     float kc;             // ku = skap hi + kc
     float cp1, cm1;       // 2^-beta2 where bin cur +- 1 exists, else 0
-    float c4p, c4m;       // 2^(-4 beta2) where bin cur +- 2 exists, else 0
+    float c4p, c4m;       // 2^(-4 beta2) where bin cur +- 2 exists, else 0 (h = 2)
     float* wp;            // LDS row of bin cur - H
 };
 
-template <int H>
+template <int H, bool WIDE>
 __device__ __forceinline__ void rel_centre(const ScatterArgs& a, RelWindow<H>& w, int b)
 {
     w.cur = b;
     const float kc = slide_kc_f(a, b);
-    if constexpr (H == 1) {
+    if constexpr (WIDE) {
         w.kp = kc - a.f_beta2;
         w.km = -kc - a.f_beta2;
         w.mp1 = b + 1 < a.n_bins ? 1.0f : 0.0f;
@@ -644,8 +644,10 @@ __device__ __forceinline__ void rel_centre(const ScatterArgs& a, RelWindow<H>& w
         w.kc = kc;
         w.cp1 = b + 1 < a.n_bins ? a.f_c : 0.0f;
         w.cm1 = b > 0 ? a.f_c : 0.0f;
-        w.c4p = b + 2 < a.n_bins ? a.f_c4 : 0.0f;
-        w.c4m = b > 1 ? a.f_c4 : 0.0f;
+        if constexpr (H == 2) {
+            w.c4p = b + 2 < a.n_bins ? a.f_c4 : 0.0f;
+            w.c4m = b > 1 ? a.f_c4 : 0.0f;
+        }
     }
 }
 
@@ -659,7 +661,7 @@ __device__ __forceinline__ void rel_flush(RelWindow<H>& w, float coef)
     }
 }
 
-template <int H, int SIGN>
+template <int H, int SIGN, bool WIDE>
 __device__ __forceinline__ void rel_sample(const ScatterArgs& a, float* col,
                                            RelWindow<H>& w, int64_t x, int64_t nq, float av, float coef,
                                            float skap)
@@ -692,10 +694,10 @@ __device__ __forceinline__ void rel_sample(const ScatterArgs& a, float* col,
             w.wp = col + (b - H + a.slide_row0) * kWave;
         }
         w.thr = thr;
-        rel_centre(a, w, b);
+        rel_centre<H, WIDE>(a, w, b);
     }
     const float fh = (float)hi;
-    if constexpr (H == 1) {
+    if constexpr (WIDE) {
         const float tp = fast_exp2(fminf(__builtin_fmaf(fh, skap, w.kp), 30.0f)) * w.mp1;
         const float tm = fast_exp2(fminf(__builtin_fmaf(fh, -skap, w.km), 30.0f)) * w.mm1;
         const float D = 1.0f + (tp + tm);
@@ -703,6 +705,20 @@ __device__ __forceinline__ void rel_sample(const ScatterArgs& a, float* col,
         w.A[1] += g;
         w.A[2] = __builtin_fmaf(g, tp, w.A[2]);
         w.A[0] = __builtin_fmaf(g, tm, w.A[0]);
+    } else if constexpr (H == 1) {
+        // one exp2: the weights c, E, c E^2 over D = c + E + c E^2 (E = 2^ku,
+        // c = 2^-beta2); with the nearest-bin centre |ku| <= beta2, and
+        // beta2 <= 95 keeps every term a normal float; past the end bins ku
+        // is held to +-(beta2 + 30), where the far side weighs under
+        // 2^-(2 beta2 + 30)
+        const float ku = __builtin_amdgcn_fmed3f(__builtin_fmaf(fh, skap, w.kc), -a.f_kclamp1, a.f_kclamp1);
+        const float E = fast_exp2(ku);
+        const float tp = (w.cp1 * E) * E;
+        const float D = (w.cm1 + E) + tp;
+        const float g = av * fast_rcp(D);
+        w.A[0] = __builtin_fmaf(g, w.cm1, w.A[0]);
+        w.A[1] = __builtin_fmaf(g, E, w.A[1]);
+        w.A[2] = __builtin_fmaf(g, tp, w.A[2]);
     } else {
         // h = 2 with one exp2: the weights 2^(d ku - d^2 beta2) times E^2,
         // E = 2^ku: c^4, c E, E^2, c E^3, c^4 E^4 (c = 2^-beta2).  With the
@@ -738,20 +754,20 @@ __device__ __forceinline__ int rel_first(const ScatterArgs& a, int hi)
     return b;
 }
 
-template <int H, int SIGN>
+template <int H, int SIGN, bool WIDE>
 __device__ __forceinline__ void rel_group(const ScatterArgs& a, float* col,
                                           RelWindow<H>& w, int64_t x, const uint4& q01, const uint4& q23,
                                           const uint4& av, float coef, float skap)
 {
-    rel_sample<H, SIGN>(a, col, w, x, nq_lo(q01), __uint_as_float(av.x), coef, skap);
-    rel_sample<H, SIGN>(a, col, w, x, nq_hi(q01), __uint_as_float(av.y), coef, skap);
-    rel_sample<H, SIGN>(a, col, w, x, nq_lo(q23), __uint_as_float(av.z), coef, skap);
-    rel_sample<H, SIGN>(a, col, w, x, nq_hi(q23), __uint_as_float(av.w), coef, skap);
+    rel_sample<H, SIGN, WIDE>(a, col, w, x, nq_lo(q01), __uint_as_float(av.x), coef, skap);
+    rel_sample<H, SIGN, WIDE>(a, col, w, x, nq_hi(q01), __uint_as_float(av.y), coef, skap);
+    rel_sample<H, SIGN, WIDE>(a, col, w, x, nq_lo(q23), __uint_as_float(av.z), coef, skap);
+    rel_sample<H, SIGN, WIDE>(a, col, w, x, nq_hi(q23), __uint_as_float(av.w), coef, skap);
 }
 
 // fold_path for kTofSlideRel: groups of 4 records, two groups in registers
 // (as slide_path).
-template <int H, int SIGN>
+template <int H, int SIGN, bool WIDE>
 __device__ __forceinline__ void rel_path(const ScatterArgs& a, float* __restrict__ col,
                                          RelWindow<H>& w, int64_t base, int n, int64_t shift, float coef)
 {
@@ -766,17 +782,17 @@ __device__ __forceinline__ void rel_path(const ScatterArgs& a, float* __restrict
         const int b = rel_first<SIGN>(a, hi);
         w.thr = slide_thr_f<SIGN, true>(a, b);
         w.wp = col + (b - H + a.slide_row0) * kWave;
-        rel_centre(a, w, b);
+        rel_centre<H, WIDE>(a, w, b);
 #pragma unroll
         for (int i = 0; i < 2 * H + 1; ++i) w.A[i] = 0.0f;
     }
     for (int k = n;; ) {
-        rel_group<H, SIGN>(a, col, w, x, c0, c1, c2, coef, skap);
+        rel_group<H, SIGN, WIDE>(a, col, w, x, c0, c1, c2, coef, skap);
         c0 = g[6]; c1 = g[7]; c2 = g[8];
         k -= kSampleGroup;
         g += 3;
         if (k <= 0) break;
-        rel_group<H, SIGN>(a, col, w, x, n0, n1, n2, coef, skap);
+        rel_group<H, SIGN, WIDE>(a, col, w, x, n0, n1, n2, coef, skap);
         n0 = g[6]; n1 = g[7]; n2 = g[8];
         k -= kSampleGroup;
         g += 3;
@@ -793,10 +809,11 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
 {
     extern __shared__ unsigned char smem_raw[];
     constexpr bool SLIDE = TM == kTofSlide;
-    constexpr bool REL = TM == kTofSlideRel;
+    constexpr bool REL = TM == kTofSlideRel || TM == kTofSlideRelWide;
+    constexpr bool WIDE = TM == kTofSlideRelWide;
     static_assert(!SLIDE || (sizeof(R) == 4 && HM == 1 && !RANGE), "sliding window: float, h = 1, no range test");
-    static_assert(!REL || (sizeof(R) == 4 && (HM == 1 || HM == 2) && !RANGE),
-                  "relative sliding window: float, h = 1 or 2, no range test");
+    static_assert(!REL || (sizeof(R) == 4 && (HM == 1 || HM == 2) && !RANGE && (!WIDE || HM == 1)),
+                  "relative sliding window: float, h = 1 or 2 (wide: h = 1), no range test");
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
     // LDS: one column per lane; the one-bin FP32 path: the energy table
@@ -899,8 +916,8 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
                     if (p.n1 > 0) slide_path<-1>(a, tc, col, sl, p.b1, p.n1, p.shift, p.coef1);
                     if (p.n2 > 0) slide_path<+1>(a, tc, col, sl, p.b2, p.n2, p.shift, p.coef2);
                 } else if constexpr (REL) {
-                    if (p.n1 > 0) rel_path<HM, -1>(a, col, rw, p.b1, p.n1, p.shift, p.coef1);
-                    if (p.n2 > 0) rel_path<HM, +1>(a, col, rw, p.b2, p.n2, p.shift, p.coef2);
+                    if (p.n1 > 0) rel_path<HM, -1, WIDE>(a, col, rw, p.b1, p.n1, p.shift, p.coef1);
+                    if (p.n2 > 0) rel_path<HM, +1, WIDE>(a, col, rw, p.b2, p.n2, p.shift, p.coef2);
                 } else if (a.t_hi >= a.t_lo) {   // else every TOF weight is zero
                     fold_path<R, HM, TM, RANGE, -1>(a, tc, col, win, p.b1, p.n1, p.shift, p.coef1);
                     fold_path<R, HM, TM, RANGE, +1>(a, tc, col, win, p.b2, p.n2, p.shift, p.coef2);
@@ -1104,6 +1121,8 @@ hipError_t launch_hm(const ScatterArgs& a, hipStream_t st)
             if (h == 1) return launch_variant<R, 1, kTofSlideRel, false, false>(a, st);
             if (h == 2) return launch_variant<R, 2, kTofSlideRel, false, false>(a, st);
         }
+        if (a.tof_mode == kTofSlideRelWide && h == 1)
+            return launch_variant<R, 1, kTofSlideRelWide, false, false>(a, st);
     }
     if (h == 1) {
         if constexpr (!RANGE) {

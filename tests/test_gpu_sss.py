@@ -116,10 +116,12 @@ TOF_VARIANTS = {
     'h1-14bins-even': ((14, 214.0, 3335.64), 3, 0),
     'h2-21bins': ((21, 214.0, 3335.64), 4, 0),
     'h2-17bins': ((17, 214.0, 3335.64), 4, 0),
-    'h1-6bins-wide': ((6, 214.0, 3335.64), 4, 0),
-    'h1-2bins': ((2, 214.0, 3335.64), 4, 0),
+    'h1-8bins': ((8, 214.0, 3335.64), 4, 0),
+    'h1-7bins': ((7, 214.0, 3335.64), 4, 0),
+    'h1-6bins-wide': ((6, 214.0, 3335.64), 5, 0),
+    'h1-2bins': ((2, 214.0, 3335.64), 5, 0),
     'h0-1bin-zero-columns': ((1, 214.0, 3335.64), 0, 0),
-    'h1-3bins-narrow': ((3, 214.0, 3335.64), 4, 0),
+    'h1-3bins-narrow': ((3, 214.0, 3335.64), 5, 0),
     'h1-3bins-masked': ((3, 600.0, 2417.0), 4, 1),
 }
 
@@ -128,11 +130,12 @@ TOF_VARIANTS = {
 def test_tof_kernel_variants_vs_oracle(oracle, case):
     """15 and 14 bins at 214 ps: one exp2 per sample with a sliding window (odd
     and even bin counts, bin centres at integer / half-integer multiples of W);
-    21 and 17 bins: h = 2 (FP32: relative sliding window); 6, 3 and 2 bins at
-    214 ps: bins so wide (beta2 108 to 3e3) that only the relative window's
-    float range holds; one bin: zero TOF columns beyond +-3508 ps; 3 bins of
-    1611 ps at 600 ps: offsets far beyond the outer bins, so the window keeps
-    masked end bins."""
+    21 and 17 bins: h = 2 (FP32: relative sliding window, one exp2); 8 and 7
+    bins: wide bins (beta2 61, 79), the relative window's one-exp2 form; 6, 3
+    and 2 bins at 214 ps: bins so wide (beta2 108 to 3e3) that only its
+    two-exp2 form keeps to the float range; one bin: zero TOF columns beyond
+    +-3508 ps; 3 bins of 1611 ps at 600 ps: offsets far beyond the outer
+    bins, so the window keeps masked end bins."""
     import dataclasses
     (n_bins, res, rng), fold32, fold64 = TOF_VARIANTS[case]
     w = phantoms.small_tof(seed=12, n_bins=n_bins, n_sp=30)
