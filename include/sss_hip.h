@@ -106,8 +106,9 @@ typedef struct {
                                     window centre inside the bins, 3 as 2 with a window that
                                     slides along each half path (FP32), 4 a sliding window
                                     on the nearest bin, one exp2 per sample (FP32, h = 1 or
-                                    2), 5 as 4 with the side weights by two exp2 (FP32,
-                                    h = 1, bins far wider than the kernel) */
+                                    2), 5 as 4 keeping only the side weight that the
+                                    offset points to (FP32, h = 1, bins far wider than the
+                                    kernel) */
 } sss_subject_info;
 
 const char* sss_last_error(void);

@@ -44,9 +44,9 @@ constexpr int kTofSlide = 3;
 // the side weights in relative form, 2^(+-ku - beta2) (two exp2 per sample):
 // no float range limit on beta2 (h = 1) or on the offsets' reach
 constexpr int kTofSlideRel = 4;
-// float only, h = 1 with beta2 above ~95 (bins far wider than the kernel):
-// kTofSlideRel's window with the side weights 2^(+-ku - beta2) by one exp2
-// each (any beta2)
+// float only, h = 1 with beta2 above 95 (bins far wider than the kernel):
+// kTofSlideRel's window keeping only the side weight that ku points to,
+// 2^(|ku| - beta2) (the other is below 2^-95 of the centre)
 constexpr int kTofSlideRelWide = 5;
 constexpr int kMaxHalfWindow = 8;
 

@@ -606,13 +606,11 @@ __device__ __forceinline__ void slide_path(const ScatterArgs& a, const TofConst<
 // Sliding window in relative form (kTofSlideRel, float, h = H in {1, 2}).
 // The centre is the nearest bin, b in [0, B-1] (thresholds as above, INT_MIN
 // only at the two ends).  Relative to the centre the weights of bins
-// cur + d are 2^(d ku - d^2 beta2), ku = kappa2 (t - s_cur).  h = 1: for
-// d = +-1 2^(+-ku - beta2), one exp2 each; inside the nearest bin
-// |ku| <= beta2, so no term exceeds 1 whatever beta2 is (terms that
-// underflow weigh under 2^-126 of the centre), and bins past the ends are
-// masked by per-centre factors (their exponents are capped first, so a
-// masked term is 0, never inf * 0).  h = 2 (beta2 < 30): one exp2, see
-// rel_sample.  The window holds
+// cur + d are 2^(d ku - d^2 beta2), ku = kappa2 (t - s_cur); bins past the
+// ends are masked by per-centre factors.  One exp2 per sample in each form
+// (see rel_sample): h = 1 with beta2 <= 95 and h = 2 with beta2 < 30 scale
+// the weights to keep every term a normal float; h = 1 with wider bins
+// keeps only the side that ku points to.  The window holds
 // the weights without the path coefficient, which is applied when a bin
 // leaves the window.
 template <int H>
@@ -620,11 +618,8 @@ struct RelWindow {
     float A[2 * H + 1];   // bins cur - H .. cur + H
     int cur;
     int thr;              // the centre changes once hi < thr
-    // two-exp2 form (h = 1, wide bins)
-    float kp, km;         // ku - beta2 = skap hi + kp, -ku - beta2 = -skap hi + km
-    float mp1, mm1;       // 1 where bin cur +- 1 exists, else 0
-    // one-exp2 forms
     float kc;             // ku = skap hi + kc
+    float mp1, mm1;       // 1 where bin cur +- 1 exists, else 0 (h = 1, wide bins)
     float cp1, cm1;       // 2^-beta2 where bin cur +- 1 exists, else 0
     float c4p, c4m;       // 2^(-4 beta2) where bin cur +- 2 exists, else 0 (h = 2)
     float* wp;            // LDS row of bin cur - H
@@ -636,8 +631,7 @@ __device__ __forceinline__ void rel_centre(const ScatterArgs& a, RelWindow<H>& w
     w.cur = b;
     const float kc = slide_kc_f(a, b);
     if constexpr (WIDE) {
-        w.kp = kc - a.f_beta2;
-        w.km = -kc - a.f_beta2;
+        w.kc = kc;
         w.mp1 = b + 1 < a.n_bins ? 1.0f : 0.0f;
         w.mm1 = b > 0 ? 1.0f : 0.0f;
     } else {
@@ -698,13 +692,18 @@ __device__ __forceinline__ void rel_sample(const ScatterArgs& a, float* col,
     }
     const float fh = (float)hi;
     if constexpr (WIDE) {
-        const float tp = fast_exp2(fminf(__builtin_fmaf(fh, skap, w.kp), 30.0f)) * w.mp1;
-        const float tm = fast_exp2(fminf(__builtin_fmaf(fh, -skap, w.km), 30.0f)) * w.mm1;
-        const float D = 1.0f + (tp + tm);
-        const float g = av * fast_rcp(D);
+        // beta2 > 95: the two side weights multiply to 2^(-2 beta2), so only
+        // the side ku points to can matter; the other weighs under
+        // 2^-beta2 < 2^-95 of the centre and is dropped.  One exp2:
+        // 2^(|ku| - beta2), capped before the end-bin mask.
+        const float ku = __builtin_fmaf(fh, skap, w.kc);
+        const bool up = ku > 0.0f;
+        const float t = fast_exp2(fminf(__builtin_fabsf(ku) - a.f_beta2, 30.0f)) * (up ? w.mp1 : w.mm1);
+        const float g = av * fast_rcp(1.0f + t);
+        const float gt = g * t;
         w.A[1] += g;
-        w.A[2] = __builtin_fmaf(g, tp, w.A[2]);
-        w.A[0] = __builtin_fmaf(g, tm, w.A[0]);
+        w.A[2] += up ? gt : 0.0f;
+        w.A[0] += up ? 0.0f : gt;
     } else if constexpr (H == 1) {
         // one exp2: the weights c, E, c E^2 over D = c + E + c E^2 (E = 2^ku,
         // c = 2^-beta2); with the nearest-bin centre |ku| <= beta2, and
