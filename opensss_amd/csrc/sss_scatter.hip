@@ -146,17 +146,53 @@ __device__ __forceinline__ bool pair_eval(const ScatterArgs& a, int sp, int d1, 
     return true;
 }
 
-// Energy row of a pair (law of cosines at the scatter point, FP64 as the
-// reference rounds it); carg, s1q, s2q are kept for the float terms.
+// Energy row of a pair (law of cosines at the scatter point, FP64 in the
+// reference's operation order, no fused multiply-add); carg, s1q, s2q are
+// kept for the float terms.
 __device__ __forceinline__ int energy_row(const ScatterArgs& a, double dist1, double dist2, double lor2,
                                           double& carg, double& s1q, double& s2q)
 {
+#pragma clang fp contract(off)
     s1q = dist1 * dist1;
     s2q = dist2 * dist2;
     carg = (s1q + s2q - lor2) / (2.0 * sqrt(s1q * s2q));
     carg = carg < -1.0 ? -1.0 : (carg > 1.0 ? 1.0 : carg);
     const double e_kev = floor(511.0 / (2.0 + carg) + 0.5);
     const int erow = (int)(e_kev * 2.0) - 1;
+    return erow < 0 ? 0 : (erow >= a.n_energy ? a.n_energy - 1 : erow);
+}
+
+// energy_row from a quick float64 estimate: each quotient is a product with
+// v_rcp_f64 refined by two Newton steps, and the root sqrt(s1q s2q) is
+// d1 d2, so 511/(2 + c) + 1/2 comes out within about 1e-12 of energy_row's
+// value.  Where its fraction lies more than kErowMargin from an integer both
+// floor to the same energy; the other lanes (near ties, a zero or non-finite
+// distance) run energy_row's sequence.  The cosine kept for the float terms
+// is the estimate's (a few float64 ulps from energy_row's).
+constexpr double kErowMargin = 1e-9;
+
+__device__ __forceinline__ double rcp_refined(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+    return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+}
+
+__device__ __forceinline__ int energy_row_fast(const ScatterArgs& a, double dist1, double dist2, double lor2,
+                                               double& carg, double& s1q, double& s2q)
+{
+    const double den = (2.0 * dist1) * dist2;
+    const double num = (dist1 * dist1 + dist2 * dist2) - lor2;
+    const double c = __builtin_fmin(__builtin_fmax(num * rcp_refined(den), -1.0), 1.0);
+    const double v = 511.0 * rcp_refined(2.0 + c) + 0.5;
+    const double k = __builtin_floor(v);
+    const double f = v - k;
+    if (!(den > 0.0 && den < 1e300 && f > kErowMargin && f < 1.0 - kErowMargin))
+        return energy_row(a, dist1, dist2, lor2, carg, s1q, s2q);
+    carg = c;
+    s1q = dist1 * dist1;
+    s2q = dist2 * dist2;
+    const int erow = (int)k * 2 - 1;
     return erow < 0 ? 0 : (erow >= a.n_energy ? a.n_energy - 1 : erow);
 }
 
@@ -194,7 +230,7 @@ __device__ __forceinline__ bool pair_eval_rec(const ScatterArgs& a, int sp, int 
     const int32_t f = r1.n_flags | r2.n_flags;
     if ((f & kHalfAttZero) || !(f & kHalfActive)) return false;
     double carg, s1q, s2q;
-    const int erow = energy_row(a, r1.dist, r2.dist, lor2, carg, s1q, s2q);
+    const int erow = energy_row_fast(a, r1.dist, r2.dist, lor2, carg, s1q, s2q);
     const double eff = a.eff[erow] * eff_ref;
     if (eff == 0.0) return false;
     pair_coef(a, r1, r2, carg, s1q, s2q, (float)eff, a.ratio[erow], p);
@@ -224,7 +260,7 @@ __device__ __forceinline__ bool pair_eval_one_bin(const ScatterArgs& a, const fl
     const int32_t f = r1.n_flags | r2.n_flags;
     if ((f & kHalfAttZero) || !(f & kHalfActive)) return false;
     double carg, s1q, s2q;
-    const int erow = energy_row(a, r1.dist, r2.dist, lor2, carg, s1q, s2q);
+    const int erow = energy_row_fast(a, r1.dist, r2.dist, lor2, carg, s1q, s2q);
     const float2 e = et[energy_slot(a, erow)];
     if (e.x < 0.0f) return false;
     pair_coef(a, r1, r2, carg, s1q, s2q, e.x, e.y, p);
