@@ -849,7 +849,9 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
     static_assert(!SLIDE || (sizeof(R) == 4 && HM == 1 && !RANGE), "sliding window: float, h = 1, no range test");
     static_assert(!REL || (sizeof(R) == 4 && (HM == 1 || HM == 2) && !RANGE && (!WIDE || HM == 1)),
                   "relative sliding window: float, h = 1 or 2 (wide: h = 1), no range test");
-    const int wave = threadIdx.x / kWave;
+    // the wave index is uniform: held in a scalar register, so the work item,
+    // chunk and scatter-point loop below are scalar too
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int lane = threadIdx.x % kWave;
     // LDS: one column per lane; the one-bin FP32 path: the energy table
     constexpr bool ONE_BIN_REC = TRIVIAL && sizeof(R) == 4;
@@ -904,15 +906,21 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
         // take turns (unrolled by three, so no register copy waits for a
         // load), and the energy terms come from LDS (no global load queues
         // behind the records).  Clamped detector indices keep the loads of
-        // invalid lanes in bounds, and indices past the chunk read its last
-        // point again.
-        const int64_t per_sp = (int64_t)a.nru * a.ndu;
-        const PairHalf* rec = a.hp_rec;
+        // invalid lanes in bounds, and loads past the chunk read its last
+        // point again.  The points are loaded in order, so the record row is
+        // a scalar pointer stepped once per load (the lanes add their
+        // detectors' 32-bit byte offsets).
+        const int64_t row_bytes = (int64_t)a.nru * a.ndu * (int64_t)sizeof(PairHalf);
         const int last = p_end - 1;
-        auto load = [&](int sp, PairHalf& r1, PairHalf& r2) {
-            const int64_t o = (int64_t)min(sp, last) * per_sp;
-            r1 = rec[o + d1];
-            r2 = rec[o + d2];
+        const unsigned char* row = reinterpret_cast<const unsigned char*>(a.hp_rec) + p_begin * row_bytes;
+        int row_sp = p_begin;
+        const uint32_t o1 = (uint32_t)d1 * (uint32_t)sizeof(PairHalf);
+        const uint32_t o2 = (uint32_t)d2 * (uint32_t)sizeof(PairHalf);
+        auto load = [&](int, PairHalf& r1, PairHalf& r2) {
+            r1 = *reinterpret_cast<const PairHalf*>(row + o1);
+            r2 = *reinterpret_cast<const PairHalf*>(row + o2);
+            if (row_sp < last) row += row_bytes;
+            ++row_sp;
         };
         auto eval = [&](int sp, const PairHalf& r1, const PairHalf& r2) {
             Pair<float> p;
