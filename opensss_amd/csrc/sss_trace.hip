@@ -142,9 +142,23 @@ __device__ __forceinline__ void activity_block(HalfPathTable t, SampleTable s, i
     if (h < n_hp) {
         const int64_t base = t.off[h];
         const int n = t.n[h];
+        // the voxel index and length of the next step are loaded before
+        // this step's gather, so the two loads of a step do not wait in turn
+        int vox_next = 0;
+        double len_next = 0;
+        if (w < n) {
+            vox_next = s.vox[base + (int64_t)w * kWave];
+            len_next = sample_len(s, base + (int64_t)w * kWave);
+        }
         for (int k = w; k < n; k += kRefreshWaves) {
             const int64_t i = base + (int64_t)k * kWave;
-            const double ar = lam[s.vox[i]] * sample_len(s, i);
+            const int vox = vox_next;
+            const double len = len_next;
+            if (k + kRefreshWaves < n) {
+                vox_next = s.vox[i + kRefreshWaves * kWave];
+                len_next = sample_len(s, i + kRefreshWaves * kWave);
+            }
+            const double ar = lam[vox] * len;
             act += ar;
             if constexpr (FOLD) {
                 s.a[i] = (float)ar;
