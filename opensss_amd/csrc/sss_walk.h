@@ -104,6 +104,9 @@ template <typename F>
 __device__ __forceinline__ int walk_run(const Grid& g, Walk w, F&& f, bool* truncated = nullptr)
 {
     int cur = voxel_index(g, w.X, w.Y, w.Z);
+    // index steps of a move along x, y, z (voxel_index is linear in X, Y, Z
+    // and exact in float64, so the running index equals it at every voxel)
+    const int dx_idx = (int)w.xStep, dy_idx = (int)w.yStep * g.ny, dz_idx = (int)w.zStep * g.ny * g.nx;
     double px = w.xVec * w.tMin, py = w.yVec * w.tMin, pz = w.zVec * w.tMin;
     double prev = 0;
     int k = 0;
@@ -122,10 +125,16 @@ __device__ __forceinline__ int walk_run(const Grid& g, Walk w, F&& f, bool* trun
             if (truncated) *truncated = true;
             break;
         }
-        double len;
-        if (axis == 0) { w.X = nX; len = w.xTM; w.xTM += w.xTD; }
-        else if (axis == 1) { w.Y = nY; len = w.yTM; w.yTM += w.yTD; }
-        else { w.Z = nZ; len = w.zTM; w.zTM += w.zTD; }
+        // selects rather than branches: lanes of a wave step along
+        // different axes, and every value is the one the branch would give
+        const double len = axis == 0 ? w.xTM : (axis == 1 ? w.yTM : w.zTM);
+        w.xTM = axis == 0 ? w.xTM + w.xTD : w.xTM;
+        w.yTM = axis == 1 ? w.yTM + w.yTD : w.yTM;
+        w.zTM = axis == 2 ? w.zTM + w.zTD : w.zTM;
+        w.X = nX;
+        w.Y = nY;
+        w.Z = nZ;
+        const int next = cur + (axis == 0 ? dx_idx : (axis == 1 ? dy_idx : dz_idx));
         const double dx = px - w.xVec * len, dy = py - w.yVec * len, dz = pz - w.zVec * len;
         const double L = sqrt(dx * dx + dy * dy + dz * dz);
         const double s = prev + L / 2;
@@ -135,7 +144,7 @@ __device__ __forceinline__ int walk_run(const Grid& g, Walk w, F&& f, bool* trun
         py = w.yVec * len;
         pz = w.zVec * len;
         ++k;
-        cur = voxel_index(g, w.X, w.Y, w.Z);
+        cur = next;
     }
     const double dx = px - w.xVec * w.tMax, dy = py - w.yVec * w.tMax, dz = pz - w.zVec * w.tMax;
     const double L = sqrt(dx * dx + dy * dy + dz * dz);
