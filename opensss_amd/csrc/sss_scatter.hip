@@ -164,8 +164,8 @@ __device__ __forceinline__ int energy_row(const ScatterArgs& a, double dist1, do
 
 // energy_row from a quick float64 estimate: each quotient is a product with
 // v_rcp_f64 refined by two Newton steps, and the root sqrt(s1q s2q) is
-// d1 d2, so 511/(2 + c) + 1/2 comes out within about 1e-12 of energy_row's
-// value.  Where its fraction lies more than kErowMargin from an integer both
+// d1 d2 (the numerator is energy_row's, bit for bit), so 511/(2 + c) + 1/2
+// comes out within about 1e-12 of energy_row's value.  Where its fraction lies more than kErowMargin from an integer both
 // floor to the same energy; the other lanes (near ties, a zero or non-finite
 // distance) run energy_row's sequence.  The cosine kept for the float terms
 // is the estimate's (a few float64 ulps from energy_row's).
@@ -181,6 +181,8 @@ __device__ __forceinline__ double rcp_refined(double x)
 __device__ __forceinline__ int energy_row_fast(const ScatterArgs& a, double dist1, double dist2, double lor2,
                                                double& carg, double& s1q, double& s2q)
 {
+    // the numerator rounds as energy_row's (no fused multiply-add)
+#pragma clang fp contract(off)
     const double den = (2.0 * dist1) * dist2;
     const double num = (dist1 * dist1 + dist2 * dist2) - lor2;
     const double c = __builtin_fmin(__builtin_fmax(num * rcp_refined(den), -1.0), 1.0);
