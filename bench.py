@@ -259,6 +259,24 @@ def rel_err_stats(got, want, frac=0.01):
             'max_abs_err_over_peak': float(np.abs(got - want).max() / peak)}
 
 
+def cpu_model():
+    try:
+        with open('/proc/cpuinfo') as fh:
+            for line in fh:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_threads():
+    """Host threads of the CPU baseline: the process's CPU share as the
+    environment states it (OMP_NUM_THREADS), else every CPU."""
+    env = os.environ.get('OMP_NUM_THREADS', '')
+    return int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
+
+
 def cpu_baseline(w, threads, target_s=12.0):
     """Oracle (float64 C restatement) on the first k scatter points of each of
     the first `threads` z slices (OpenMP runs one slice per thread); k is
@@ -268,7 +286,8 @@ def cpu_baseline(w, threads, target_s=12.0):
     the sample's points, to check the GPU path against."""
     from oracle import oracle as ora
     from opensss_amd.tables import detectors_used, group_by_slice, rings_used
-    ora.build()
+    # -O3 -march=native, built here for this host's CPU
+    ora.use_native_build()
     pts, ptr, _ = group_by_slice(w.scatter_points)
     n_sl = min(threads, len(ptr) - 1)
     water = np.load(os.path.join(ROOT, 'opensss_amd', 'data', 'attenuation_water.npy'))
@@ -300,8 +319,10 @@ def cpu_baseline(w, threads, target_s=12.0):
     rad = w.lor_coordinates[dets[i1], dets[i2], 1].astype(np.int64)
     compact = out[:, :, cell_of[rad * (N // 2) + ang]]
     base = {'value': evals / dt, 'unit': 'evals/s', 'cores': threads, 'kind': 'port',
+            'host_cpus': os.cpu_count(), 'cpu_model': cpu_model(),
             'sample': f'{n} scatter points (the first {per_slice} of each of {n_sl} z slices) of {w.name}, '
-                      f'{evals:.3e} evals in {dt:.2f} s, float64 C oracle, OpenMP over slices'}
+                      f'{evals:.3e} evals in {dt:.2f} s, float64 C oracle (-O3 -march=native), '
+                      f'OpenMP over slices'}
     return base, sample, compact
 
 
@@ -355,9 +376,8 @@ def main():
     flops = 2.0 * w.n_bins * samples + 64.0 * pairs
     achieved = flops / (k_ms * 1e-3) / 1e12
     traffic, traffic_src = profile_traffic(w.name)
-    # FP32 compute bound: MI355X's dense FP32 peak (157.3 TFLOP/s) is the same
-    # for its matrix and vector pipes; this kernel issues on the vector pipe
-    roofline = {'bound': 'mfma', 'pipe': 'valu_fp32', 'achieved': achieved,
+    # FP32 vector (VALU) bound, priced against MI355X's 157.3 TFLOP/s FP32 peak
+    roofline = {'bound': 'valu', 'pipe': 'valu_fp32', 'achieved': achieved,
                 'peak': FP32_VALU_PEAK_TFLOPS, 'unit': 'TFLOP/s',
                 'frac': achieved / FP32_VALU_PEAK_TFLOPS, 'traffic': traffic,
                 'traffic_source': traffic_src, 'kernel': 'k_scatter', 'kernel_ms': k_ms,
@@ -378,7 +398,7 @@ def main():
     cpu = None
     err_oracle = None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or cpu_threads()
         cpu, sample, want = cpu_baseline(w, threads)
         print(f"[bench] cpu baseline {cpu['value']:.3e} evals/s", file=sys.stderr, flush=True)
         # the FP32 GPU path on the same scatter points, checked against the oracle

@@ -58,10 +58,19 @@ def run_batch(instances, attenuation_table, n_streams=4, precision='fp32', conte
 
     def advance():
         k, s = queued.popleft()
-        s.trace_end()
-        s.scatter()
-        addr = s.ctx.host_alloc(s.output_nbytes)
-        s.output_compact_async(addr)
+        try:
+            s.trace_end()
+            s.scatter()
+            addr = s.ctx.host_alloc(s.output_nbytes)
+        except BaseException:
+            s.close()
+            raise
+        try:
+            s.output_compact_async(addr)
+        except BaseException:
+            s.ctx.host_free(addr)
+            s.close()
+            raise
         running.append((k, s, addr))
 
     try:
@@ -81,8 +90,17 @@ def run_batch(instances, attenuation_table, n_streams=4, precision='fp32', conte
             k, s, addr = running.popleft()
             outs[k] = _collect(s, addr)
     finally:
-        for item in list(queued) + list(running):
-            item[1].close()
+        # on an error, instances still queued or running are released: a
+        # running one's download targets its pinned block, so it is waited
+        # for before the block goes back to the context
+        for _, s in queued:
+            s.close()
+        for _, s, addr in running:
+            try:
+                s.wait()
+            finally:
+                s.ctx.host_free(addr)
+                s.close()
         if own:
             for c in ctxs:
                 c.close()

@@ -91,6 +91,22 @@ __device__ __forceinline__ LaneCell lane_cell(const ScatterArgs& a, int4 tl, int
     return c;
 }
 
+// Energy row of a pair (law of cosines at the scatter point, FP64 in the
+// reference's operation order, no fused multiply-add); carg, s1q, s2q are
+// kept for the float terms.
+__device__ __forceinline__ int energy_row(const ScatterArgs& a, double dist1, double dist2, double lor2,
+                                          double& carg, double& s1q, double& s2q)
+{
+#pragma clang fp contract(off)
+    s1q = dist1 * dist1;
+    s2q = dist2 * dist2;
+    carg = (s1q + s2q - lor2) / (2.0 * sqrt(s1q * s2q));
+    carg = carg < -1.0 ? -1.0 : (carg > 1.0 ? 1.0 : carg);
+    const double e_kev = floor(511.0 / (2.0 + carg) + 0.5);
+    const int erow = (int)(e_kev * 2.0) - 1;
+    return erow < 0 ? 0 : (erow >= a.n_energy ? a.n_energy - 1 : erow);
+}
+
 // Geometry and coefficients of one (scatter point, cell) pair.
 template <typename R>
 struct Pair {
@@ -116,12 +132,9 @@ __device__ __forceinline__ bool pair_eval(const ScatterArgs& a, int sp, int d1, 
     if (att1 == 0.0 || att2 == 0.0 || (act1 == 0.0 && act2 == 0.0)) return false;
     const double dist1 = a.hp_dist[h1], dist2 = a.hp_dist[h2];
     // law of cosines at the scatter point, energy of the scattered photon
-    const double s1q = dist1 * dist1, s2q = dist2 * dist2;
-    double carg = (s1q + s2q - lor2) / (2.0 * sqrt(s1q * s2q));
-    carg = carg < -1.0 ? -1.0 : (carg > 1.0 ? 1.0 : carg);
-    const double e_kev = floor(511.0 / (2.0 + carg) + 0.5);
-    int erow = (int)(e_kev * 2.0) - 1;
-    erow = erow < 0 ? 0 : (erow >= a.n_energy ? a.n_energy - 1 : erow);
+    // (the reference's operation order, no fused multiply-add)
+    double carg, s1q, s2q;
+    const int erow = energy_row(a, dist1, dist2, lor2, carg, s1q, s2q);
     const double eff = a.eff[erow] * eff_ref;
     if (eff == 0.0) return false;
     const R ratio = (R)a.ratio[erow];
@@ -144,22 +157,6 @@ __device__ __forceinline__ bool pair_eval(const ScatterArgs& a, int sp, int d1, 
     p.act2 = (R)act2;
     p.counts = (act1 != 0.0 && act2 != 0.0) ? 2.0f : 1.0f;
     return true;
-}
-
-// Energy row of a pair (law of cosines at the scatter point, FP64 in the
-// reference's operation order, no fused multiply-add); carg, s1q, s2q are
-// kept for the float terms.
-__device__ __forceinline__ int energy_row(const ScatterArgs& a, double dist1, double dist2, double lor2,
-                                          double& carg, double& s1q, double& s2q)
-{
-#pragma clang fp contract(off)
-    s1q = dist1 * dist1;
-    s2q = dist2 * dist2;
-    carg = (s1q + s2q - lor2) / (2.0 * sqrt(s1q * s2q));
-    carg = carg < -1.0 ? -1.0 : (carg > 1.0 ? 1.0 : carg);
-    const double e_kev = floor(511.0 / (2.0 + carg) + 0.5);
-    const int erow = (int)(e_kev * 2.0) - 1;
-    return erow < 0 ? 0 : (erow >= a.n_energy ? a.n_energy - 1 : erow);
 }
 
 // energy_row from a quick float64 estimate: each quotient is a product with

@@ -166,7 +166,13 @@ class ScatterServer:
             if len(self.inflight) >= self.max_inflight:
                 break
             req = self.queue.popleft()
-            s = None if req.key in self.inflight else self.load(req.key)
+            try:
+                s = None if req.key in self.inflight else self.load(req.key)
+            except BaseException:
+                # the request goes back to the front of the queue, for the
+                # caller to retry or drop
+                self.queue.appendleft(req)
+                raise
             if s is None:
                 self.queue.append(req)
                 continue

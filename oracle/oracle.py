@@ -36,6 +36,18 @@ def build(force: bool = False) -> str:
     return _LIB_PATH
 
 
+def use_native_build() -> str:
+    """Build the oracle at -O3 -march=native on this machine (always afresh:
+    a copy built elsewhere may target another CPU) and load that build from
+    now on.  Used by the timed CPU baseline only; the parity tests use the
+    portable build."""
+    global _LIB_PATH, _lib
+    subprocess.run(['make', '-C', _HERE, '-B', 'native'], check=True, stdout=subprocess.DEVNULL)
+    _LIB_PATH = os.path.join(_HERE, 'build', 'liboracle_native.so')
+    _lib = None
+    return _LIB_PATH
+
+
 def lib():
     global _lib
     if _lib is None:

@@ -84,3 +84,31 @@ def test_run_batch_order_contexts_and_residency(monkeypatch):
     assert Subject.contexts == [ctxs[i % 3] for i in range(11)]
     assert Subject.alive == 0 and Subject.peak <= 2 * len(ctxs) + 1, Subject.peak
     assert all(not c.blocks for c in ctxs)
+
+
+class FailingSubject(Subject):
+    """Fails in scatter for the instance whose mu value is 5."""
+
+    def scatter(self):
+        if self.key == 5.0:
+            raise RuntimeError('planted failure')
+        super().scatter()
+
+
+def test_run_batch_error_releases_pinned_blocks(monkeypatch):
+    """An error inside the pipeline closes every subject still queued or
+    running and hands their pinned output blocks back to their contexts."""
+    import pytest
+    monkeypatch.setattr(batch, 'ScatterSubject', FailingSubject)
+    Subject.alive = Subject.peak = 0
+    Subject.contexts = []
+    insts = [SimpleNamespace(attenuation=np.full((2, 2, 2), float(i + 1)), activity=np.full((2, 2, 2), 1.0),
+                             image_size=None, geometry=None, normals=None, det_size=None,
+                             energy_resolution=0.1, energy_threshold=435.0, n_rings_used=1, n_det_used=2,
+                             tof_resolution=214.0, tof_range=3335.64, n_bins=1, scatter_points=None)
+             for i in range(9)]
+    ctxs = [Ctx(), Ctx()]
+    with pytest.raises(RuntimeError, match='planted'):
+        batch.run_batch(insts, None, contexts=ctxs)
+    assert Subject.alive == 0
+    assert all(not c.blocks for c in ctxs)

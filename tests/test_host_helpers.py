@@ -67,6 +67,58 @@ def test_span_lut_full_mrd_plane_count_and_symmetry():
     assert planes.shape[0] == int(np.count_nonzero(counts))
 
 
+def _michelogram_planes(nr, span, mrd):
+    """Span plane of every ring pair (0-based), built as the usual
+    Michelogram order and nothing else: segment 0 holds |r2 - r1| < (span+1)/2,
+    segment s >= 1 holds |r2 - r1| in [s span - (span-1)/2, s span + (span-1)/2];
+    segment 0 first, then per segment its r2 < r1 block before its r2 > r1
+    block; inside a block one plane per ring sum r1 + r2, in increasing sum.
+    Ring pairs beyond mrd have no plane (-1)."""
+    half = (span + 1) // 2
+    lut = np.full((nr, nr), -1, dtype=np.int64)
+    blocks = [(0, 0)]                  # (segment, sign) in plane order
+    s = 1
+    while s * span - (span - 1) // 2 <= nr - 1:
+        blocks += [(s, -1), (s, +1)]
+        s += 1
+    base = 0
+    for seg, sign in blocks:
+        dmin = 0 if seg == 0 else seg * span - (span - 1) // 2
+        dmax = half - 1 if seg == 0 else seg * span + (span - 1) // 2
+        lo = dmin if seg else 0
+        for r1 in range(nr):
+            for r2 in range(nr):
+                d = r2 - r1
+                if mrd is not None and abs(d) > mrd:
+                    continue
+                if not dmin <= abs(d) <= dmax or (seg and (d > 0) != (sign > 0)):
+                    continue
+                lut[r1, r2] = base + (r1 + r2 - lo)
+        base += 2 * (nr - lo) - 1     # distinct ring sums of the block
+    return lut
+
+
+@pytest.mark.parametrize('nr,span,mrd', [(16, 3, None), (16, 5, None), (12, 3, 7), (24, 5, 13),
+                                         (9, 3, None), (80, 19, 9)])
+def test_span_lut_matches_michelogram_order(nr, span, mrd):
+    """Every ring pair's plane, in every segment, against an independent
+    enumeration of the segment/block order; the averaged rings are the means
+    of each plane's ring pairs."""
+    planes, _, lut, counts = dr.SinogramLUT(nr, span, mrd)
+    want = _michelogram_planes(nr, span, mrd)
+    assert np.array_equal(lut.astype(np.int64), want)
+    # the averaged rings list only the planes some ring pair reaches (an mrd
+    # inside a segment leaves every other ring sum of its last block empty),
+    # while the LUT keeps the full numbering, as the reference's does
+    full = np.nonzero(counts > 0)[0]
+    assert planes.shape[0] == len(full)
+    for k, p in enumerate(full):
+        r1, r2 = np.nonzero(want == p)
+        assert counts[p] == len(r1) > 0
+        assert abs(planes[k, 0] - r2.mean()) < 1e-5 and abs(planes[k, 1] - r1.mean()) < 1e-5
+    assert int(counts.sum()) == int(np.count_nonzero(want >= 0))
+
+
 def test_sum_lor_counts():
     geom, _ = geometry.cylinder_geometry(4, 8, 10.0, 1.0)
     _, sino, _ = sinogram.SinogramCoordinates(8, 1, 1, 1, 1, 4)

@@ -7,8 +7,18 @@
   region is a box of whole voxels (a LOR is a tail iff it misses the box);
 * tail fit: prompts = k S + R recovers k; the sensitivity image counts the
   LORs through each voxel (brute force over the oracle walk).
+
+The span branches (InterpolateAllBinsSpan, MaskGeneratorMashing,
+BackProjectionScatterRandFitMashing) and the activity-variant mask are pinned
+the same way: the span plane's source rings by a brute-force nearest-ring
+search with ties to the lower ring; the span mask against the box hull on
+the extended geometry's rings int(2 m); the activity mask against the
+emission hull (with the reference's discarded threshold); the span fit by k
+recovery, brute-force sensitivity and the conservation of each LOR's value
+under value * L / sum(L).
 """
 import numpy as np
+import pytest
 
 from opensss_amd import phantoms
 from opensss_amd.tables import detectors_used, rings_used
@@ -130,3 +140,221 @@ def test_tail_fit_sensitivity_counts_lors(oracle):
                     _, idx, _ = oracle.trace(grid, w.image_size, seg)
                     count[idx - 1] += 1
     assert np.array_equal(imgs['sens'], count.astype(np.float32))
+
+
+# --- span branches (InterpolateAllBinsSpan, MaskGeneratorMashing,
+# --- BackProjectionScatterRandFitMashing) and the activity-variant mask ------
+
+def _nearest_index_lower_tie(grid, q):
+    """Index of the grid value nearest to q, ties to the lower index (brute force)."""
+    d = np.abs(np.asarray(grid, dtype=np.float64) - float(q))
+    return int(np.flatnonzero(d == d.min())[0])
+
+
+def _span_case():
+    from opensss_amd.data_reduction import SinogramLUT
+    from opensss_amd.geometry import expandRings
+    w = _case()
+    NR = w.geometry.shape[0]
+    msi = SinogramLUT(NR, 3, 4)[0]
+    geom_ext, _ = expandRings(w.geometry, w.normals)
+    return w, msi, geom_ext
+
+
+def test_span_interpolation_takes_the_nearest_sampled_rings(oracle):
+    """Span plane p reads the sampled plane of the sampled rings nearest to
+    its averaged ring pair (ring 2 from column 0, ring 1 from column 1, ties
+    to the lower ring), found here by brute force; sampled detector pairs
+    come back unchanged and every other cell holds the value of a nearest
+    sampled (radial, angular) point."""
+    w, msi, _ = _span_case()
+    NR, N = w.geometry.shape[0], w.geometry.shape[1]
+    rings = rings_used(NR, w.n_rings_used)
+    dets = detectors_used(N, w.n_det_used)
+    lor = w.lor_coordinates
+    nru = w.n_rings_used
+    rng = np.random.default_rng(12)
+    i1, i2 = np.triu_indices(len(dets))
+    vals = rng.uniform(0.1, 2.0, size=(2, len(i1), nru * nru)).astype(np.float32)
+    for r in range(nru):                      # same-ring D1 == D2 cells are never computed
+        vals[:, i1 == i2, r + r * nru] = 0
+    rad_s, ang_s = lor[dets[i1], dets[i2], 1], lor[dets[i1], dets[i2], 0]
+    dense = np.zeros((2, N + 1, N // 2, nru * nru), dtype=np.float32)
+    dense[:, rad_s, ang_s, :] = vals
+    out = oracle.interpolate_bins(dense, rings, dets, lor, MashedSinogramIndices=msi)
+    assert out.shape == (2, N + 1, N // 2, msi.shape[0])
+    pts = np.stack([rad_s, ang_s], axis=1).astype(np.float64)
+    gr, ga = np.mgrid[0:N + 1, 0:N // 2]
+    d2 = (gr.reshape(-1, 1) - pts[:, 0]) ** 2 + (ga.reshape(-1, 1) - pts[:, 1]) ** 2
+    nearest_sets = [np.flatnonzero(row == row.min()) for row in d2]
+    swapped = 0
+    for p in range(msi.shape[0]):
+        a = _nearest_index_lower_tie(rings, msi[p, 0])
+        b = _nearest_index_lower_tie(rings, msi[p, 1])
+        src = vals[:, :, a + b * nru]
+        assert np.array_equal(out[:, rad_s, ang_s, p], src), p
+        flat = out[:, :, :, p].reshape(2, -1)
+        for c, cand in enumerate(nearest_sets):
+            assert any(np.array_equal(flat[:, c], src[:, k]) for k in cand), (p, c)
+        swapped += int(a != b)
+    assert swapped > 0                        # planes off the ring diagonal are exercised
+
+
+def test_span_tail_mask_is_the_box_hull(oracle):
+    """Span planes are traced between rings int(2 m2) and int(2 m1) of the
+    extended geometry (m: the plane's averaged rings); with mu > 0 on a box of
+    whole voxels a LOR is a tail iff it misses the box."""
+    w, msi, geom = _span_case()
+    N = geom.shape[1]
+    X, Y, Z = w.attenuation.shape
+    mu = np.zeros((X, Y, Z))
+    mu[9:16, 5:14, 1:5] = 0.096                # off-centre in z, so ring order matters
+    b = np.asarray(w.image_size, dtype=np.float64)
+    vox = (b[3:] - b[:3]) / np.array([X, Y, Z])
+    lo = b[:3] + vox * np.array([9, 5, 1])
+    hi = b[:3] + vox * np.array([16, 14, 5])
+    mask = oracle.tail_mask(np.zeros_like(mu), mu, w.image_size, geom, w.lor_coordinates, True, 2,
+                            MashedSinogramIndices=msi)
+    lor = w.lor_coordinates
+    assert mask.shape == (N + 1, N // 2, msi.shape[0])
+    seen = np.zeros_like(mask, dtype=bool)
+    for p in range(msi.shape[0]):
+        ra, rb = int(np.float32(msi[p, 0]) * np.float32(2)), int(np.float32(msi[p, 1]) * np.float32(2))
+        for d1 in range(0, N, 2):
+            for d2 in range(d1, N, 2):
+                if d1 == d2 and ra == rb:
+                    continue
+                seg = [geom[ra, d1, 0], geom[ra, d1, 1], geom[ra, 0, 2],
+                       geom[rb, d2, 0], geom[rb, d2, 1], geom[rb, 0, 2]]
+                want = 0 if _segment_hits_box(seg, lo, hi) else 1
+                assert mask[lor[d1, d2, 1], lor[d1, d2, 0], p] == want, (p, d1, d2)
+                seen[lor[d1, d2, 1], lor[d1, d2, 0], p] = True
+    assert not mask[~seen].any()               # cells no LOR reaches stay 0
+    assert mask[seen].min() == 0 and mask[seen].max() == 1
+
+
+@pytest.mark.parametrize('span', [False, True])
+def test_activity_mask_is_the_emission_hull(oracle, span):
+    """UseAttenuation=False: a LOR is a tail iff its activity integral is not
+    positive.  The reference's activity threshold (< 1e-5 -> 0) is discarded
+    before use, so LORs through a faint box (activity 1e-6) are not tails."""
+    w, msi, geom_ext = _span_case()
+    geom = geom_ext if span else w.geometry
+    NR, N = geom.shape[0], geom.shape[1]
+    X, Y, Z = w.attenuation.shape
+    lam = np.zeros((X, Y, Z))
+    lam[4:9, 12:18, 3:8] = 1.0
+    lam[14:20, 3:8, 6:10] = 1e-6               # below the discarded threshold
+    b = np.asarray(w.image_size, dtype=np.float64)
+    vox = (b[3:] - b[:3]) / np.array([X, Y, Z])
+    boxes = [(b[:3] + vox * np.array([4, 12, 3]), b[:3] + vox * np.array([9, 18, 8])),
+             (b[:3] + vox * np.array([14, 3, 6]), b[:3] + vox * np.array([20, 8, 10]))]
+    mu = np.full((X, Y, Z), 0.096)             # would make every LOR a non-tail if it were used
+    if span:
+        mask = oracle.tail_mask(lam, mu, w.image_size, geom, w.lor_coordinates, False, 3,
+                                MashedSinogramIndices=msi)
+        planes = [(int(np.float32(msi[p, 0]) * 2), int(np.float32(msi[p, 1]) * 2), p)
+                  for p in range(msi.shape[0])]
+    else:
+        mask = oracle.tail_mask(lam, mu, w.image_size, geom, w.lor_coordinates, False, 3,
+                                SinogramIndex=w.sinogram_index)
+        planes = [(r1, r2, w.sinogram_index[r1, r2]) for r1 in range(NR) for r2 in range(NR)]
+    lor = w.lor_coordinates
+    faint_only = 0
+    for ra, rb, p in planes:
+        for d1 in range(0, N, 3):
+            for d2 in range(d1, N, 3):
+                if d1 == d2 and ra == rb:
+                    continue
+                seg = [geom[ra, d1, 0], geom[ra, d1, 1], geom[ra, 0, 2],
+                       geom[rb, d2, 0], geom[rb, d2, 1], geom[rb, 0, 2]]
+                hits = [_segment_hits_box(seg, lo, hi) for lo, hi in boxes]
+                want = 0 if any(hits) else 1
+                faint_only += int(hits[1] and not hits[0])
+                assert mask[lor[d1, d2, 1], lor[d1, d2, 0], p] == want, (ra, rb, d1, d2)
+    assert faint_only > 0
+
+
+def _span_fit_inputs(w, msi, seed):
+    rng = np.random.default_rng(seed)
+    N = w.geometry.shape[1]
+    shape = (N + 1, N // 2, msi.shape[0])
+    S = rng.uniform(0.5, 2.0, size=shape).astype(np.float32)
+    R = rng.uniform(0.0, 0.3, size=shape).astype(np.float32)
+    M = (rng.uniform(size=shape) < 0.6).astype(np.float32)
+    return S, R, M
+
+
+def test_span_tail_fit_recovers_k(oracle):
+    w, msi, geom = _span_case()
+    S, R, M = _span_fit_inputs(w, msi, 3)
+    k = 0.8125
+    P = (np.float64(k) * S + R).astype(np.float32)
+    scale, imgs = oracle.tail_fit(P, S, R, M, np.array([12, 12, 6]), w.image_size, geom,
+                                  w.lor_coordinates, 2, MashedSinogramIndices=msi)
+    assert abs(scale[0] - k) <= 2e-5 * k
+    assert imgs['sens'].max() > 0
+
+
+def test_span_tail_fit_sensitivity_and_sums_by_brute_force(oracle):
+    """Span: LOR (d1 of ring int(2 m2), d2 of ring int(2 m1)) of plane p
+    reads its values from plane p; each kept LOR adds 1 to the sensitivity of
+    every voxel it crosses and value * L / sum(L) to the images, so an
+    image's total is the sum of the kept LORs' values."""
+    w, msi, geom = _span_case()
+    N = geom.shape[1]
+    S, R, M = _span_fit_inputs(w, msi, 4)
+    P = (1.5 * S + R).astype(np.float32)
+    grid = np.array([12, 12, 6])
+    _, imgs = oracle.tail_fit(P, S, R, M, grid, w.image_size, geom, w.lor_coordinates, 3,
+                              MashedSinogramIndices=msi)
+    lor = w.lor_coordinates
+    count = np.zeros(int(np.prod(grid)))
+    total_s = total_p = 0.0
+    Sm, Pm = M * S, M * P
+    for p in range(msi.shape[0]):
+        ra, rb = int(np.float32(msi[p, 0]) * 2), int(np.float32(msi[p, 1]) * 2)
+        for d1 in range(0, N, 3):
+            for d2 in range(d1, N, 3):
+                if d1 == d2 and ra == rb:
+                    continue
+                cell = (lor[d1, d2, 1], lor[d1, d2, 0], p)
+                if Sm[cell] == 0:
+                    continue
+                seg = np.array([geom[ra, d1, 0], geom[ra, d1, 1], geom[ra, 0, 2],
+                                geom[rb, d2, 0], geom[rb, d2, 1], geom[rb, 0, 2]])
+                L, idx, _ = oracle.trace(grid, w.image_size, seg)
+                if len(L) == 0:
+                    continue
+                count[idx - 1] += 1
+                total_s += float(Sm[cell])
+                total_p += float(Pm[cell])
+    assert np.array_equal(imgs['sens'], count.astype(np.float32))
+    assert abs(float(imgs['scatter'].astype(np.float64).sum()) - total_s) <= 1e-5 * total_s
+    assert abs(float(imgs['prompt'].astype(np.float64).sum()) - total_p) <= 1e-5 * total_p
+
+
+@pytest.mark.parametrize('mode', [0, 1])
+def test_backproject_conserves_each_lors_value(oracle, mode):
+    """ora_backproject on single segments: mode 0 spreads value / n over the
+    n voxels, mode 1 value * L / sum(L); either way the image sums to the
+    value, and mode 1's voxel shares are proportional to the lengths."""
+    grid = np.array([10, 10, 8])
+    bounds = np.array([-5.0, -5.0, -4.0, 5.0, 5.0, 4.0])
+    rng = np.random.default_rng(mode + 30)
+    for _ in range(20):
+        seg = np.concatenate([rng.uniform(-6, 6, 2), rng.uniform(-5, 5, 1),
+                              rng.uniform(-6, 6, 2), rng.uniform(-5, 5, 1)])
+        L, idx, _ = oracle.trace(grid, bounds, seg)
+        if len(L) < 3:
+            continue
+        v = np.float32(rng.uniform(1, 10))
+        imgs = oracle.backproject(grid, bounds, seg[None, :], [v], [v], [v], mode)
+        img = imgs[1]
+        assert abs(float(img.astype(np.float64).sum()) - float(v)) <= 1e-5 * float(v)
+        share = img[idx - 1].astype(np.float64)
+        if mode == 0:
+            assert np.allclose(share, float(v) / len(L), rtol=1e-6)
+        else:
+            assert np.allclose(share, float(v) * L / L.sum(), rtol=1e-6, atol=1e-7 * float(v))
+        assert np.array_equal(imgs[3][idx - 1], np.ones(len(L), np.float32))

@@ -202,3 +202,19 @@ def test_autotune_keeps_the_fastest_tile_shape(fake):
     for r in reqs:
         assert np.all(r.output == np.float32((r.key + 1) * float(r.activity.sum())))
     server.close()
+
+
+def test_request_stays_queued_when_its_subject_cannot_load(fake):
+    """A load that raises (the subject alone exceeds the budget) leaves the
+    request at the front of the queue instead of dropping it."""
+    subs = {0: _instance(0), 1: _instance(1)}
+    FakeSubject.sizes = {}
+    server = serve.ScatterServer(subs, None, contexts=fake, hbm_budget=10)
+    reqs = _requests([1, 0])
+    for r in reqs:
+        server.submit(r)
+    with pytest.raises(MemoryError):
+        server.admit(0.0)
+    assert [r.tag for r in server.queue] == [0, 1]
+    server.queue.clear()
+    server.close()
