@@ -243,10 +243,22 @@ hipError_t launch_interp(const float* sampled, int n_bins, int n_sampled_planes,
 hipError_t launch_tail_mask(const Grid& g, const double* image, int use_attenuation, const double* geom,
                             int n_det, const int16_t* lor, int n_planes, const int32_t* plane_rings,
                             int accel, uint8_t* mask, hipStream_t st);
+// Device scratch of the tail fit: 64-bit fixed-point images (prompts,
+// scatter, randoms; sensitivity as a count), 4 * nvox; the per-block partial
+// sums of the reduction; the three largest masked values (float bits).
+struct FitScratch {
+    unsigned long long* img;
+    double* part;
+    unsigned int* vmax;
+};
+size_t fit_scratch_bytes(int64_t nvox);
+FitScratch fit_scratch_carve(void* base, int64_t nvox);   // base 256-byte aligned, fit_scratch_bytes long
+// sums (device, 2 doubles) = (sum S_bp T, sum S_bp^2); images_out (device,
+// 4 * nvox doubles, may be null) = the four images in float64.
 hipError_t launch_fit(const Grid& g, const double* geom, int n_det, const int16_t* lor, int n_entries,
                       const int32_t* plane_rings, int n_value_planes, int accel, int length_weighted,
                       const float* scatter, const float* prompts, const float* randoms, const float* mask,
-                      double* img, double* sums, hipStream_t st);
+                      FitScratch scratch, double* sums, double* images_out, hipStream_t st);
 hipError_t launch_dense_scatter_typed(const void* compact, bool fp64, int n_bins, int nru, int ndu,
                                       int n_det, const int32_t* dets, const int16_t* lor,
                                       float* dense, hipStream_t st);

@@ -1378,14 +1378,15 @@ int sss_tail_fit(sss_ctx* ctx, const int32_t grid[3], const double bounds[6], in
     int16_t* d_lor = nullptr;
     int32_t* d_pr = nullptr;
     float *d_s = nullptr, *d_p = nullptr, *d_r = nullptr, *d_m = nullptr;
+    unsigned char* d_scratch = nullptr;
     int64_t tally = 0;
-    if ((rc = dev_alloc(&d_geom, ngeom, &tally)) || (rc = dev_alloc(&d_img, 4 * nvox, &tally)) ||
+    if ((rc = dev_alloc(&d_geom, ngeom, &tally)) || (images_out && (rc = dev_alloc(&d_img, 4 * nvox, &tally))) ||
         (rc = dev_alloc(&d_sums, 2, &tally)) || (rc = dev_alloc(&d_lor, nlor, &tally)) ||
         (rc = dev_alloc(&d_pr, (size_t)3 * n_entries + 1, &tally)) || (rc = dev_alloc(&d_s, nsino, &tally)) ||
         (rc = dev_alloc(&d_p, nsino, &tally)) || (rc = dev_alloc(&d_r, nsino, &tally)) ||
-        (rc = dev_alloc(&d_m, nsino, &tally))) {
+        (rc = dev_alloc(&d_m, nsino, &tally)) || (rc = dev_alloc(&d_scratch, fit_scratch_bytes((int64_t)nvox), &tally))) {
         dev_free(d_geom); dev_free(d_img); dev_free(d_sums); dev_free(d_lor); dev_free(d_pr);
-        dev_free(d_s); dev_free(d_p); dev_free(d_r); dev_free(d_m);
+        dev_free(d_s); dev_free(d_p); dev_free(d_r); dev_free(d_m); dev_free(d_scratch);
         return rc;
     }
     hipError_t e = hipMemcpyAsync(d_geom, geom, ngeom * sizeof(double), hipMemcpyHostToDevice, st);
@@ -1396,17 +1397,15 @@ int sss_tail_fit(sss_ctx* ctx, const int32_t grid[3], const double bounds[6], in
     if (e == hipSuccess) e = hipMemcpyAsync(d_p, prompts, nsino * sizeof(float), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(d_r, randoms, nsino * sizeof(float), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(d_m, mask, nsino * sizeof(float), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemsetAsync(d_img, 0, 4 * nvox * sizeof(double), st);
-    if (e == hipSuccess) e = hipMemsetAsync(d_sums, 0, 2 * sizeof(double), st);
     if (e == hipSuccess)
         e = launch_fit(g, d_geom, n_det, d_lor, n_entries, d_pr, n_value_planes, accel, length_weighted, d_s,
-                       d_p, d_r, d_m, d_img, d_sums, st);
+                       d_p, d_r, d_m, fit_scratch_carve(d_scratch, (int64_t)nvox), d_sums, d_img, st);
     if (e == hipSuccess) e = hipMemcpyAsync(sums_out, d_sums, 2 * sizeof(double), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess && images_out)
         e = hipMemcpyAsync(images_out, d_img, 4 * nvox * sizeof(double), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     dev_free(d_geom); dev_free(d_img); dev_free(d_sums); dev_free(d_lor); dev_free(d_pr);
-    dev_free(d_s); dev_free(d_p); dev_free(d_r); dev_free(d_m);
+    dev_free(d_s); dev_free(d_p); dev_free(d_r); dev_free(d_m); dev_free(d_scratch);
     if (e != hipSuccess) return hip_fail(e, "sss_tail_fit");
     return SSS_OK;
 }

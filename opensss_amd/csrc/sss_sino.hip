@@ -8,11 +8,15 @@
 //                 mu (or lambda); a cell is a tail when the float64 attenuation
 //                 factor 1/exp(-sum mu L) is exactly 1 (or the activity is 0).
 //   k_fit_bp      BackProjection.py:31-113,217-300: masked scatter, prompts and
-//                 randoms backprojected along every surviving LOR into float64
-//                 images plus the sensitivity image.
+//                 randoms backprojected along every surviving LOR into 64-bit
+//                 fixed-point images plus the sensitivity image (integer
+//                 atomics: the same bits in every run).
 //   k_fit_reduce  BackProjection.py:191-210: per-voxel ratios and the float64
-//                 normal equations sum(x y), sum(x x) of the one-parameter fit.
+//   k_fit_final   normal equations sum(x y), sum(x x) of the one-parameter
+//                 fit, reduced in a fixed order.
 #include "sss_walk.h"
+
+#include <algorithm>
 
 namespace sss {
 namespace {
@@ -91,13 +95,72 @@ __global__ void k_tail_mask(Grid g, const double* __restrict__ image, int use_at
     mask[((int64_t)rad * (n_det / 2) + ang) * n_planes + p] = v;
 }
 
+// Fixed-point images of the tail fit.  Each contribution v to the prompt,
+// scatter and randoms images is added as the integer llrint(v 2^f) with
+// 64-bit integer atomics; integer addition is associative, so the images and
+// the fitted scale are the same bits in every run whatever order the LORs
+// land in.  f is chosen per image from the largest masked value |v|max of its
+// sinogram and a bound on the LORs: a walk visits a voxel at most once and a
+// LOR adds at most |v|max to it (value / n, or value L / sum L), so no voxel
+// sum exceeds |v|max n_lors < 2^61 2^-f.  The sensitivity image counts LORs.
+__device__ __forceinline__ double fit_scale(const unsigned int* vmax, int i, double n_lors)
+{
+    const double m = (double)__uint_as_float(vmax[i]);
+    if (!(m > 0.0) || !isfinite(m)) return 1.0;   // all zero, or a non-finite sinogram
+    int e;
+    (void)frexp(m * n_lors, &e);   // m n_lors < 2^e
+    return ldexp(1.0, 61 - e);
+}
+
+__device__ __forceinline__ void fit_add(unsigned long long* p, double v, double scale)
+{
+    const long long q = llrint(v * scale);
+    if (q != 0) atomicAdd(p, (unsigned long long)q);
+}
+
+// Every fit image and the maxima to zero (a kernel, so the whole fit can be
+// captured in a graph: no memset nodes).
+__global__ void k_fit_clear(unsigned long long* __restrict__ img, int64_t n, unsigned int* __restrict__ vmax)
+{
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i0 < 3) vmax[i0] = 0u;
+    for (int64_t i = i0; i < n; i += (int64_t)gridDim.x * blockDim.x) img[i] = 0ull;
+}
+
+// Largest |masked value| of the prompt, scatter and randoms sinograms (float
+// bits of non-negative floats order as unsigned integers, so the atomic max
+// is exact and order-independent).
+__global__ void k_fit_max(const float* __restrict__ scatter, const float* __restrict__ prompts,
+                          const float* __restrict__ randoms, const float* __restrict__ maskv, int64_t n,
+                          unsigned int* __restrict__ vmax)
+{
+    unsigned int mp = 0, ms = 0, mr = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float m = maskv[i];
+        mp = max(mp, __float_as_uint(fabsf(m * prompts[i])));
+        ms = max(ms, __float_as_uint(fabsf(m * scatter[i])));
+        mr = max(mr, __float_as_uint(fabsf(m * randoms[i])));
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        mp = max(mp, (unsigned int)__shfl_xor((int)mp, o));
+        ms = max(ms, (unsigned int)__shfl_xor((int)ms, o));
+        mr = max(mr, (unsigned int)__shfl_xor((int)mr, o));
+    }
+    if ((threadIdx.x % kWave) == 0) {
+        atomicMax(vmax + 0, mp);
+        atomicMax(vmax + 1, ms);
+        atomicMax(vmax + 2, mr);
+    }
+}
+
 // plane_rings: (n_entries, 3) = ring of d1, ring of d2, plane the values are read from.
 __global__ void k_fit_bp(Grid g, const double* __restrict__ geom, int n_det,
                          const int16_t* __restrict__ lor, int n_entries,
                          const int32_t* __restrict__ plane_rings, int n_value_planes, int accel, int nd,
-                         int length_weighted, const float* __restrict__ scatter,
+                         int length_weighted, double n_lors, const float* __restrict__ scatter,
                          const float* __restrict__ prompts, const float* __restrict__ randoms,
-                         const float* __restrict__ maskv, double* __restrict__ img)
+                         const float* __restrict__ maskv, const unsigned int* __restrict__ vmax,
+                         unsigned long long* __restrict__ img)
 {
     const int64_t per_plane = (int64_t)nd * nd;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -122,16 +185,22 @@ __global__ void k_fit_bp(Grid g, const double* __restrict__ geom, int n_det,
     Walk w;
     if (!walk_setup(g, lc, w)) return;
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
+    const double sc_p = fit_scale(vmax, 0, n_lors), sc_s = fit_scale(vmax, 1, n_lors),
+                 sc_r = fit_scale(vmax, 2, n_lors);
+    unsigned long long* const ip = img;
+    unsigned long long* const is = img + nvox;
+    unsigned long long* const ir = img + 2 * nvox;
+    unsigned long long* const isens = img + 3 * nvox;
     if (!length_weighted) {
         // value / n per voxel, the division in float32 (BackProjection.py:108-111)
         const int n = walk_run(g, w, [&](int, int, double, double) {});
         if (n <= 0) return;
         const double ps = (double)(pv / (float)n), ss = (double)(sv / (float)n), rs = (double)(rv / (float)n);
         walk_run(g, w, [&](int, int vox, double, double) {
-            atomicAdd(img + 0 * nvox + vox, ps);
-            atomicAdd(img + 1 * nvox + vox, ss);
-            atomicAdd(img + 2 * nvox + vox, rs);
-            atomicAdd(img + 3 * nvox + vox, 1.0);
+            fit_add(ip + vox, ps, sc_p);
+            fit_add(is + vox, ss, sc_s);
+            fit_add(ir + vox, rs, sc_r);
+            atomicAdd(isens + vox, 1ull);
         });
     } else {
         // value * L / sum(L) per voxel in float64 (BackProjection.py:294-297)
@@ -139,25 +208,35 @@ __global__ void k_fit_bp(Grid g, const double* __restrict__ geom, int n_det,
         const int n = walk_run(g, w, [&](int, int, double L, double) { lsum += L; });
         if (n <= 0) return;
         walk_run(g, w, [&](int, int vox, double L, double) {
-            atomicAdd(img + 0 * nvox + vox, (double)pv * L / lsum);
-            atomicAdd(img + 1 * nvox + vox, (double)sv * L / lsum);
-            atomicAdd(img + 2 * nvox + vox, (double)rv * L / lsum);
-            atomicAdd(img + 3 * nvox + vox, 1.0);
+            fit_add(ip + vox, (double)pv * L / lsum, sc_p);
+            fit_add(is + vox, (double)sv * L / lsum, sc_s);
+            fit_add(ir + vox, (double)rv * L / lsum, sc_r);
+            atomicAdd(isens + vox, 1ull);
         });
     }
 }
 
-__global__ void k_fit_reduce(const double* __restrict__ img, int64_t nvox, double* __restrict__ sums)
+// Per-voxel ratios and the two normal-equation sums, per block of a fixed
+// grid (kFitBlocks x 256 threads, voxels strided over the grid): each block
+// writes its partial sums; k_fit_final adds them in a fixed order.
+constexpr int kFitBlocks = 256;
+constexpr int kFitThreads = 256;
+
+__global__ __launch_bounds__(kFitThreads) void k_fit_reduce(const unsigned long long* __restrict__ img, int64_t nvox,
+                                                           double n_lors, const unsigned int* __restrict__ vmax,
+                                                           double* __restrict__ part)
 {
-    __shared__ double sxy[256], sxx[256];
+    __shared__ double sxy[kFitThreads], sxx[kFitThreads];
+    const double inv_p = 1.0 / fit_scale(vmax, 0, n_lors), inv_s = 1.0 / fit_scale(vmax, 1, n_lors),
+                 inv_r = 1.0 / fit_scale(vmax, 2, n_lors);
     double xy = 0.0, xx = 0.0;
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nvox;
          v += (int64_t)gridDim.x * blockDim.x) {
-        const double sens = img[3 * nvox + v];
+        const double sens = (double)(long long)img[3 * nvox + v];
         if (sens == 0.0) continue;   // 0/0 and x/0 are set to 0 in the reference
-        const double sbp = img[1 * nvox + v] / sens;
-        const double abp = img[0 * nvox + v] / sens;
-        const double rbp = img[2 * nvox + v] / sens;
+        const double sbp = (double)(long long)img[1 * nvox + v] * inv_s / sens;
+        const double abp = (double)(long long)img[0 * nvox + v] * inv_p / sens;
+        const double rbp = (double)(long long)img[2 * nvox + v] * inv_r / sens;
         double tails = abp - rbp;
         tails = tails < 0.0 ? 0.0 : tails;
         xy += sbp * tails;
@@ -166,7 +245,7 @@ __global__ void k_fit_reduce(const double* __restrict__ img, int64_t nvox, doubl
     sxy[threadIdx.x] = xy;
     sxx[threadIdx.x] = xx;
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    for (int s = kFitThreads / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
             sxy[threadIdx.x] += sxy[threadIdx.x + s];
             sxx[threadIdx.x] += sxx[threadIdx.x + s];
@@ -174,9 +253,40 @@ __global__ void k_fit_reduce(const double* __restrict__ img, int64_t nvox, doubl
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        atomicAdd(sums + 0, sxy[0]);
-        atomicAdd(sums + 1, sxx[0]);
+        part[2 * blockIdx.x + 0] = sxy[0];
+        part[2 * blockIdx.x + 1] = sxx[0];
     }
+}
+
+// One block: the partial sums in a fixed tree order.
+__global__ __launch_bounds__(kFitBlocks) void k_fit_final(const double* __restrict__ part, double* __restrict__ sums)
+{
+    __shared__ double sxy[kFitBlocks], sxx[kFitBlocks];
+    sxy[threadIdx.x] = part[2 * threadIdx.x + 0];
+    sxx[threadIdx.x] = part[2 * threadIdx.x + 1];
+    __syncthreads();
+    for (int s = kFitBlocks / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sxy[threadIdx.x] += sxy[threadIdx.x + s];
+            sxx[threadIdx.x] += sxx[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        sums[0] = sxy[0];
+        sums[1] = sxx[0];
+    }
+}
+
+// The fixed-point images as float64 (prompts, scatter, randoms, sensitivity).
+__global__ void k_fit_images(const unsigned long long* __restrict__ img, int64_t nvox, double n_lors,
+                             const unsigned int* __restrict__ vmax, double* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 4 * nvox) return;
+    const int k = (int)(i / nvox);
+    const double inv = k == 3 ? 1.0 : 1.0 / fit_scale(vmax, k, n_lors);
+    out[i] = (double)(long long)img[i] * inv;
 }
 
 inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -206,22 +316,60 @@ hipError_t launch_tail_mask(const Grid& g, const double* image, int use_attenuat
     return hipGetLastError();
 }
 
+size_t fit_scratch_bytes(int64_t nvox)
+{
+    return (size_t)(4 * nvox) * sizeof(unsigned long long) + 4 * sizeof(unsigned int) +
+           (size_t)(2 * kFitBlocks) * sizeof(double) + 256;
+}
+
+FitScratch fit_scratch_carve(void* base, int64_t nvox)
+{
+    unsigned char* p = static_cast<unsigned char*>(base);
+    FitScratch f;
+    f.img = reinterpret_cast<unsigned long long*>(p);
+    p += (size_t)(4 * nvox) * sizeof(unsigned long long);
+    f.part = reinterpret_cast<double*>(p);
+    p += (size_t)(2 * kFitBlocks) * sizeof(double);
+    f.vmax = reinterpret_cast<unsigned int*>(p);
+    return f;
+}
+
 hipError_t launch_fit(const Grid& g, const double* geom, int n_det, const int16_t* lor, int n_entries,
                       const int32_t* plane_rings, int n_value_planes, int accel, int length_weighted,
                       const float* scatter, const float* prompts, const float* randoms, const float* mask,
-                      double* img, double* sums, hipStream_t st)
+                      FitScratch scratch, double* sums, double* images_out, hipStream_t st)
 {
     const int nd = (n_det + accel - 1) / accel;
-    const int64_t n = (int64_t)nd * nd * n_entries;
-    if (n > 0)
-        hipLaunchKernelGGL(k_fit_bp, dim3(blocks_for(n, 128)), dim3(128), 0, st, g, geom, n_det, lor,
-                           n_entries, plane_rings, n_value_planes, accel, nd, length_weighted, scatter,
-                           prompts, randoms, mask, img);
+    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
+    const int64_t n_sino = (int64_t)(n_det + 1) * (n_det / 2) * n_value_planes;
+    // every LOR the fit can trace: entries x detector pairs d1 <= d2
+    const double n_lors = (double)n_entries * ((double)nd * (nd + 1) / 2) + 1.0;
+    hipLaunchKernelGGL(k_fit_clear, dim3(1024), dim3(256), 0, st, scratch.img, 4 * nvox, scratch.vmax);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
-    hipLaunchKernelGGL(k_fit_reduce, dim3(256), dim3(256), 0, st, img, nvox, sums);
-    return hipGetLastError();
+    if (n_sino > 0) {
+        hipLaunchKernelGGL(k_fit_max, dim3((unsigned)std::min<int64_t>(1024, blocks_for(n_sino, 256))), dim3(256), 0,
+                           st, scatter, prompts, randoms, mask, n_sino, scratch.vmax);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    const int64_t n = (int64_t)nd * nd * n_entries;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_fit_bp, dim3(blocks_for(n, 128)), dim3(128), 0, st, g, geom, n_det, lor,
+                           n_entries, plane_rings, n_value_planes, accel, nd, length_weighted, n_lors, scatter,
+                           prompts, randoms, mask, scratch.vmax, scratch.img);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_fit_reduce, dim3(kFitBlocks), dim3(kFitThreads), 0, st, scratch.img, nvox, n_lors,
+                       scratch.vmax, scratch.part);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_fit_final, dim3(1), dim3(kFitBlocks), 0, st, scratch.part, sums);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (images_out) {
+        hipLaunchKernelGGL(k_fit_images, dim3(blocks_for(4 * nvox, 256)), dim3(256), 0, st, scratch.img, nvox,
+                           n_lors, scratch.vmax, images_out);
+        e = hipGetLastError();
+    }
+    return e;
 }
 
 }  // namespace sss

@@ -195,3 +195,27 @@ def test_scale_scatters_to_prompts_drop_in(oracle, tmp_path):
                                   np.float32(mask >= 1), grid, w.image_size, w.geometry, w.lor_coordinates, 2,
                                   SinogramOrder=w.sinogram_index[:NR, :NR].T)
         assert abs(got[b] - want[0]) <= 1e-5 * abs(want[0])
+
+
+@pytest.mark.parametrize('span', [False, True])
+def test_tail_fit_is_deterministic(span):
+    """The fit's images are 64-bit fixed-point sums of integer atomics and its
+    normal equations are reduced in a fixed order: two runs give the same
+    bits (scale, sums and images), whatever order the LORs land in."""
+    from opensss_amd.backprojection import nospan_entries, span_entries, tail_fit
+    w = _case()
+    NR = w.geometry.shape[0]
+    if span:
+        geom, _ = expandRings(w.geometry, w.normals)
+        msi = SinogramLUT(NR, 3, 4)[0]
+        entries, n_planes, lw = span_entries(msi), msi.shape[0], True
+    else:
+        geom = w.geometry
+        entries, n_planes, lw = nospan_entries(w.sinogram_index[:NR, :NR].T, NR), NR * NR, False
+    P, S, R, M = _fit_inputs(w, n_planes, 11)
+    runs = [tail_fit(P, S, R, M, np.array([24, 24, 12]), w.image_size, geom, w.lor_coordinates, entries, 1, lw,
+                     return_images=True) for _ in range(3)]
+    for r in runs[1:]:
+        assert r[0] == runs[0][0] and r[1] == runs[0][1] and r[2] == runs[0][2]
+        assert np.array_equal(r[3], runs[0][3])
+    assert runs[0][2] > 0

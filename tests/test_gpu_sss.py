@@ -243,6 +243,66 @@ def test_weights_only_refresh_and_its_fallback(precision):
         s.close()
 
 
+def _swap_voxels_on_the_same_paths(oracle, w, n_pairs=4):
+    """Voxel pairs (u, v), both active, that the same half paths cross with
+    positive length, found by tracing every half path of the subject: zeroing
+    u in one image and v in the next keeps every path's count of non-zero
+    terms but moves its zero term."""
+    from opensss_amd.tables import detectors_used, rings_used, scatter_point_coordinates
+    NR, N = w.geometry.shape[0], w.geometry.shape[1]
+    rings = rings_used(NR, w.n_rings_used)
+    dets = detectors_used(N, w.n_det_used)
+    xyz = scatter_point_coordinates(w.scatter_points, w.attenuation.shape, w.image_size)
+    grid = np.array(w.attenuation.shape)
+    lam = w.activity.ravel(order='F')
+    visits = {}
+    path = 0
+    for p in xyz:
+        for r in rings:
+            for d in dets:
+                det = [w.geometry[r, d, 0], w.geometry[r, d, 1], w.geometry[r, 0, 2]]
+                L, idx, _ = oracle.trace(grid, w.image_size, np.concatenate([p, det]))
+                for li, vi in zip(L, idx - 1):
+                    if li > 0 and lam[vi] > 0:
+                        visits.setdefault(int(vi), set()).add(path)
+                path += 1
+    by_set = {}
+    for v, paths in visits.items():
+        by_set.setdefault(frozenset(paths), []).append(v)
+    pairs = [vs[:2] for vs in by_set.values() if len(vs) >= 2]
+    return pairs[:n_pairs]
+
+
+@pytest.mark.parametrize('precision', ['fp32', 'fp64'])
+def test_weights_only_refresh_sees_moved_zero_terms(oracle, precision):
+    """Images A and B with the same count of non-zero terms on every half
+    path but the zero terms in other voxels: the weights-only refresh must
+    notice the moved zeros (a fold record whose term became zero) and rebuild
+    the fold table; each result equals a fresh subject's, bit for bit."""
+    w = phantoms.small_tof(seed=9, n_sp=6)
+    pairs = _swap_voxels_on_the_same_paths(oracle, w)
+    assert pairs, 'no voxel pair shares its half paths'
+    X, Y, Z = w.attenuation.shape
+    a = w.activity.copy()
+    b = w.activity.copy()
+    for u, v in pairs:
+        a[np.unravel_index(u, (X, Y, Z), order='F')] = 0.0
+        b[np.unravel_index(v, (X, Y, Z), order='F')] = 0.0
+    images = [a, a * 1.1, b, b * 0.9]
+    s = gpu_subject(w, precision)
+    try:
+        s.run(images[0])
+        for k, img in enumerate(images):
+            if k:
+                s.set_activity(img)
+                s.scatter()
+            got = s.output_compact()
+            want = gpu_compact(w, precision, lam=img)
+            assert np.array_equal(got, want), f'image {k}'
+    finally:
+        s.close()
+
+
 @pytest.mark.parametrize('n_bins', [1, 7])
 def test_volume_upload_orders_agree(n_bins):
     """Volumes sent in C order (transposed on the device, through a 32 x 32
