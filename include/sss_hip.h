@@ -237,6 +237,26 @@ int sss_tail_fit(sss_ctx* ctx, const int32_t grid[3], const double bounds[6], in
                  const float* prompts, const float* randoms, const float* mask, double* sums_out,
                  double* images_out);
 
+/* sss_tail_fit on device buffers, allocating nothing.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  Every
+ * pointer except grid and bounds (host) is device memory on the context's
+ * device: geom, lor_coords, plane_rings, the four sinograms as in
+ * sss_tail_fit; sums_out (2 doubles) and images_out (4 * X*Y*Z doubles, may
+ * be NULL) receive what sss_tail_fit returns, bit for bit.  Asynchronous on
+ * the context's stream.  Entries or coordinates out of range are skipped.
+ * The fit's scratch (fixed-point images, reduction partials) is a static
+ * device array of the library, one per GPU, sized for fit grids of at most
+ * SSS_TAIL_FIT_NOALLOC_MAX_VOXELS voxels (larger grids: SSS_ERR_UNSUPPORTED).
+ * Calls that overlap on one GPU (from several contexts or threads) share
+ * that array and race; each call clears it before use. */
+#define SSS_TAIL_FIT_NOALLOC_MAX_VOXELS (1 << 20)
+int sss_tail_fit_noalloc(sss_ctx* ctx, const int32_t grid[3], const double bounds[6], int n_rings_geom,
+                         int n_det, const double* geom, const int16_t* lor_coords, int n_entries,
+                         const int32_t* plane_rings, int n_value_planes, int accel, int length_weighted,
+                         const float* scatter, const float* prompts, const float* randoms, const float* mask,
+                         double* sums_out, double* images_out);
+
 #ifdef __cplusplus
 }
 #endif

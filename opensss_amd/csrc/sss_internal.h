@@ -251,14 +251,25 @@ struct FitScratch {
     double* part;
     unsigned int* vmax;
 };
-size_t fit_scratch_bytes(int64_t nvox);
+constexpr int kFitBlocks = 256;              // blocks of the fit's fixed-order reduction
+constexpr size_t fit_scratch_bytes(int64_t nvox)
+{
+    return (size_t)(4 * nvox) * sizeof(unsigned long long) + (size_t)(2 * kFitBlocks) * sizeof(double) +
+           4 * sizeof(unsigned int) + 256;
+}
 FitScratch fit_scratch_carve(void* base, int64_t nvox);   // base 256-byte aligned, fit_scratch_bytes long
 // sums (device, 2 doubles) = (sum S_bp T, sum S_bp^2); images_out (device,
 // 4 * nvox doubles, may be null) = the four images in float64.
-hipError_t launch_fit(const Grid& g, const double* geom, int n_det, const int16_t* lor, int n_entries,
+hipError_t launch_fit(const Grid& g, const double* geom, int n_rings_geom, int n_det, const int16_t* lor, int n_entries,
                       const int32_t* plane_rings, int n_value_planes, int accel, int length_weighted,
                       const float* scatter, const float* prompts, const float* randoms, const float* mask,
                       FitScratch scratch, double* sums, double* images_out, hipStream_t st);
+// Device address of this library's no-allocation fit scratch on the current
+// device (looked up once per device at context creation).
+int noalloc_scratch_init(int device);
+void* noalloc_scratch(int device);
+int ctx_device(const sss_ctx* ctx);
+
 hipError_t launch_dense_scatter_typed(const void* compact, bool fp64, int n_bins, int nru, int ndu,
                                       int n_det, const int32_t* dets, const int16_t* lor,
                                       float* dense, hipStream_t st);

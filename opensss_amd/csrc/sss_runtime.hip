@@ -231,6 +231,8 @@ struct sss_ctx {
     std::vector<hipEvent_t> events[2];
 };
 
+int sss::ctx_device(const sss_ctx* ctx) { return ctx->device; }
+
 // An event from the context's cache, or a new one.
 static hipError_t ctx_event(sss_ctx* c, bool timing, hipEvent_t* e)
 {
@@ -417,6 +419,12 @@ int sss_ctx_create(int device, sss_ctx** out)
         return hip_fail(e, "sss_ctx_create");
     }
     for (int i = kPinnedSlots - 1; i >= 0; --i) c->pinned_free.push_back(i);
+    if (int rc = noalloc_scratch_init(device)) {
+        (void)hipHostFree(c->pinned);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    }
     *out = c;
     return SSS_OK;
 }
@@ -1398,7 +1406,7 @@ int sss_tail_fit(sss_ctx* ctx, const int32_t grid[3], const double bounds[6], in
     if (e == hipSuccess) e = hipMemcpyAsync(d_r, randoms, nsino * sizeof(float), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(d_m, mask, nsino * sizeof(float), hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
-        e = launch_fit(g, d_geom, n_det, d_lor, n_entries, d_pr, n_value_planes, accel, length_weighted, d_s,
+        e = launch_fit(g, d_geom, n_rings_geom, n_det, d_lor, n_entries, d_pr, n_value_planes, accel, length_weighted, d_s,
                        d_p, d_r, d_m, fit_scratch_carve(d_scratch, (int64_t)nvox), d_sums, d_img, st);
     if (e == hipSuccess) e = hipMemcpyAsync(sums_out, d_sums, 2 * sizeof(double), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess && images_out)

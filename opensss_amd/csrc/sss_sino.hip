@@ -154,7 +154,7 @@ __global__ void k_fit_max(const float* __restrict__ scatter, const float* __rest
 }
 
 // plane_rings: (n_entries, 3) = ring of d1, ring of d2, plane the values are read from.
-__global__ void k_fit_bp(Grid g, const double* __restrict__ geom, int n_det,
+__global__ void k_fit_bp(Grid g, const double* __restrict__ geom, int n_rings_geom, int n_det,
                          const int16_t* __restrict__ lor, int n_entries,
                          const int32_t* __restrict__ plane_rings, int n_value_planes, int accel, int nd,
                          int length_weighted, double n_lors, const float* __restrict__ scatter,
@@ -171,9 +171,14 @@ __global__ void k_fit_bp(Grid g, const double* __restrict__ geom, int n_det,
     const int d2 = (int)(rem % nd) * accel;
     const int ra = plane_rings[3 * e], rb = plane_rings[3 * e + 1], vp = plane_rings[3 * e + 2];
     if ((d1 == d2 && ra == rb) || d1 > d2) return;
+    // entries and coordinates may come from device memory unchecked: out of
+    // range ones are skipped
+    if ((unsigned)ra >= (unsigned)n_rings_geom || (unsigned)rb >= (unsigned)n_rings_geom ||
+        (unsigned)vp >= (unsigned)n_value_planes)
+        return;
     const int ang = lor[((int64_t)d1 * n_det + d2) * 2 + 0];
     const int rad = lor[((int64_t)d1 * n_det + d2) * 2 + 1];
-    if (ang < 0 || rad < 0) return;
+    if (ang < 0 || rad < 0 || ang >= n_det / 2 || rad > n_det) return;
     const int64_t cidx = ((int64_t)rad * (n_det / 2) + ang) * n_value_planes + vp;
     const float m = maskv[cidx];
     const float sv = m * scatter[cidx];           // float32 products as the reference's masking
@@ -219,7 +224,6 @@ __global__ void k_fit_bp(Grid g, const double* __restrict__ geom, int n_det,
 // Per-voxel ratios and the two normal-equation sums, per block of a fixed
 // grid (kFitBlocks x 256 threads, voxels strided over the grid): each block
 // writes its partial sums; k_fit_final adds them in a fixed order.
-constexpr int kFitBlocks = 256;
 constexpr int kFitThreads = 256;
 
 __global__ __launch_bounds__(kFitThreads) void k_fit_reduce(const unsigned long long* __restrict__ img, int64_t nvox,
@@ -316,12 +320,6 @@ hipError_t launch_tail_mask(const Grid& g, const double* image, int use_attenuat
     return hipGetLastError();
 }
 
-size_t fit_scratch_bytes(int64_t nvox)
-{
-    return (size_t)(4 * nvox) * sizeof(unsigned long long) + 4 * sizeof(unsigned int) +
-           (size_t)(2 * kFitBlocks) * sizeof(double) + 256;
-}
-
 FitScratch fit_scratch_carve(void* base, int64_t nvox)
 {
     unsigned char* p = static_cast<unsigned char*>(base);
@@ -334,7 +332,7 @@ FitScratch fit_scratch_carve(void* base, int64_t nvox)
     return f;
 }
 
-hipError_t launch_fit(const Grid& g, const double* geom, int n_det, const int16_t* lor, int n_entries,
+hipError_t launch_fit(const Grid& g, const double* geom, int n_rings_geom, int n_det, const int16_t* lor, int n_entries,
                       const int32_t* plane_rings, int n_value_planes, int accel, int length_weighted,
                       const float* scatter, const float* prompts, const float* randoms, const float* mask,
                       FitScratch scratch, double* sums, double* images_out, hipStream_t st)
@@ -354,7 +352,7 @@ hipError_t launch_fit(const Grid& g, const double* geom, int n_det, const int16_
     }
     const int64_t n = (int64_t)nd * nd * n_entries;
     if (n > 0) {
-        hipLaunchKernelGGL(k_fit_bp, dim3(blocks_for(n, 128)), dim3(128), 0, st, g, geom, n_det, lor,
+        hipLaunchKernelGGL(k_fit_bp, dim3(blocks_for(n, 128)), dim3(128), 0, st, g, geom, n_rings_geom, n_det, lor,
                            n_entries, plane_rings, n_value_planes, accel, nd, length_weighted, n_lors, scatter,
                            prompts, randoms, mask, scratch.vmax, scratch.img);
         if ((e = hipGetLastError()) != hipSuccess) return e;
