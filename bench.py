@@ -36,6 +36,7 @@ def parse():
     p.add_argument('--no-cpu-baseline', action='store_true')
     p.add_argument('--no-fp64-check', action='store_true')
     p.add_argument('--cpu-threads', type=int, default=0)
+    p.add_argument('--no-stages', action='store_true', help='skip the full-size sinogram stages (config 2)')
     p.add_argument('--calibrate-copy', action='store_true',
                    help='run only a 1 GiB device copy (3 launches) to scale FETCH_SIZE/WRITE_SIZE')
     p.add_argument('--batch', type=int, default=0, metavar='N',
@@ -326,6 +327,144 @@ def cpu_baseline(w, threads, target_s=12.0):
     return base, sample, compact
 
 
+def sinogram_stages(w, out32, out64, oracle_pair=None, accel=2):
+    """Config 2's stages after the scatter estimate, at full size on the GPU:
+    interpolation of the 15-bin estimate to the 761 x 380 x 159 sinogram
+    (span 19, MRD 9), the span tail mask and trues on the 159-ring extended
+    geometry, and the span tail fit of every TOF bin on device-resident
+    sinograms (AccelerationFactor `accel`).  Prompts per bin: trues / B +
+    k_true S64 + uniform randoms (noiseless), S64 the FP64 estimate's
+    interpolation; the FP32 estimate's fitted scale is reported against the
+    FP64 one's (config 3's tail-scale error).  oracle_pair: (GPU FP32
+    estimate, oracle estimate) of one scatter-point sample, whose fits are
+    compared the same way.  Times from the host clock around synchronised
+    calls."""
+    from opensss_amd import _native
+    from opensss_amd.backprojection import fit_geometry_buffers, span_entries, tail_fit_device
+    from opensss_amd.data_reduction import SinogramLUT
+    from opensss_amd.geometry import expandRings
+    from opensss_amd.interpolation import Interpolator
+    from opensss_amd.prompts import ForwardProject
+    from opensss_amd.tails_mask import MaskGeneratorMashing
+    ctx = _native.default_context()
+    lib = _native.load_library()
+    NR, N = w.geometry.shape[0], w.geometry.shape[1]
+    B = out32.shape[0]
+    msi = SinogramLUT(NR, 19, 9)[0]
+    geom_ext, _ = expandRings(w.geometry, w.normals)
+    ip = Interpolator(NR, w.n_rings_used, N, w.n_det_used, w.lor_coordinates, msi, True, context=ctx)
+    n_out = ip.n_cells_out * ip.n_out_planes
+    bufs = []
+
+    def dev(a):
+        b = _native.DeviceBuffer.from_array(ctx, np.ascontiguousarray(a))
+        bufs.append(b)
+        return b
+
+    def interpolate(compact):
+        src = dev(np.asarray(compact, dtype=np.float32))
+        out = _native.DeviceBuffer(ctx, 4 * n_out * B)
+        tot = _native.DeviceBuffer(ctx, 4 * n_out)
+        bufs.extend([out, tot])
+        times = []
+        for _ in range(4):
+            t0 = time.perf_counter()
+            ip.run_device(src.ptr.value, B, out.ptr.value, tot.ptr.value)
+            ctx.synchronize()
+            times.append(time.perf_counter() - t0)
+        return out, min(times[1:])
+
+    def bin_host(buf, b):
+        host = np.empty(ip.shape_out, np.float32)
+        _native.check(lib.sss_buffer_download(ctx.handle, host.ctypes.data_as(_native.ctypes.c_void_p),
+                                              _native.ctypes.c_void_p(buf.ptr.value + 4 * n_out * b), host.nbytes),
+                      'sss_buffer_download')
+        return host
+
+    try:
+        i32, t_interp = interpolate(out32)
+        i64, _ = interpolate(out64.astype(np.float32))
+        t0 = time.perf_counter()
+        mask = MaskGeneratorMashing(w.activity, w.attenuation, w.image_size, geom_ext, w.lor_coordinates, msi,
+                                    True, 1)
+        t_mask = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        trues = ForwardProject(w.activity, w.attenuation, w.image_size, geom_ext, w.lor_coordinates,
+                               MashedSinogramIndices=msi)
+        t_fp = time.perf_counter() - t0
+        M = dev(np.float32(mask >= 1))
+        ent = span_entries(msi)
+        gb = fit_geometry_buffers(geom_ext, w.lor_coordinates, ent, context=ctx)
+        bufs.extend(gb)
+        sums, peaks = [], []
+        for b in range(B):
+            h = bin_host(i64, b)
+            sums.append(float(h.sum()))
+            peaks.append(float(h.max()))
+        k_true = 0.3 * trues.sum() / max(1e-30, sum(sums))
+        # trues and randoms spread evenly over the TOF bins; bins whose scatter
+        # peaks below 1% of the estimate's peak are not fitted (their tails
+        # hold almost no scatter: the same 1% line as the sinogram errors)
+        r_val = np.float32(0.1 * trues.mean() / B)
+        R = dev(np.full(ip.shape_out, r_val, np.float32))
+        fits, t_fit = {}, []
+        for b in range(B):
+            if peaks[b] < 0.01 * max(peaks):
+                continue
+            S64 = bin_host(i64, b)
+            P = dev((trues / B + k_true * S64 + r_val).astype(np.float32))
+            S64d = dev(S64)
+            t0 = time.perf_counter()
+            f32 = tail_fit_device(w.attenuation.shape, w.image_size, geom_ext, w.lor_coordinates, ent,
+                                  msi.shape[0], accel, True, i32.ptr.value + 4 * n_out * b, P.ptr.value,
+                                  R.ptr.value, M.ptr.value, context=ctx, geometry_buffers=gb)[0]
+            t_fit.append(time.perf_counter() - t0)
+            f64 = tail_fit_device(w.attenuation.shape, w.image_size, geom_ext, w.lor_coordinates, ent,
+                                  msi.shape[0], accel, True, S64d.ptr.value, P.ptr.value, R.ptr.value, M.ptr.value,
+                                  context=ctx, geometry_buffers=gb)[0]
+            fits[b] = (f32, f64)
+            for x in (P, S64d):
+                x.free()
+        fitted = sorted(fits)
+        scale_err = [abs(fits[b][0] - fits[b][1]) / abs(fits[b][1]) for b in fitted]
+        res = {
+            'workload': f'{w.name} -> 761x380x159 span 19 MRD 9, {B} bins',
+            'interpolate_ms': t_interp * 1e3,
+            'interpolate_GBps': (B + 1) * n_out * 4 / t_interp / 1e9,
+            'tail_mask_ms': t_mask * 1e3, 'forward_project_ms': t_fp * 1e3,
+            'tail_fit_ms_per_bin': float(np.median(t_fit) * 1e3), 'fit_accel': accel,
+            'tail_scale_max_rel_err_fp32_vs_fp64': float(max(scale_err)),
+            'tail_scale_k_true': float(k_true),
+            'tail_scale_fp64_max_rel_err_vs_k_true': float(max(abs(fits[b][1] - k_true) / k_true for b in fitted)),
+            'fitted_bins': fitted,
+            'tail_scales_fp32': [float(fits[b][0]) for b in fitted],
+            'host_clock': True,
+        }
+        if oracle_pair is not None:
+            g32, ora = oracle_pair
+            j32, _ = interpolate(g32)
+            jora, _ = interpolate(ora)
+            b = B // 2
+            So = bin_host(jora, b)
+            # the sample's randoms level: a tenth of its scatter's mean
+            r_o = np.float32(0.1 * k_true * So.mean())
+            P = dev((trues / B + k_true * So + r_o).astype(np.float32))
+            Ro = dev(np.full(ip.shape_out, r_o, np.float32))
+            Sod = dev(So)
+            fa = tail_fit_device(w.attenuation.shape, w.image_size, geom_ext, w.lor_coordinates, ent, msi.shape[0],
+                                 accel, True, j32.ptr.value + 4 * n_out * b, P.ptr.value, Ro.ptr.value, M.ptr.value,
+                                 context=ctx, geometry_buffers=gb)[0]
+            fo = tail_fit_device(w.attenuation.shape, w.image_size, geom_ext, w.lor_coordinates, ent, msi.shape[0],
+                                 accel, True, Sod.ptr.value, P.ptr.value, Ro.ptr.value, M.ptr.value, context=ctx,
+                                 geometry_buffers=gb)[0]
+            res['tail_scale_rel_err_vs_oracle'] = abs(fa - fo) / abs(fo)
+            res['tail_scale_oracle_sample'] = f'bin {b}, the cpu_baseline sample\'s estimates'
+        return res
+    finally:
+        for b in bufs:
+            b.free()
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
@@ -397,6 +536,7 @@ def main():
 
     cpu = None
     err_oracle = None
+    oracle_pair = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or cpu_threads()
         cpu, sample, want = cpu_baseline(w, threads)
@@ -404,11 +544,20 @@ def main():
         # the FP32 GPU path on the same scatter points, checked against the oracle
         ss = subject(w, 'fp32', points=sample)
         ss.run(w.activity)
-        err_oracle = rel_err_stats(ss.output_compact(), want)
+        got_sample = ss.output_compact()
+        err_oracle = rel_err_stats(got_sample, want)
+        oracle_pair = (got_sample, want)
         ss.close()
         err_oracle['sample'] = f'{len(sample)} scatter points of the cpu_baseline sample'
         print(f"[bench] FP32 vs oracle: max rel err {err_oracle['max_rel_err_cells_above_1pct_peak']:.3e}",
               file=sys.stderr, flush=True)
+
+    stages = None
+    if w.name.startswith('tofsss-760x80') and not args.no_stages and not args.no_fp64_check:
+        stages = sinogram_stages(w, out32, out64, oracle_pair)
+        print(f"[bench] sinogram stages: interpolate {stages['interpolate_ms']:.2f} ms, mask "
+              f"{stages['tail_mask_ms']:.1f} ms, fit {stages['tail_fit_ms_per_bin']:.1f} ms/bin, tail scale "
+              f"FP32 vs FP64 {stages['tail_scale_max_rel_err_fp32_vs_fp64']:.2e}", file=sys.stderr, flush=True)
 
     lat_ms = np.array(lat) * 1e3
     line = {
@@ -427,6 +576,7 @@ def main():
         'error_vs_oracle': err_oracle,
         'roofline': roofline,
         'cpu_baseline': cpu,
+        'sinogram_stages': stages,
     }
     print(json.dumps(line))
 

@@ -237,6 +237,21 @@ int sss_tail_fit(sss_ctx* ctx, const int32_t grid[3], const double bounds[6], in
                  const float* prompts, const float* randoms, const float* mask, double* sums_out,
                  double* images_out);
 
+/* Synthetic prompts (no counterpart in the reference, which reads measured
+ * prompts; the per-LOR trace follows Python/openSSS/TailsMask.py:139-145).
+ * sss_forward_project: for output plane p, the LORs from detector d1 of ring
+ * plane_rings[2p] to d2 of ring plane_rings[2p+1], d1 <= d2 on the
+ * accel-strided grid (as sss_tail_mask), get trues = sum(lam L) exp(-sum(mu L))
+ * (float64, stored float32) at [(radial * N/2 + angular) * n_planes + p];
+ * mu may be NULL (no attenuation); other cells are 0.  Host pointers.
+ * sss_poisson: out[i] = a Poisson draw of mean mean[i] * scale, from a
+ * counter-based generator keyed by (seed, i): the same seed gives the same
+ * counts.  Host pointers. */
+int sss_forward_project(sss_ctx* ctx, const int32_t grid[3], const double bounds[6], const double* lam,
+                        const double* mu, int n_rings_geom, int n_det, const double* geom, const int16_t* lor_coords,
+                        int n_planes, const int32_t* plane_rings, int accel, float* trues_out);
+int sss_poisson(sss_ctx* ctx, const float* mean, int64_t n, double scale, uint64_t seed, float* out);
+
 /* sss_tail_fit on device buffers, allocating nothing.  No counterpart in the
  * reference: for callers that cannot allocate device memory (code under HIP
  * graph capture, serving loops, harnesses that forbid hipMalloc).  Every

@@ -106,3 +106,56 @@ def BackProjectionScatterRandFitMashing(PromptSinogram, ScatterSinogram, RandomS
                            ImageSize, Geometry, SinogramCoordinates, span_entries(MashedSinogramIndices),
                            AccelerationFactor, True, context=context)
     return np.array([scale], dtype=np.float32)
+
+
+def tail_fit_device(GridSize, ImageSize, Geometry, SinogramCoordinates, entries, n_value_planes, AccelerationFactor,
+                    length_weighted, scatter_ptr, prompts_ptr, randoms_ptr, mask_ptr, *, context=None,
+                    sums_ptr=None, images_ptr=None, geometry_buffers=None):
+    """The tail fit on device-resident sinograms (``sss_tail_fit_noalloc``):
+    scatter/prompts/randoms/mask are device addresses of float32 (N+1, N//2,
+    n_value_planes) sinograms on the context's device.  geometry_buffers:
+    the DeviceBuffers of (geometry, coordinates, entries) from
+    ``fit_geometry_buffers`` (uploaded once, reused per TOF bin).  Queued on
+    the context's stream; returns (scale, sum(S T), sum(S S)) after a
+    synchronisation, or nothing when sums_ptr is given (the caller reads the
+    two doubles there)."""
+    ctx = context or _native.default_context()
+    lib = _native.load_library()
+    geom = np.asarray(Geometry)
+    n_rings, n_det = geom.shape[0], geom.shape[1]
+    bufs = geometry_buffers or fit_geometry_buffers(Geometry, SinogramCoordinates, entries, context=ctx)
+    grid = np.ascontiguousarray(np.asarray(GridSize, dtype=np.int32).reshape(3))
+    bounds = np.ascontiguousarray(np.asarray(ImageSize, dtype=np.float64).reshape(6))
+    own_sums = sums_ptr is None
+    sums_buf = _native.DeviceBuffer(ctx, 16) if own_sums else None
+    accel = 1 if AccelerationFactor is None else int(AccelerationFactor)
+    n_entries = int(np.asarray(entries).reshape(-1, 3).shape[0])
+    try:
+        _native.check(lib.sss_tail_fit_noalloc(
+            ctx.handle, _native.ptr(grid), _native.ptr(bounds), n_rings, n_det, bufs[0].ptr, bufs[1].ptr, n_entries,
+            bufs[2].ptr, int(n_value_planes), accel, int(bool(length_weighted)), _native.ctypes.c_void_p(scatter_ptr),
+            _native.ctypes.c_void_p(prompts_ptr), _native.ctypes.c_void_p(randoms_ptr),
+            _native.ctypes.c_void_p(mask_ptr), sums_buf.ptr if own_sums else _native.ctypes.c_void_p(sums_ptr),
+            None if images_ptr is None else _native.ctypes.c_void_p(images_ptr)), 'sss_tail_fit_noalloc')
+        if not own_sums:
+            return None
+        sums = sums_buf.download(np.empty(2, np.float64))
+    finally:
+        if sums_buf is not None:
+            sums_buf.free()
+        if geometry_buffers is None:
+            for b in bufs:
+                b.free()
+    scale = sums[0] / sums[1] if sums[1] != 0 else 0.0
+    return scale, float(sums[0]), float(sums[1])
+
+
+def fit_geometry_buffers(Geometry, SinogramCoordinates, entries, *, context=None):
+    """Device copies of the geometry (f64), the coordinates (int16) and the
+    fit entries (int32 (n, 3)) for ``tail_fit_device``."""
+    ctx = context or _native.default_context()
+    geom = np.ascontiguousarray(np.asarray(Geometry, dtype=np.float64))
+    lor = np.ascontiguousarray(np.asarray(SinogramCoordinates, dtype=np.int16))
+    ent = np.ascontiguousarray(np.asarray(entries, dtype=np.int32).reshape(-1, 3))
+    return (_native.DeviceBuffer.from_array(ctx, geom), _native.DeviceBuffer.from_array(ctx, lor),
+            _native.DeviceBuffer.from_array(ctx, ent if ent.size else np.zeros(3, np.int32)))

@@ -264,6 +264,11 @@ hipError_t launch_fit(const Grid& g, const double* geom, int n_rings_geom, int n
                       const int32_t* plane_rings, int n_value_planes, int accel, int length_weighted,
                       const float* scatter, const float* prompts, const float* randoms, const float* mask,
                       FitScratch scratch, double* sums, double* images_out, hipStream_t st);
+hipError_t launch_forward_project(const Grid& g, const double* lam, const double* mu, const double* geom, int n_det,
+                                  const int16_t* lor, int n_planes, const int32_t* plane_rings, int accel,
+                                  float* out, hipStream_t st);
+hipError_t launch_poisson(const float* mean, int64_t n, double scale, uint64_t seed, float* out, hipStream_t st);
+
 // Device address of this library's no-allocation fit scratch on the current
 // device (looked up once per device at context creation).
 int noalloc_scratch_init(int device);

@@ -1364,6 +1364,72 @@ int sss_tail_mask(sss_ctx* ctx, const int32_t grid[3], const double bounds[6], c
     return SSS_OK;
 }
 
+int sss_forward_project(sss_ctx* ctx, const int32_t grid[3], const double bounds[6], const double* lam,
+                        const double* mu, int n_rings_geom, int n_det, const double* geom, const int16_t* lor_coords,
+                        int n_planes, const int32_t* plane_rings, int accel, float* trues_out)
+{
+    if (!ctx || !lam || !geom || !lor_coords || (n_planes > 0 && !plane_rings) || !trues_out || n_det < 2 ||
+        n_rings_geom < 1 || n_planes < 0 || accel < 1)
+        return fail(SSS_ERR_INVALID, "sss_forward_project: bad argument");
+    Grid g;
+    int rc = check_grid(grid, bounds, &g);
+    if (rc) return rc;
+    if ((rc = check_planes(plane_rings, n_planes, 2, n_rings_geom, 0))) return rc;
+    SSS_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const size_t nvox = (size_t)g.nx * g.ny * g.nz;
+    const size_t ngeom = (size_t)n_rings_geom * n_det * 3, nlor = (size_t)n_det * n_det * 2;
+    const size_t nout = (size_t)(n_det + 1) * (n_det / 2) * n_planes;
+    double *d_lam = nullptr, *d_mu = nullptr, *d_geom = nullptr;
+    int16_t* d_lor = nullptr;
+    int32_t* d_pr = nullptr;
+    float* d_out = nullptr;
+    int64_t tally = 0;
+    if ((rc = dev_alloc(&d_lam, nvox, &tally)) || (mu && (rc = dev_alloc(&d_mu, nvox, &tally))) ||
+        (rc = dev_alloc(&d_geom, ngeom, &tally)) || (rc = dev_alloc(&d_lor, nlor, &tally)) ||
+        (rc = dev_alloc(&d_pr, (size_t)2 * n_planes + 1, &tally)) || (rc = dev_alloc(&d_out, nout + 1, &tally))) {
+        dev_free(d_lam); dev_free(d_mu); dev_free(d_geom); dev_free(d_lor); dev_free(d_pr); dev_free(d_out);
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(d_lam, lam, nvox * sizeof(double), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && mu) e = hipMemcpyAsync(d_mu, mu, nvox * sizeof(double), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_geom, geom, ngeom * sizeof(double), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_lor, lor_coords, nlor * sizeof(int16_t), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && n_planes > 0)
+        e = hipMemcpyAsync(d_pr, plane_rings, (size_t)2 * n_planes * sizeof(int32_t), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemsetAsync(d_out, 0, nout * sizeof(float), st);
+    if (e == hipSuccess)
+        e = launch_forward_project(g, d_lam, d_mu, d_geom, n_det, d_lor, n_planes, d_pr, accel, d_out, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(trues_out, d_out, nout * sizeof(float), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    dev_free(d_lam); dev_free(d_mu); dev_free(d_geom); dev_free(d_lor); dev_free(d_pr); dev_free(d_out);
+    if (e != hipSuccess) return hip_fail(e, "sss_forward_project");
+    return SSS_OK;
+}
+
+int sss_poisson(sss_ctx* ctx, const float* mean, int64_t n, double scale, uint64_t seed, float* out)
+{
+    if (!ctx || n < 0 || (n > 0 && (!mean || !out)) || !(scale >= 0.0))
+        return fail(SSS_ERR_INVALID, "sss_poisson: bad argument");
+    if (n == 0) return SSS_OK;
+    SSS_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    float *d_in = nullptr, *d_out = nullptr;
+    int64_t tally = 0;
+    int rc;
+    if ((rc = dev_alloc(&d_in, (size_t)n, &tally)) || (rc = dev_alloc(&d_out, (size_t)n, &tally))) {
+        dev_free(d_in); dev_free(d_out);
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(d_in, mean, (size_t)n * sizeof(float), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_poisson(d_in, n, scale, seed, d_out, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    dev_free(d_in); dev_free(d_out);
+    if (e != hipSuccess) return hip_fail(e, "sss_poisson");
+    return SSS_OK;
+}
+
 int sss_tail_fit(sss_ctx* ctx, const int32_t grid[3], const double bounds[6], int n_rings_geom, int n_det,
                  const double* geom, const int16_t* lor_coords, int n_entries, const int32_t* plane_rings,
                  int n_value_planes, int accel, int length_weighted, const float* scatter,

@@ -119,8 +119,11 @@ def _workload(name, n_rings, n_det, ring_pitch, shape, voxel, n_sp, seed, nru, n
 
 
 def config1(seed=1):
-    """Plain non-TOF SSS: 32 det x 8 rings, 48x48x24 of 8x8x10.7 mm, 500 sp, B = 1."""
-    return _workload('sss-nontof-32x8-48x48x24-500sp', 8, 32, 3.2, (48, 48, 24),
+    """Plain non-TOF SSS: 32 sampled detectors x 8 rings, 48x48x24 of 8x8x10.7 mm,
+    500 sp, B = 1.  The scanner has 192 detectors per ring (a 193 x 96 x 64
+    sinogram, the layout of the tail fit's prompts); the SSS samples every
+    sixth detector, at the angles of a 32-detector ring."""
+    return _workload('sss-nontof-32x8-48x48x24-500sp', 8, 192, 3.2, (48, 48, 24),
                      (0.8, 0.8, 1.07), 500, seed, 8, 32, 1, 1e9, 1000.0)
 
 
