@@ -37,12 +37,17 @@ def parse():
     p.add_argument('--no-fp64-check', action='store_true')
     p.add_argument('--cpu-threads', type=int, default=0)
     p.add_argument('--no-stages', action='store_true', help='skip the full-size sinogram stages (config 2)')
+    p.add_argument('--default-batch', type=int, default=64, metavar='N',
+                   help='the default line\'s batched reading: N uneven random instances (0: none)')
+    p.add_argument('--default-serve', type=float, default=120.0, metavar='SECONDS',
+                   help='the default line\'s served reading: a Poisson stream of this length (0: none)')
     p.add_argument('--calibrate-copy', action='store_true',
                    help='run only a 1 GiB device copy (3 launches) to scale FETCH_SIZE/WRITE_SIZE')
     p.add_argument('--batch', type=int, default=0, metavar='N',
                    help='instead of the default line: N uneven random instances (seeds 0..N-1), '
                         'the one-instance path against the batched path')
-    p.add_argument('--streams', type=int, default=4, help='contexts of the batched path')
+    p.add_argument('--streams', type=int, default=4, help='contexts of the served path (and of --batch)')
+    p.add_argument('--batch-streams', type=int, default=2, help='contexts of the default line\'s batched reading')
     p.add_argument('--batch-kind', default='random', choices=['random', 'config1'],
                    help='uneven random instances (seeds 0..N-1) or config 1 instances of N seeds')
     p.add_argument('--serve', type=float, default=0.0, metavar='SECONDS',
@@ -65,27 +70,28 @@ def instance_evals(w):
     return len(w.scatter_points) * lors * w.n_bins
 
 
-def batch_bench(args):
-    """Batched/served reading: N uneven instances through the one-instance path
-    (create, trace, scatter, download, destroy, one at a time) and through
-    opensss_amd.batch.run_batch (several in flight on several contexts, device
-    blocks reused); both outputs compared bit for bit."""
-    from opensss_amd import phantoms
+def batch_reading(kind, n, streams, fp64_sample=4):
+    """Batched reading (config 4 with kind 'random': n uneven instances of
+    seeds 0..n-1): the one-instance path (create, trace, scatter, download,
+    destroy, one at a time) against opensss_amd.batch.run_batch (several in
+    flight on `streams` contexts, device blocks reused); outputs compared bit
+    for bit, and the FP32 outputs of `fp64_sample` instances against FP64
+    runs."""
+    from opensss_amd import _native, phantoms
     from opensss_amd.batch import run_batch, run_plain
     water = np.load(os.path.join(ROOT, 'opensss_amd', 'data', 'attenuation_water.npy'))
-    if args.batch_kind == 'config1':
-        ws = [phantoms.config1(seed=seed) for seed in range(args.batch)]
+    if kind == 'config1':
+        ws = [phantoms.config1(seed=seed) for seed in range(n)]
     else:
-        ws = [phantoms.random_instance(seed) for seed in range(args.batch)]
+        ws = [phantoms.random_instance(seed) for seed in range(n)]
     evals = sum(instance_evals(w) for w in ws)
-    print(f'[bench] batch of {len(ws)} instances, {evals:.3e} evals', file=sys.stderr, flush=True)
+    print(f'[bench] batch of {len(ws)} {kind} instances, {evals:.3e} evals', file=sys.stderr, flush=True)
     run_plain(ws[:1], water)   # library and device warm-up
     t0 = time.perf_counter()
     plain = run_plain(ws, water)
     t_plain = time.perf_counter() - t0
     print(f'[bench] one-instance path {t_plain:.3f} s', file=sys.stderr, flush=True)
-    from opensss_amd import _native
-    ctxs = [_native.Context(0) for _ in range(max(1, args.streams))]
+    ctxs = [_native.Context(0) for _ in range(max(1, streams))]
     run_batch(ws[:2 * len(ctxs)], water, contexts=ctxs)   # contexts and their caches warm, as the default one
     t0 = time.perf_counter()
     batched = run_batch(ws, water, contexts=ctxs)
@@ -94,48 +100,60 @@ def batch_bench(args):
         c.close()
     print(f'[bench] batched path {t_batch:.3f} s', file=sys.stderr, flush=True)
     same = all(np.array_equal(a, b) for a, b in zip(plain, batched))
-    line = {
-        'metric': 'batched scatter-pt x LOR x TOF-bin evals/s over uneven instances', 'value': evals / t_batch,
-        'unit': 'evals/s', 'n_gpus': 1, 'higher_is_better': True, 'dtype': 'f32', 'data': 'synthetic',
-        'config': {'workload': f'{args.batch_kind}-instances-0..{len(ws) - 1}', 'instances': len(ws),
-                   'evals': int(evals),
-                   'streams': args.streams},
+    errs = []
+    for i in range(0, len(ws), max(1, len(ws) // max(1, fp64_sample)))[:fp64_sample]:
+        ref = run_plain([ws[i]], water, precision='fp64')[0]
+        if np.abs(ref).max() > 0:
+            errs.append(rel_err_stats(batched[i], ref))
+    return {
+        'workload': f'{kind}-instances-0..{len(ws) - 1}', 'instances': len(ws), 'evals': int(evals),
+        'streams': streams,
+        'value': evals / t_batch, 'unit': 'evals/s',
         'one_instance_path': {'value': evals / t_plain, 'seconds': t_plain},
         'batched_path': {'value': evals / t_batch, 'seconds': t_batch},
         'speedup': t_plain / t_batch, 'outputs_identical': bool(same),
+        'max_rel_err_vs_fp64': max(e['max_rel_err_cells_above_1pct_peak'] for e in errs) if errs else None,
+        'fp64_checked_instances': len(errs),
     }
+
+
+def batch_bench(args):
+    r = batch_reading(args.batch_kind, args.batch, args.streams)
+    line = {'metric': 'batched scatter-pt x LOR x TOF-bin evals/s over uneven instances', 'n_gpus': 1,
+            'higher_is_better': True, 'dtype': 'f32', 'data': 'synthetic', **r}
     print(json.dumps(line))
 
 
-def serve_bench(args):
-    """Served reading: Poisson arrivals at --rate for --serve seconds, each
-    request a subject (uniform over --subjects) and its activity times
-    (1 + 0.05 N(0,1)) per voxel.  Subjects are loaded (created and traced)
-    before the stream starts and stay resident.  Reports completed
-    evaluations per second over the stream, request latency (arrival to
-    result in host memory) p50/p99, the one-instance path's rate on the
-    first requests, and the FP32 error against FP64 runs of sampled requests."""
+def serve_reading(kind, n_subjects, rate, seconds, streams, autotune=False, noise_fields=3, fp64_sample=4):
+    """Served reading (config 5 with kind 'random'): Poisson arrivals at
+    `rate` for `seconds`, each request a subject (uniform over `n_subjects`)
+    and its activity times (1 + 0.05 N(0,1)) per voxel.  Subjects are loaded
+    (created and traced) before the stream starts and stay resident.
+    Reports completed evaluations and requests per second over the stream,
+    request latency (arrival to result in host memory) p50/p99, the
+    one-instance path's rate on the first requests, and the FP32 error
+    against FP64 runs of sampled requests."""
     from opensss_amd import phantoms
     from opensss_amd.batch import run_plain
     from opensss_amd.serve import Request, ScatterServer, serve
     water = np.load(os.path.join(ROOT, 'opensss_amd', 'data', 'attenuation_water.npy'))
     rng = np.random.default_rng(2024)
-    if args.subject_kind == 'config1':
-        subs = {k: phantoms.config1(seed=k) for k in range(args.subjects)}
+    if kind == 'config1':
+        subs = {k: phantoms.config1(seed=k) for k in range(n_subjects)}
     else:
-        subs = {k: phantoms.random_instance(k) for k in range(args.subjects)}
+        subs = {k: phantoms.random_instance(k) for k in range(n_subjects)}
     fields = {k: [w.activity * (1.0 + 0.05 * rng.standard_normal(w.activity.shape))
-                  for _ in range(max(1, args.noise_fields))] for k, w in subs.items()}
-    n_req = int(rng.poisson(args.rate * args.serve))
-    arrivals = np.sort(rng.uniform(0.0, args.serve, size=n_req))
-    keys = rng.integers(0, args.subjects, size=n_req)
+                  for _ in range(max(1, noise_fields))] for k, w in subs.items()}
+    n_req = int(rng.poisson(rate * seconds))
+    arrivals = np.sort(rng.uniform(0.0, seconds, size=n_req))
+    keys = rng.integers(0, n_subjects, size=n_req)
     reqs = [Request(key=int(k), activity=fields[int(k)][i % len(fields[int(k)])], arrival=float(t), tag=i)
             for i, (t, k) in enumerate(zip(arrivals, keys))]
     evals = {k: instance_evals(w) for k, w in subs.items()}
-    print(f'[bench] serve: {n_req} requests over {args.serve:.0f} s, {args.subjects} subjects', file=sys.stderr,
+    print(f'[bench] serve: {n_req} requests over {seconds:.0f} s, {n_subjects} subjects', file=sys.stderr,
           flush=True)
 
-    server = ScatterServer(subs, water, n_contexts=args.streams, autotune=args.autotune)
+    server = ScatterServer(subs, water, n_contexts=streams, autotune=autotune)
     t0 = time.perf_counter()
     for k in subs:
         server.load(k)
@@ -144,14 +162,14 @@ def serve_bench(args):
     t_load = time.perf_counter() - t0
     # warm-up: one request per subject (not timed)
     serve(server, [Request(key=k, activity=subs[k].activity) for k in subs], on_result=lambda r, v: None)
-    sample = set(range(0, n_req, max(1, n_req // 4)))
+    sample = set(range(0, n_req, max(1, n_req // max(1, fp64_sample))))
     kept = {}
 
     def on_result(req, view):
         if req.tag in sample:
             kept[req.tag] = view.copy()
 
-    start = serve(server, reqs, on_result=on_result)
+    serve(server, reqs, on_result=on_result)
     t_end = max(r.done for r in reqs)
     server_stats = dict(server.stats)
     resident = server.resident_bytes()
@@ -159,12 +177,11 @@ def serve_bench(args):
     lat = np.array([r.done - r.arrival for r in reqs])
     total = sum(evals[r.key] for r in reqs)
     served_rate = total / t_end
+    print(f'[bench] served {n_req / t_end:.1f} req/s, p99 {np.percentile(lat, 99) * 1e3:.0f} ms', file=sys.stderr,
+          flush=True)
     # the one-instance path (create, trace, scatter, download, destroy) on the first requests
     head = reqs[:min(16, n_req)]
-    plain_inst = []
-    for r in head:
-        w = subs[r.key]
-        plain_inst.append(dataclasses.replace(w, activity=r.activity))
+    plain_inst = [dataclasses.replace(subs[r.key], activity=r.activity) for r in head]
     run_plain(plain_inst[:1], water)
     t0 = time.perf_counter()
     run_plain(plain_inst, water)
@@ -176,27 +193,33 @@ def serve_bench(args):
         r = reqs[i]
         w = dataclasses.replace(subs[r.key], activity=r.activity)
         ref = run_plain([w], water, precision='fp64')[0]
-        errs.append(rel_err_stats(kept[i], ref))
-    line = {
-        'metric': 'served scatter-pt x LOR x TOF-bin evals/s; p50/p99 request latency',
-        'value': served_rate, 'unit': 'evals/s', 'n_gpus': 1, 'higher_is_better': True, 'dtype': 'f32',
-        'data': 'synthetic',
-        'config': {'workload': f'{args.subject_kind}-subjects-0..{args.subjects - 1}', 'requests': n_req,
-                   'offered_rate_per_s': args.rate, 'seconds': args.serve, 'contexts': args.streams,
-                   'autotune': bool(args.autotune),
-                   'noise': '5% multiplicative per voxel'},
+        if np.abs(ref).max() > 0:
+            errs.append(rel_err_stats(kept[i], ref))
+    saturated = bool(n_req / t_end < 0.95 * rate)
+    return {
+        'workload': f'{kind}-subjects-0..{n_subjects - 1}', 'requests': n_req, 'offered_rate_per_s': rate,
+        'seconds': seconds, 'contexts': streams, 'autotune': bool(autotune),
+        'noise': '5% multiplicative per voxel',
+        'value': served_rate, 'unit': 'evals/s',
         'completed_per_s': n_req / t_end,
         'latency_ms': {'p50': float(np.percentile(lat, 50) * 1e3), 'p99': float(np.percentile(lat, 99) * 1e3),
                        'max': float(lat.max() * 1e3)},
         'subject_load_s': t_load, 'resident_bytes': int(resident), 'server': server_stats,
         'one_instance_path': {'value': plain_rate, 'requests': len(head), 'seconds': t_plain},
         # below saturation the served rate is the offered load, not a capacity
-        'saturated': bool(n_req / t_end < 0.95 * args.rate),
-        'speedup': served_rate / plain_rate if n_req / t_end < 0.95 * args.rate else None,
+        'saturated': saturated,
+        'ratio_to_one_instance_path': served_rate / plain_rate,
         'max_rel_err_vs_fp64': max(e['max_rel_err_cells_above_1pct_peak'] for e in errs) if errs else None,
         'mean_rel_err_vs_fp64': float(np.mean([e['mean_rel_err_cells_above_1pct_peak'] for e in errs]))
         if errs else None,
     }
+
+
+def serve_bench(args):
+    r = serve_reading(args.subject_kind, args.subjects, args.rate, args.serve, args.streams, args.autotune,
+                      args.noise_fields)
+    line = {'metric': 'served scatter-pt x LOR x TOF-bin evals/s; p50/p99 request latency', 'n_gpus': 1,
+            'higher_is_better': True, 'dtype': 'f32', 'data': 'synthetic', **r}
     print(json.dumps(line))
 
 
@@ -552,6 +575,12 @@ def main():
         print(f"[bench] FP32 vs oracle: max rel err {err_oracle['max_rel_err_cells_above_1pct_peak']:.3e}",
               file=sys.stderr, flush=True)
 
+    batched = served = None
+    if args.workload == 'config2' and args.default_batch > 0:
+        batched = batch_reading('random', args.default_batch, args.batch_streams)
+    if args.workload == 'config2' and args.default_serve > 0:
+        served = serve_reading('random', args.subjects, args.rate, args.default_serve, args.streams)
+
     stages = None
     if w.name.startswith('tofsss-760x80') and not args.no_stages and not args.no_fp64_check:
         stages = sinogram_stages(w, out32, out64, oracle_pair)
@@ -577,6 +606,8 @@ def main():
         'roofline': roofline,
         'cpu_baseline': cpu,
         'sinogram_stages': stages,
+        'batched': batched,
+        'served': served,
     }
     print(json.dumps(line))
 

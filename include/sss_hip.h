@@ -156,7 +156,9 @@ int sss_subject_trace(sss_subject* s);
 /* sss_subject_trace in two halves, so a caller with several subjects on
  * several contexts waits once for all of them: begin queues the walk counts
  * (asynchronous); end waits for this subject's counts, sizes the sample
- * tables and queues the walks (asynchronous after that wait). */
+ * tables and queues the walks (asynchronous after that wait).  When end
+ * returns SSS_ERR_NOMEM the trace stays begun: free device memory (other
+ * subjects, sss_ctx_release_cache) and call end again. */
 int sss_subject_trace_begin(sss_subject* s);
 int sss_subject_trace_end(sss_subject* s);
 /* Upload lam (X*Y*Z f64, host, in the subject's volume_order) and refresh

@@ -42,24 +42,72 @@ def _collect(s, addr):
     return out
 
 
-def run_batch(instances, attenuation_table, n_streams=4, precision='fp32', contexts=None):
+SSS_ERR_NOMEM = -3
+
+
+def _out_of_memory(e):
+    return isinstance(e, _native.SssError) and e.code == SSS_ERR_NOMEM
+
+
+def run_batch(instances, attenuation_table, n_streams=4, precision='fp32', contexts=None,
+              memory_budget=None):
     """Compact outputs (B, NRU^2, NDU(NDU+1)/2) of every instance, in order.
 
     n_streams contexts take the instances round-robin; up to 2 x n_streams + 1
-    instances hold device memory at once.  An instance's download is queued
-    right behind its scatter into pinned memory, so collecting it waits for
-    that instance only.
+    instances hold device memory at once, fewer when their estimated
+    footprints (``serve.footprint_estimate``) would pass `memory_budget`
+    bytes (default: 60% of the free device memory at the start); an
+    allocation that still fails releases the contexts' cached blocks and is
+    retried.  An
+    instance's download is queued right behind its scatter into pinned
+    memory, so collecting it waits for that instance only.
     """
+    from .serve import footprint_estimate
     own = contexts is None
     ctxs = contexts or [_native.Context(0) for _ in range(max(1, int(n_streams)))]
     lag = len(ctxs)
     queued, running = deque(), deque()
     outs = [None] * len(instances)
+    held = {}   # instance -> estimated device bytes while it is queued or running
+    if memory_budget is None:
+        memory_budget = int(0.6 * ctxs[0].memory()[0])
+
+    def retire_one():
+        if not running:
+            advance()
+        k, s, addr = running.popleft()
+        outs[k] = _collect(s, addr)
+        held.pop(k, None)
+
+    def drain():
+        while running:
+            retire_one()
+
+    def make_room(need):
+        """Collect the oldest instances while the held estimates and this one
+        pass the budget."""
+        while sum(held.values()) + need > memory_budget and (queued or running):
+            retire_one()
+
+    def with_memory(fn):
+        """fn(); when device memory runs out, the contexts' cached blocks are
+        released and fn retried, then once more after every running
+        instance has been collected."""
+        for attempt in range(3):
+            try:
+                return fn()
+            except _native.SssError as e:
+                if not _out_of_memory(e) or attempt == 2:
+                    raise
+            if attempt == 1:
+                drain()
+            for c in ctxs:
+                c.release_cache()
 
     def advance():
         k, s = queued.popleft()
         try:
-            s.trace_end()
+            with_memory(s.trace_end)
             s.scatter()
             addr = s.ctx.host_alloc(s.output_nbytes)
         except BaseException:
@@ -75,20 +123,21 @@ def run_batch(instances, attenuation_table, n_streams=4, precision='fp32', conte
 
     try:
         for i, inst in enumerate(instances):
-            s = _subject(inst, attenuation_table, ctxs[i % lag], precision)
+            need = footprint_estimate(inst)
+            make_room(need)
+            s = with_memory(lambda: _subject(inst, attenuation_table, ctxs[i % lag], precision))
+            held[i] = need
             s.set_activity(inst.activity)
             s.trace_begin()
             queued.append((i, s))
             if len(queued) > lag:
                 advance()
             if len(running) > lag:
-                k, s, addr = running.popleft()
-                outs[k] = _collect(s, addr)
+                retire_one()
         while queued:
             advance()
         while running:
-            k, s, addr = running.popleft()
-            outs[k] = _collect(s, addr)
+            retire_one()
     finally:
         # on an error, instances still queued or running are released: a
         # running one's download targets its pinned block, so it is waited

@@ -36,7 +36,8 @@ int fail(int code, const std::string& msg)
 int hip_fail(hipError_t e, const char* what)
 {
     g_error = std::string(what) + ": " + hipGetErrorString(e);
-    return SSS_ERR_HIP;
+    // a caller can free memory and retry an out-of-memory failure
+    return e == hipErrorOutOfMemory ? SSS_ERR_NOMEM : SSS_ERR_HIP;
 }
 
 namespace {
@@ -419,6 +420,7 @@ int sss_ctx_create(int device, sss_ctx** out)
         return hip_fail(e, "sss_ctx_create");
     }
     for (int i = kPinnedSlots - 1; i >= 0; --i) c->pinned_free.push_back(i);
+
     if (int rc = noalloc_scratch_init(device)) {
         (void)hipHostFree(c->pinned);
         (void)hipStreamDestroy(c->stream);
@@ -916,10 +918,10 @@ int sss_subject_trace_end(sss_subject* s)
 {
     if (!s) return fail(SSS_ERR_INVALID, "sss_subject_trace_end: NULL subject");
     if (!s->trace_begun) return fail(SSS_ERR_INVALID, "sss_subject_trace_end: call sss_subject_trace_begin first");
-    s->trace_begun = false;
     SSS_HIP(hipSetDevice(s->ctx->device));
     hipStream_t st = s->ctx->stream;
     if (s->n_hp == 0) {
+        s->trace_begun = false;
         s->n_samples = 0;
         s->traced = true;
         return SSS_OK;
@@ -966,6 +968,10 @@ int sss_subject_trace_end(sss_subject* s)
     SSS_HIP(hipMemsetAsync(s->d_flag, 0, sizeof(int32_t), st));
     hipLaunchKernelGGL(k_count_status, dim3(256), dim3(256), 0, st, s->hp.status, s->n_hp, s->d_flag);
     SSS_HIP(hipGetLastError());
+    // the walks are queued: a failure before this point (device memory)
+    // leaves the trace begun, so the caller may free memory and call
+    // sss_subject_trace_end again
+    s->trace_begun = false;
     s->traced = true;
     if (s->has_activity) return refresh_fold(s);
     return SSS_OK;

@@ -38,6 +38,8 @@ import numpy as np
 from . import _native
 from .sss import ScatterSubject
 
+SSS_ERR_NOMEM = -3
+
 
 @dataclass
 class Request:
@@ -57,7 +59,7 @@ def footprint_estimate(inst) -> int:
     X, Y, Z = (int(v) for v in inst.attenuation.shape)
     n_vox = X * Y * Z
     nru, ndu = int(inst.n_rings_used), int(inst.n_det_used)
-    n_hp = len(inst.scatter_points) * nru * ndu
+    n_hp = (0 if inst.scatter_points is None else len(inst.scatter_points)) * nru * ndu
     walk = (X + Y + Z) // 2
     out = int(inst.n_bins) * nru * nru * ndu * (ndu + 1) // 2 * 4
     return 4 * n_vox * 8 + n_hp * 120 + n_hp * walk * 48 + 3 * out
@@ -144,12 +146,27 @@ class ScatterServer:
         if self.autotune and key not in self.tile_rings:
             self.tile_rings[key] = tune_tile_rings(inst, self.table, precision=self.precision,
                                                    context=self._context(key))[0]
-        s = ScatterSubject(inst.attenuation, inst.image_size, inst.geometry, inst.normals, inst.det_size,
-                           self.table, inst.energy_resolution, inst.energy_threshold, inst.n_rings_used,
-                           inst.n_det_used, inst.tof_resolution, inst.tof_range, inst.n_bins,
-                           scatter_points=inst.scatter_points, precision=self.precision,
-                           context=self._context(key), tile_rings=self.tile_rings.get(key, 0))
-        s.trace()
+        while True:
+            s = None
+            try:
+                s = ScatterSubject(inst.attenuation, inst.image_size, inst.geometry, inst.normals, inst.det_size,
+                                   self.table, inst.energy_resolution, inst.energy_threshold, inst.n_rings_used,
+                                   inst.n_det_used, inst.tof_resolution, inst.tof_range, inst.n_bins,
+                                   scatter_points=inst.scatter_points, precision=self.precision,
+                                   context=self._context(key), tile_rings=self.tile_rings.get(key, 0))
+                s.trace()
+                break
+            except _native.SssError as e:
+                if s is not None:
+                    s.close()
+                # device memory ran out below the estimate: drop the
+                # contexts' cached blocks and an idle subject, then retry
+                if e.code != SSS_ERR_NOMEM:
+                    raise
+                for c in self.ctxs:
+                    c.release_cache()
+                if not self._evict_one():
+                    raise MemoryError(f'subject {key!r} does not fit in device memory') from e
         self.resident[key] = (s, int(s.info['device_bytes']))
         self.stats['loaded'] += 1
         return s

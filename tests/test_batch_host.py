@@ -15,6 +15,12 @@ class Ctx:
     def __init__(self):
         self.blocks = {}
 
+    def memory(self):
+        return 1 << 40, 1 << 40
+
+    def release_cache(self):
+        pass
+
     def host_alloc(self, nbytes):
         buf = ctypes.create_string_buffer(max(1, nbytes))
         self.blocks[ctypes.addressof(buf)] = buf
@@ -112,3 +118,41 @@ def test_run_batch_error_releases_pinned_blocks(monkeypatch):
         batch.run_batch(insts, None, contexts=ctxs)
     assert Subject.alive == 0
     assert all(not c.blocks for c in ctxs)
+
+
+class ShortOfMemory(Subject):
+    """trace_end runs out of device memory once per subject of mu value 3."""
+    failed = set()
+
+    def trace_end(self):
+        from opensss_amd import _native
+        if self.key == 3.0 and self.key not in ShortOfMemory.failed:
+            ShortOfMemory.failed.add(self.key)
+            raise _native.SssError(-3, 'sss_subject_trace_end', 'hipMalloc failed: out of memory')
+        super().trace_end()
+
+
+class CacheCtx(Ctx):
+    def __init__(self):
+        super().__init__()
+        self.released = 0
+
+    def release_cache(self):
+        self.released += 1
+
+
+def test_run_batch_releases_caches_and_retries_when_memory_runs_out(monkeypatch):
+    monkeypatch.setattr(batch, 'ScatterSubject', ShortOfMemory)
+    Subject.alive = Subject.peak = 0
+    Subject.contexts = []
+    ShortOfMemory.failed = set()
+    insts = [SimpleNamespace(attenuation=np.full((2, 2, 2), float(i + 1)), activity=np.full((2, 2, 2), 1.0),
+                             image_size=None, geometry=None, normals=None, det_size=None,
+                             energy_resolution=0.1, energy_threshold=435.0, n_rings_used=1, n_det_used=2,
+                             tof_resolution=214.0, tof_range=3335.64, n_bins=1, scatter_points=None)
+             for i in range(6)]
+    ctxs = [CacheCtx(), CacheCtx()]
+    outs = batch.run_batch(insts, None, contexts=ctxs)
+    assert all(np.all(o == np.float32((i + 1) * 8.0)) for i, o in enumerate(outs))
+    assert all(c.released == 1 for c in ctxs)
+    assert Subject.alive == 0
