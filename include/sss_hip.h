@@ -22,6 +22,8 @@
  *   sss_tail_mask          Python/openSSS/TailsMask.py:28-376 (the per-LOR mask loops)
  *   sss_tail_fit           Python/openSSS/BackProjection.py:31-392 (back-projections
  *                          and the scale of Scaling.py:27-121)
+ *   sss_scale_to_prompts   Python/openSSS/Scaling.py:60-113 (every TOF bin,
+ *                          with the LORCounts division and the mask threshold)
  */
 #ifndef SSS_HIP_H
 #define SSS_HIP_H
@@ -238,6 +240,38 @@ int sss_tail_fit(sss_ctx* ctx, const int32_t grid[3], const double bounds[6], in
                  int n_value_planes, int accel, int length_weighted, const float* scatter,
                  const float* prompts, const float* randoms, const float* mask, double* sums_out,
                  double* images_out);
+
+/* The scaling loop over every TOF bin (Python/openSSS/Scaling.py:60-113:
+ * Prompts[..., b] / LORCounts, Randoms * Normalization, float32(mask >= 1),
+ * then BackProjectionScatterRandFit or ...Mashing per bin) in one call.
+ * Geometry, entries and weighting are those of sss_tail_fit.  With cells
+ * c = (radial * N/2 + angular) * n_value_planes + plane:
+ *   scatter     float32 [n_bins][cells]: the per-bin interpolated sinograms
+ *   prompts     [cells][n_bins], uint16 (SSS_PROMPTS_U16, what the list-mode
+ *               histogrammer writes) or float32 (SSS_PROMPTS_F32)
+ *   lor_counts  float32 divisor of the prompts: NULL (none), one per plane
+ *               (lor_counts_per_plane != 0; SumLORsCounts is constant over a
+ *               plane's cells) or one per cell
+ *   randoms     float32 [cells] or NULL (0); norm float32 [cells] or NULL (1)
+ *   mask        uint8 [cells]; cells with mask >= 1 are fitted
+ * sums_out[2b], sums_out[2b+1] are bin b's two sums, the same bits as
+ * sss_tail_fit on that bin's sinograms with the prompts divided in float32
+ * beforehand.  Each LOR is traced once per group of 8 bins.  flags
+ * SSS_FIT_DEVICE: every array pointer (geom, lor_coords, plane_rings, the
+ * sinograms, sums_out) is device memory on the context's device and the call
+ * is asynchronous on the context's stream (entries out of range are skipped);
+ * otherwise they are host pointers and the call returns when sums_out is
+ * written.  Scratch: 32 * n_bins * X*Y*Z bytes from the context's cache. */
+#define SSS_PROMPTS_F32 0
+#define SSS_PROMPTS_U16 1
+#define SSS_FIT_DEVICE 1
+#define SSS_MAX_FIT_BINS 1024
+int sss_scale_to_prompts(sss_ctx* ctx, const int32_t grid[3], const double bounds[6], int n_rings_geom, int n_det,
+                         const double* geom, const int16_t* lor_coords, int n_entries, const int32_t* plane_rings,
+                         int n_value_planes, int accel, int length_weighted, int n_bins, const float* scatter,
+                         const void* prompts, int prompts_type, const float* lor_counts, int lor_counts_per_plane,
+                         const float* randoms, const float* norm, const uint8_t* mask, int flags,
+                         double* sums_out);
 
 /* Synthetic prompts (no counterpart in the reference, which reads measured
  * prompts; the per-LOR trace follows Python/openSSS/TailsMask.py:139-145).

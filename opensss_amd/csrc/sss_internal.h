@@ -264,6 +264,35 @@ hipError_t launch_fit(const Grid& g, const double* geom, int n_rings_geom, int n
                       const int32_t* plane_rings, int n_value_planes, int accel, int length_weighted,
                       const float* scatter, const float* prompts, const float* randoms, const float* mask,
                       FitScratch scratch, double* sums, double* images_out, hipStream_t st);
+// The sinograms of a tail fit over every TOF bin (device pointers, cell
+// c = (radial * N/2 + angular) * P + plane): bin b fits scatter[b * n_cells +
+// c] against prompts[c * n_bins + b] / counts (float32 division; counts null
+// = 1, one per plane, or one per cell), randoms[c] * norm[c] (null randoms =
+// 0, null norm = 1) and the cells with mask[c] >= 1.
+struct BinFitIn {
+    const float* scatter;
+    const void* prompts;
+    int prompts_u16;          // prompts are uint16 (else float32)
+    const float* counts;
+    int counts_per_plane;
+    const float* randoms;
+    const float* norm;
+    const uint8_t* mask;
+    int n_bins;
+    int64_t n_cells;
+    int n_value_planes;
+};
+constexpr size_t fit_bins_scratch_bytes(int64_t nvox, int n_bins)
+{
+    return (size_t)((4 * n_bins + 2) * nvox) * sizeof(unsigned long long) +
+           (size_t)(2 * kFitBlocks) * n_bins * sizeof(double) + (size_t)(3 * n_bins + 1) * sizeof(unsigned int) + 512;
+}
+FitScratch fit_bins_scratch_carve(void* base, int64_t nvox, int n_bins);
+// sums (device, 2 n_bins doubles): bin b's two sums at [2b], [2b + 1], the
+// same bits as launch_fit of that bin's sinograms (prompts already divided).
+hipError_t launch_fit_bins(const Grid& g, const double* geom, int n_rings_geom, int n_det, const int16_t* lor,
+                           int n_entries, const int32_t* plane_rings, int accel, int length_weighted,
+                           const BinFitIn& in, FitScratch scratch, double* sums, hipStream_t st);
 hipError_t launch_forward_project(const Grid& g, const double* lam, const double* mu, const double* geom, int n_det,
                                   const int16_t* lor, int n_planes, const int32_t* plane_rings, int accel,
                                   float* out, hipStream_t st);

@@ -1490,4 +1490,117 @@ int sss_tail_fit(sss_ctx* ctx, const int32_t grid[3], const double bounds[6], in
     return SSS_OK;
 }
 
+int sss_scale_to_prompts(sss_ctx* ctx, const int32_t grid[3], const double bounds[6], int n_rings_geom, int n_det,
+                         const double* geom, const int16_t* lor_coords, int n_entries, const int32_t* plane_rings,
+                         int n_value_planes, int accel, int length_weighted, int n_bins, const float* scatter,
+                         const void* prompts, int prompts_type, const float* lor_counts, int lor_counts_per_plane,
+                         const float* randoms, const float* norm, const uint8_t* mask, int flags,
+                         double* sums_out)
+{
+    if (!ctx || !geom || !lor_coords || (n_entries > 0 && !plane_rings) || !scatter || !prompts || !mask ||
+        !sums_out || n_det < 2 || n_rings_geom < 1 || n_entries < 0 || n_value_planes < 1 || accel < 1 ||
+        n_bins < 1 || n_bins > SSS_MAX_FIT_BINS ||
+        (prompts_type != SSS_PROMPTS_F32 && prompts_type != SSS_PROMPTS_U16) || (flags & ~SSS_FIT_DEVICE))
+        return fail(SSS_ERR_INVALID, "sss_scale_to_prompts: bad argument");
+    Grid g;
+    int rc = check_grid(grid, bounds, &g);
+    if (rc) return rc;
+    const bool on_device = (flags & SSS_FIT_DEVICE) != 0;
+    // device entries cannot be read here: the kernels skip out-of-range ones
+    if (!on_device && (rc = check_planes(plane_rings, n_entries, 3, n_rings_geom, n_value_planes))) return rc;
+    SSS_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
+    const size_t n_cells = (size_t)(n_det + 1) * (n_det / 2) * n_value_planes;
+    const size_t n_counts = lor_counts_per_plane ? (size_t)n_value_planes : n_cells;
+    const size_t psize = prompts_type == SSS_PROMPTS_U16 ? sizeof(uint16_t) : sizeof(float);
+    const size_t ngeom = (size_t)n_rings_geom * n_det * 3, nlor = (size_t)n_det * n_det * 2;
+
+    BinFitIn in{};
+    in.prompts_u16 = prompts_type == SSS_PROMPTS_U16;
+    in.counts_per_plane = lor_counts_per_plane ? 1 : 0;
+    in.n_bins = n_bins;
+    in.n_cells = (int64_t)n_cells;
+    in.n_value_planes = n_value_planes;
+    const double* d_geom = geom;
+    const int16_t* d_lor = lor_coords;
+    const int32_t* d_pr = plane_rings;
+    double* d_sums = sums_out;
+    unsigned char* d_scratch = nullptr;
+    // host mode: device copies of every input, from the context's cache
+    double *h_geom = nullptr, *h_sums = nullptr;
+    int16_t* h_lor = nullptr;
+    int32_t* h_pr = nullptr;
+    float *h_s = nullptr, *h_c = nullptr, *h_r = nullptr, *h_n = nullptr;
+    unsigned char* h_p = nullptr;
+    uint8_t* h_m = nullptr;
+    auto release = [&]() {
+        pool_free(ctx, h_geom); pool_free(ctx, h_sums); pool_free(ctx, h_lor); pool_free(ctx, h_pr);
+        pool_free(ctx, h_s); pool_free(ctx, h_c); pool_free(ctx, h_r); pool_free(ctx, h_n);
+        pool_free(ctx, h_p); pool_free(ctx, h_m); pool_free(ctx, d_scratch);
+    };
+    int64_t tally = 0;
+    if ((rc = pool_alloc(ctx, &d_scratch, fit_bins_scratch_bytes(nvox, n_bins), &tally))) {
+        release();
+        return rc;
+    }
+    hipError_t e = hipSuccess;
+    if (on_device) {
+        in.scatter = scatter;
+        in.prompts = prompts;
+        in.counts = lor_counts;
+        in.randoms = randoms;
+        in.norm = norm;
+        in.mask = mask;
+    } else {
+        if ((rc = pool_alloc(ctx, &h_geom, ngeom, &tally)) || (rc = pool_alloc(ctx, &h_lor, nlor, &tally)) ||
+            (rc = pool_alloc(ctx, &h_pr, (size_t)3 * n_entries + 1, &tally)) ||
+            (rc = pool_alloc(ctx, &h_sums, (size_t)2 * n_bins, &tally)) ||
+            (rc = pool_alloc(ctx, &h_s, n_cells * n_bins, &tally)) ||
+            (rc = pool_alloc(ctx, &h_p, n_cells * n_bins * psize, &tally)) ||
+            (lor_counts && (rc = pool_alloc(ctx, &h_c, n_counts, &tally))) ||
+            (randoms && (rc = pool_alloc(ctx, &h_r, n_cells, &tally))) ||
+            (randoms && norm && (rc = pool_alloc(ctx, &h_n, n_cells, &tally))) ||
+            (rc = pool_alloc(ctx, &h_m, n_cells, &tally))) {
+            release();
+            return rc;
+        }
+        auto up = [&](void* dst, const void* src, size_t bytes) {
+            if (e == hipSuccess && bytes) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+        };
+        up(h_geom, geom, ngeom * sizeof(double));
+        up(h_lor, lor_coords, nlor * sizeof(int16_t));
+        if (n_entries > 0) up(h_pr, plane_rings, (size_t)3 * n_entries * sizeof(int32_t));
+        up(h_s, scatter, n_cells * n_bins * sizeof(float));
+        up(h_p, prompts, n_cells * n_bins * psize);
+        if (h_c) up(h_c, lor_counts, n_counts * sizeof(float));
+        if (h_r) up(h_r, randoms, n_cells * sizeof(float));
+        if (h_n) up(h_n, norm, n_cells * sizeof(float));
+        up(h_m, mask, n_cells);
+        d_geom = h_geom;
+        d_lor = h_lor;
+        d_pr = h_pr;
+        d_sums = h_sums;
+        in.scatter = h_s;
+        in.prompts = h_p;
+        in.counts = h_c;
+        in.randoms = h_r;
+        in.norm = h_n;
+        in.mask = h_m;
+    }
+    if (e == hipSuccess)
+        e = launch_fit_bins(g, d_geom, n_rings_geom, n_det, d_lor, n_entries, d_pr, accel, length_weighted, in,
+                            fit_bins_scratch_carve(d_scratch, nvox, n_bins), d_sums, st);
+    if (!on_device) {
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(sums_out, d_sums, (size_t)2 * n_bins * sizeof(double), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+    }
+    // the blocks go back to the context's cache: their next user works on
+    // this stream, after the kernels queued above
+    release();
+    if (e != hipSuccess) return hip_fail(e, "sss_scale_to_prompts");
+    return SSS_OK;
+}
+
 }  // extern "C"

@@ -13,6 +13,8 @@ Functions mirror the reference entry points they restate:
 * ``line_integrals`` -> the per-pair trace + sums of ``TailsMask.py:139-148``
 * ``backproject``    -> ``BackProjection.py:31-113`` (mode 0) / ``:217-300`` (mode 1)
 * ``sinogram_coordinates_loops`` -> ``Sinogram.py:27-141`` as plain loops
+* ``tail_fit``       -> ``BackProjection.py:115-214`` / ``:302-392``
+* ``scale_scatters_to_prompts`` -> the scaling loop of ``Scaling.py:60-113``
 """
 from __future__ import annotations
 
@@ -445,3 +447,34 @@ def tail_fit(PromptSinogram, ScatterSinogram, RandomSinogram, MaskSinogram, Grid
     a, _, _, _ = np.linalg.lstsq(sbp[:, np.newaxis] * np.float32(1e10), tails, rcond=None)
     scale = a * np.float32(1e10)
     return scale, {'prompt': prompt_img, 'scatter': scatter_img, 'random': random_img, 'sens': sens}
+
+
+def scale_scatters_to_prompts(Prompts, Scatters, Randoms, Normalization, Mask, LORCounts, SinogramIndex,
+                              SinogramCoordinates, DesiredDimensions, FittingSize, Geometry, ExtendedGeometry,
+                              AccelerationFactor, SpanFlag):
+    """Restatement of the scaling loop of ``Scaling.py:60-113`` on in-memory
+    arrays: Randoms * Normalization (:69-71, zeros without randoms), float32
+    (mask >= 1) (:75-76), and per bin Prompts[..., b] / LORCounts (:90) into
+    the no-span (:92-98, mirrored plane order SinogramIndex.T) or span
+    (:99-104, extended geometry) fit.  Returns float32 (B,)."""
+    prompts = np.asarray(Prompts)
+    n_bins = prompts.shape[-1]
+    cell_shape = prompts.shape[:-1]
+    norm = np.ones(cell_shape, np.single) if Normalization is None else np.asarray(Normalization, np.float32)
+    randoms = (np.zeros(cell_shape, np.single) if Randoms is None
+               else np.asarray(Randoms, np.float32) * norm)
+    mask = np.float32(np.asarray(Mask) >= 1)
+    out = np.zeros(n_bins, dtype=np.float32)
+    for b in range(n_bins):
+        bin_prompts = prompts[:, :, :, b] / LORCounts
+        if not SpanFlag:
+            NR = np.asarray(SinogramIndex).shape[0]
+            order = np.asarray(SinogramIndex)[0:NR, 0:NR].T
+            f, _ = tail_fit(bin_prompts, Scatters[b], randoms, mask, DesiredDimensions, FittingSize, Geometry,
+                            SinogramCoordinates, AccelerationFactor, SinogramOrder=order)
+        else:
+            f, _ = tail_fit(bin_prompts, Scatters[b], randoms, mask, DesiredDimensions, FittingSize,
+                            ExtendedGeometry, SinogramCoordinates, AccelerationFactor,
+                            MashedSinogramIndices=SinogramIndex)
+        out[b] = f[0]
+    return out

@@ -219,3 +219,130 @@ def test_tail_fit_is_deterministic(span):
         assert r[0] == runs[0][0] and r[1] == runs[0][1] and r[2] == runs[0][2]
         assert np.array_equal(r[3], runs[0][3])
     assert runs[0][2] > 0
+
+
+def _scaling_inputs(w, span, n_bins, seed, u16):
+    """Per-bin scatter, prompts (N+1, N/2, P, B), randoms, norm, uint8 mask
+    (values 0..2) and the span case's per-plane LOR counts (4 and 8, mash 2)."""
+    from opensss_amd.data_reduction import SumLORsCounts
+    NR, N = w.geometry.shape[0], w.geometry.shape[1]
+    if span:
+        msi, _, lut, counts = SinogramLUT(NR, 3, 4)
+        P = msi.shape[0]
+        lor_counts = SumLORsCounts(w.geometry, lut, w.sinogram_index, 2, 3, counts.shape[0])[:, :, :P]
+        sino = msi
+    else:
+        P = NR * NR
+        lor_counts = SumLORsCounts(w.geometry, None, w.sinogram_index, 2, 1, P)
+        sino = w.sinogram_index
+    rng = np.random.default_rng(seed)
+    shape = (N + 1, N // 2, P)
+    S = rng.uniform(0.5, 2.0, size=(n_bins,) + shape).astype(np.float32)
+    if n_bins > 3:
+        S[3] = 0.0                                  # a bin with no scatter: its sums are 0
+    S[:, rng.uniform(size=shape) < 0.2] = 0.0       # LORs skipped in every bin
+    R = rng.uniform(0.0, 0.3, size=shape).astype(np.float32)
+    Nm = rng.uniform(0.8, 1.2, size=shape).astype(np.float32)
+    M = rng.integers(0, 3, size=shape).astype(np.uint8)
+    if u16:
+        prompts = rng.integers(0, 60, size=shape + (n_bins,)).astype(np.uint16)
+    else:
+        prompts = rng.uniform(0.0, 40.0, size=shape + (n_bins,)).astype(np.float32)
+    return sino, S, prompts, R, Nm, M, lor_counts
+
+
+@pytest.mark.parametrize('span,u16,counts_kind', [(False, True, 'plane'), (False, False, 'cell'),
+                                                   (True, True, 'cell'), (True, False, 'plane'),
+                                                   (True, True, None)])
+def test_scale_to_prompts_is_the_per_bin_fit_bit_for_bit(span, u16, counts_kind):
+    """sss_scale_to_prompts over 11 bins (two groups of 8, the second partial)
+    against sss_tail_fit per bin on the same sinograms with the prompts
+    divided by the LOR counts in numpy float32, the randoms multiplied by
+    the normalisation and the mask thresholded beforehand: the same sums,
+    bit for bit."""
+    from opensss_amd.backprojection import nospan_entries, span_entries, tail_fit
+    from opensss_amd.scaling import scale_to_prompts
+    w = _case()
+    NR = w.geometry.shape[0]
+    n_bins = 11
+    sino, S, prompts, R, Nm, M, lor_counts = _scaling_inputs(w, span, n_bins, 5, u16)
+    if span:
+        geom, _ = expandRings(w.geometry, w.normals)
+        entries = span_entries(sino)
+    else:
+        geom = w.geometry
+        entries = nospan_entries(w.sinogram_index[:NR, :NR].T, NR)
+    counts_arg = None if counts_kind is None else (lor_counts[0, 0, :] if counts_kind == 'plane' else lor_counts)
+    grid = np.array([24, 24, 12])
+    factors, sums = scale_to_prompts(prompts, S, R, Nm, M, counts_arg, sino, w.lor_coordinates, grid,
+                                     w.image_size, w.geometry, geom, 2, span, return_sums=True)
+    Mf = np.float32(M >= 1)
+    for b in range(n_bins):
+        pb = prompts[..., b] / (np.float32(1) if counts_kind is None else lor_counts)
+        scale, sxy, sxx = tail_fit(pb.astype(np.float32), S[b], R * Nm, Mf, grid, w.image_size, geom,
+                                   w.lor_coordinates, entries, 2, span)
+        assert sums[b, 0] == sxy and sums[b, 1] == sxx, b
+        assert factors[b] == np.float32(scale)
+    assert np.all(sums[3] == 0) and factors[3] == 0
+    assert np.all(sums[np.arange(n_bins) != 3, 1] > 0)
+
+
+def test_scale_scatters_to_prompts_recovers_k_and_matches_oracle(oracle, tmp_path):
+    """File-based drop-in on a span case with float32 prompts =
+    (k_b S_b + R Norm) LORCounts: every bin recovers k_b and agrees with the
+    oracle's scaling loop to 1e-5."""
+    from opensss_amd.scaling import ScaleScattersToPrompts
+    w = _case()
+    sino, S, _, R, Nm, M, lor_counts = _scaling_inputs(w, True, 3, 9, False)
+    ks = [0.75, 1.25, 2.5]
+    prompts = np.stack([((np.float64(k) * S[b] + R * Nm) * lor_counts).astype(np.float32)
+                        for b, k in enumerate(ks)], axis=-1)
+    geom, _ = expandRings(w.geometry, w.normals)
+    np.savez_compressed(tmp_path / 'PromptsSinogramMashed.npz', prompts)
+    np.savez_compressed(tmp_path / 'RandomsSinogram.npz', R)
+    np.savez_compressed(tmp_path / 'NormSinogram.npz', Nm)
+    np.save(tmp_path / 'AttenuationMask.npy', M)
+    for b in range(3):
+        np.savez_compressed(tmp_path / f'SSS_mashed_bin{b}.npz', S[b])
+    grid = np.array([12, 12, 6])
+    got = ScaleScattersToPrompts(sino, w.lor_coordinates, str(tmp_path), grid, w.image_size, w.geometry, geom, 2,
+                                 True, lor_counts)
+    assert np.array_equal(np.load(tmp_path / 'ScaleFactors.npy'), got)
+    want = oracle.scale_scatters_to_prompts(prompts, S, R, Nm, M, lor_counts, sino, w.lor_coordinates, grid,
+                                            w.image_size, w.geometry, geom, 2, True)
+    for b, k in enumerate(ks):
+        assert abs(got[b] - k) <= 2e-5 * k
+        assert abs(got[b] - want[b]) <= 1e-5 * abs(want[b])
+
+
+def test_scale_to_prompts_device_pointers():
+    """SSS_FIT_DEVICE: the same sums from device-resident sinograms
+    (asynchronous call, sums read back after a synchronisation)."""
+    from opensss_amd import _native
+    from opensss_amd.backprojection import fit_geometry_buffers, span_entries
+    from opensss_amd.scaling import scale_to_prompts, scale_to_prompts_device
+    w = _case()
+    n_bins = 9
+    sino, S, prompts, R, Nm, M, lor_counts = _scaling_inputs(w, True, n_bins, 13, True)
+    geom, _ = expandRings(w.geometry, w.normals)
+    grid = np.array([24, 24, 12])
+    _, want = scale_to_prompts(prompts, S, R, Nm, M, lor_counts[0, 0, :], sino, w.lor_coordinates, grid,
+                               w.image_size, w.geometry, geom, 2, True, return_sums=True)
+    ctx = _native.default_context()
+    entries = span_entries(sino)
+    bufs = fit_geometry_buffers(geom, w.lor_coordinates, entries, context=ctx)
+    dev = [_native.DeviceBuffer.from_array(ctx, np.ascontiguousarray(a))
+           for a in (S, prompts, np.ascontiguousarray(lor_counts[0, 0, :]), R, Nm, M)]
+    sums = _native.DeviceBuffer(ctx, 16 * n_bins)
+    try:
+        scale_to_prompts_device(grid, w.image_size, geom, w.lor_coordinates, entries, sino.shape[0], 2, True,
+                                n_bins, dev[0].ptr.value, dev[1].ptr.value, True, dev[5].ptr.value,
+                                sums.ptr.value, counts_ptr=dev[2].ptr.value, counts_per_plane=True,
+                                randoms_ptr=dev[3].ptr.value, norm_ptr=dev[4].ptr.value, geometry_buffers=bufs,
+                                context=ctx)
+        ctx.synchronize()
+        got = sums.download(np.empty((n_bins, 2), np.float64))
+    finally:
+        for b in dev + list(bufs) + [sums]:
+            b.free()
+    assert np.array_equal(got, want)

@@ -358,3 +358,33 @@ def test_backproject_conserves_each_lors_value(oracle, mode):
         else:
             assert np.allclose(share, float(v) * L / L.sum(), rtol=1e-6, atol=1e-7 * float(v))
         assert np.array_equal(imgs[3][idx - 1], np.ones(len(L), np.float32))
+
+
+def test_scale_loop_divides_by_lor_counts_and_recovers_k_per_bin(oracle):
+    """Scaling.py:60-113 on a span case with mash 2: prompts = (k_b S_b +
+    R Norm) LORCounts with per-plane counts 4 and 8 (SumLORsCounts), so each
+    bin's fit recovers its k_b only when the prompts are divided by the
+    counts, the randoms multiplied by the normalisation and the mask
+    thresholded at 1."""
+    from opensss_amd.data_reduction import SinogramLUT, SumLORsCounts
+    w, _, geom = _span_case()
+    NR = w.geometry.shape[0]
+    msi, _, lut, counts = SinogramLUT(NR, 3, 4)
+    P = msi.shape[0]
+    lor_counts = SumLORsCounts(w.geometry, lut, w.sinogram_index, 2, 3, counts.shape[0])[:, :, :P]
+    assert set(np.unique(lor_counts).tolist()) == {4.0, 8.0}
+    rng = np.random.default_rng(11)
+    N = w.geometry.shape[1]
+    shape = (N + 1, N // 2, P)
+    ks = [0.8125, 1.375]
+    S = np.stack([rng.uniform(0.5, 2.0, size=shape).astype(np.float32) for _ in ks])
+    R = rng.uniform(0.0, 0.3, size=shape).astype(np.float32)
+    Nm = rng.uniform(0.8, 1.2, size=shape).astype(np.float32)
+    M = rng.integers(0, 3, size=shape).astype(np.uint8)   # 1 and 2 are both tails
+    prompts = np.stack([((np.float64(k) * S[b] + R * Nm) * lor_counts).astype(np.float32)
+                        for b, k in enumerate(ks)], axis=-1)
+    got = oracle.scale_scatters_to_prompts(prompts, S, R, Nm, M, lor_counts, msi, w.lor_coordinates,
+                                           np.array([12, 12, 6]), w.image_size, w.geometry, geom, 2, True)
+    assert got.dtype == np.float32 and got.shape == (2,)
+    for b, k in enumerate(ks):
+        assert abs(got[b] - k) <= 2e-5 * k
