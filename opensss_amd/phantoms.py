@@ -12,7 +12,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .geometry import cylinder_geometry
+from .geometry import cylinder_geometry, mashDetectors
 from .sinogram import SinogramCoordinates
 from .tables import group_by_slice
 
@@ -141,6 +141,19 @@ def small_tof(seed=3, n_sp=40, n_bins=7, shape=(24, 24, 12), n_det=48, n_rings=8
     return _workload(f'small-tof-{n_det}x{n_rings}-{shape[0]}x{shape[1]}x{shape[2]}-{n_sp}sp-{n_bins}bins',
                      n_rings, n_det, 25.6 / n_rings, shape, vox, n_sp, seed, nru, ndu, n_bins,
                      214.0, 3335.64)
+
+
+def mashed(w, mash):
+    """The workload on its scanner with `mash` neighbouring crystals merged
+    (Geometry.py:137-141 with Mash > 1: nan-mean geometry and normals, and
+    the sinogram coordinates of one transaxial sector of N/mash crystals
+    without the ID shift).  The sampled detectors are kept at most N/mash."""
+    from dataclasses import replace
+    geom, normals = mashDetectors(w.geometry, w.normals, mash)
+    n_rings, n_det = geom.shape[0], geom.shape[1]
+    lor, sino, _ = SinogramCoordinates(1, 1, 1, 1, n_det, n_rings, ID_Shidt=False)
+    return replace(w, name=f'{w.name}-mash{mash}', geometry=geom, normals=normals, lor_coordinates=lor,
+                   sinogram_index=sino, n_det_used=min(w.n_det_used, n_det))
 
 
 def random_instance(seed):
