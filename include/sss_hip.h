@@ -24,6 +24,8 @@
  *                          and the scale of Scaling.py:27-121)
  *   sss_scale_to_prompts   Python/openSSS/Scaling.py:60-113 (every TOF bin,
  *                          with the LORCounts division and the mask threshold)
+ *   sss_listmode_histogram Python/Timber/Datafile.py:29-115 (ExportPrompts)
+ *   sss_listmode_inject    Python/Timber/Datafile.py:117-240 (InjectScatters)
  */
 #ifndef SSS_HIP_H
 #define SSS_HIP_H
@@ -272,6 +274,52 @@ int sss_scale_to_prompts(sss_ctx* ctx, const int32_t grid[3], const double bound
                          const void* prompts, int prompts_type, const float* lor_counts, int lor_counts_per_plane,
                          const float* randoms, const float* norm, const uint8_t* mask, int flags,
                          double* sums_out);
+
+/* CASToR list-mode events (Python/Timber).  An event record is `words`
+ * 4-byte words (Timber/Coincidence.py:45-95): the time (uint32, word 0), the
+ * float corrections, the optional line count, then the crystal ID pairs; the
+ * first pair is at id_word and the TOF (float32, ps) at tof_word.  An event's
+ * crystal pair is parsed as ParseCASToRIDs (Coincidence.py:97-159) with
+ * crystals_per_ring (the coordinates' N times mash) and shift, and binned to
+ * (radial, angular, plane) = (lor[d1/mash][d2/mash][1], [..][0],
+ * ring_index[ring1][ring2]) of an (n_rad, n_ang, n_planes) sinogram; -1
+ * entries drop the event.  TOF bin int((TOF + R) / (2R / B)) for |TOF| <= R.
+ * Event counts are below 2^32 - 1.  Host pointers, or device pointers on the
+ * context's device with SSS_LM_DEVICE; both calls return when done.  A
+ * detector, ring or coordinate outside the tables (the reference's
+ * IndexError) is SSS_ERR_INVALID.
+ *
+ * sss_listmode_histogram: Timber/Datafile.py:29-115 (ExportPrompts): prompts
+ * (n_rad, n_ang, n_planes, n_bins) uint16 counts (wrapping at 2^16, as the
+ * reference's uint16 sinogram); with SSS_LM_RANDOMS, randoms (n_rad, n_ang,
+ * n_planes) float32 = word 0 * duration / n_bins of the cell's last event
+ * (0 where none); with SSS_LM_NORM, norm = word 1 of the cell's last event as
+ * float32 (1 where none).  Every binned event sets randoms and norm, also one
+ * outside the TOF range.
+ * sss_listmode_inject: Timber/Datafile.py:117-240 (InjectScatters): the
+ * events with |TOF| <= R, a cell and a TOF bin b, in file order, each copied
+ * with the float32 rate factor[b] * scatter[b][cell] inserted at word
+ * scatter_word (later words move up one); events_out holds up to n_events
+ * records of words + 1 words, *n_out receives the count. */
+typedef struct {
+    int32_t words;         /* 4-byte words per event record */
+    int32_t tof_word;      /* word of the TOF (float32, ps) */
+    int32_t id_word;       /* word of the first crystal ID (uint32) */
+    int32_t scatter_word;  /* sss_listmode_inject: word the scatter rate is written at */
+} sss_listmode_layout;
+#define SSS_LM_RANDOMS 1
+#define SSS_LM_NORM 2
+#define SSS_LM_DEVICE 4
+int sss_listmode_histogram(sss_ctx* ctx, const void* events, int64_t n_events, const sss_listmode_layout* layout,
+                           int64_t crystals_per_ring, int64_t shift, int mash, int n_det_lor,
+                           const int16_t* lor_coords, int n_rings_index, const int16_t* ring_index, int n_rad,
+                           int n_ang, int n_planes, int n_bins, double tof_range, double duration, int flags,
+                           uint16_t* prompts_out, float* randoms_out, float* norm_out);
+int sss_listmode_inject(sss_ctx* ctx, const void* events, int64_t n_events, const sss_listmode_layout* layout,
+                        int64_t crystals_per_ring, int64_t shift, int mash, int n_det_lor,
+                        const int16_t* lor_coords, int n_rings_index, const int16_t* ring_index, int n_rad,
+                        int n_ang, int n_planes, int n_bins, double tof_range, const float* factor,
+                        const float* scatter, int flags, void* events_out, int64_t* n_out);
 
 /* Synthetic prompts (no counterpart in the reference, which reads measured
  * prompts; the per-LOR trace follows Python/openSSS/TailsMask.py:139-145).

@@ -298,11 +298,50 @@ hipError_t launch_forward_project(const Grid& g, const double* lam, const double
                                   float* out, hipStream_t st);
 hipError_t launch_poisson(const float* mean, int64_t n, double scale, uint64_t seed, float* out, hipStream_t st);
 
+// A CASToR list-mode event record: `words` 4-byte words, the TOF float at
+// tof_word, the first crystal ID pair at id_word; InjectScatters writes the
+// scatter rate at scatter_word (the later words move up by one).
+struct ListmodeLayout {
+    int words, tof_word, id_word, scatter_word;
+};
+// Tables an event is binned through (device pointers): the sinogram
+// coordinates of the n_det_lor detectors (mashed, (N, N, 2) int16) and the
+// (n_rings_index, n_rings_index) int16 ring-pair -> plane table; the
+// sinogram is (n_rad, n_ang, n_planes).
+struct ListmodeTables {
+    const int16_t* lor;
+    int n_det_lor, mash;
+    const int16_t* ring_index;
+    int n_rings_index;
+    int n_rad, n_ang, n_planes;
+};
+hipError_t launch_lm_histogram(const uint32_t* events, int64_t n_events, int64_t first_index,
+                               const ListmodeLayout& lay, const ListmodeTables& t, int64_t ncpr, int64_t shift,
+                               int n_bins, double tof_range, double duration, uint32_t* counts,
+                               unsigned long long* randoms_last, unsigned long long* norm_last, int* bad,
+                               hipStream_t st);
+hipError_t launch_lm_finish(const uint32_t* counts, int64_t n_counts, uint16_t* prompts,
+                            const unsigned long long* randoms_last, const unsigned long long* norm_last,
+                            int64_t n_cells, float* randoms, float* norm, hipStream_t st);
+int64_t lm_scan_blocks(int64_t n_events);
+hipError_t launch_lm_select(const uint32_t* events, int64_t n_events, const ListmodeLayout& lay,
+                            const ListmodeTables& t, int64_t ncpr, int64_t shift, int n_bins, double tof_range,
+                            const float* factor, const float* scatter, int64_t n_cells, int32_t* keep, float* rate,
+                            int64_t* pos, int64_t* block_sum, int* bad, hipStream_t st);
+hipError_t launch_lm_write(const uint32_t* events, int64_t n_events, const ListmodeLayout& lay, const int32_t* keep,
+                           const int64_t* pos, const float* rate, uint32_t* out, hipStream_t st);
+
 // Device address of this library's no-allocation fit scratch on the current
 // device (looked up once per device at context creation).
 int noalloc_scratch_init(int device);
 void* noalloc_scratch(int device);
 int ctx_device(const sss_ctx* ctx);
+hipStream_t ctx_stream(const sss_ctx* ctx);
+// A device block from the context's cache (SSS_OK or SSS_ERR_NOMEM with the
+// error set); ctx_pool_put hands it back (its next user works on the
+// context's stream, after what was queued with it).
+int ctx_pool_get(sss_ctx* ctx, size_t bytes, void** p);
+void ctx_pool_put(sss_ctx* ctx, void* p);
 
 hipError_t launch_dense_scatter_typed(const void* compact, bool fp64, int n_bins, int nru, int ndu,
                                       int n_det, const int32_t* dets, const int16_t* lor,

@@ -233,6 +233,7 @@ struct sss_ctx {
 };
 
 int sss::ctx_device(const sss_ctx* ctx) { return ctx->device; }
+hipStream_t sss::ctx_stream(const sss_ctx* ctx) { return ctx->stream; }
 
 // An event from the context's cache, or a new one.
 static hipError_t ctx_event(sss_ctx* c, bool timing, hipEvent_t* e)
@@ -303,6 +304,20 @@ void host_block_free(sss_ctx* c, void* p)
 }
 
 }  // namespace
+
+int ctx_pool_get(sss_ctx* ctx, size_t bytes, void** p)
+{
+    unsigned char* b = nullptr;
+    const int rc = pool_alloc(ctx, &b, bytes > 0 ? bytes : 1, nullptr);
+    *p = b;
+    return rc;
+}
+
+void ctx_pool_put(sss_ctx* ctx, void* p)
+{
+    if (p) ctx->pool.put(p);
+}
+
 }  // namespace sss
 
 struct sss_subject {

@@ -15,6 +15,8 @@ Functions mirror the reference entry points they restate:
 * ``sinogram_coordinates_loops`` -> ``Sinogram.py:27-141`` as plain loops
 * ``tail_fit``       -> ``BackProjection.py:115-214`` / ``:302-392``
 * ``scale_scatters_to_prompts`` -> the scaling loop of ``Scaling.py:60-113``
+* ``parse_castor_ids``, ``export_prompts_loops``, ``inject_scatters_loops``
+  -> ``Timber/Coincidence.py:45-159``, ``Timber/Datafile.py:66-113,161-230``
 """
 from __future__ import annotations
 
@@ -478,3 +480,108 @@ def scale_scatters_to_prompts(Prompts, Scatters, Randoms, Normalization, Mask, L
                             MashedSinogramIndices=SinogramIndex)
         out[b] = f[0]
     return out
+
+
+# ---- CASToR list mode (Timber/Coincidence.py, Timber/Datafile.py) ----
+
+def listmode_event_format(attenuation, scatter, random, normalization, tof, max_lines):
+    """struct format and positions (scatter, random, norm, TOF, first ID) of
+    a list-mode event, Coincidence.py:45-95."""
+    lines_flag = max_lines > 1
+    fmt = '<1L'
+    nc = attenuation + scatter + random + normalization + tof
+    if nc > 0:
+        fmt += f'{nc}f'
+    if lines_flag:
+        fmt += '1I'
+    fmt += f'{2 * max_lines}L'
+    base = 1 + attenuation
+    pos = (base if scatter else None,
+           base + scatter if random else None,
+           base + scatter + random if normalization else None,
+           base + scatter + random + normalization if tof else None,
+           base + scatter + random + normalization + tof + lines_flag)
+    return fmt, pos
+
+
+def parse_castor_ids(event, pos, ncpr, shift=0):
+    """ParseCASToRIDs (Coincidence.py:97-159) on one unpacked event."""
+    a, b = event[pos[4]], event[pos[4] + 1]
+    tof = event[pos[3]]
+    lo, hi = (a, b) if a < b else (b, a)
+    tof_lo = tof if a < b else -tof
+    det_lo, det_hi = lo % ncpr, hi % ncpr
+    d1, d2 = (det_lo, det_hi) if det_lo < det_hi else (det_hi, det_lo)
+    s_lo, s_hi = lo % ncpr - shift, hi % ncpr - shift
+    s_lo += ncpr if s_lo < 0 else 0
+    s_hi += ncpr if s_hi < 0 else 0
+    if s_lo < s_hi:
+        return d1, lo // ncpr, d2, hi // ncpr, tof_lo
+    return d1, hi // ncpr, d2, lo // ncpr, -tof_lo
+
+
+def export_prompts_loops(data, flags, duration, LORCoordinates, SinogramIndex, TOF_bins, TOF_range, mash=1,
+                         Shift=0):
+    """The event loop of ExportPrompts (Datafile.py:66-113) on the datafile
+    bytes; flags = (attenuation, scatter, random, normalization, tof,
+    max_lines).  Returns (prompts uint16, randoms f32 or None, norm f32 or None)."""
+    import struct
+    fmt, pos = listmode_event_format(*flags)
+    lor = np.asarray(LORCoordinates)
+    ncpr = lor.shape[0] * mash
+    size = [int(np.max(lor[:, :, 1])) + 1, int(np.max(lor[:, :, 0])) + 1, int(np.max(SinogramIndex)) + 1]
+    prompts = np.zeros(size + [TOF_bins], dtype=np.uint16)
+    randoms = np.zeros(size, dtype=np.single) if flags[2] else None
+    norm = np.ones(size, dtype=np.single) if flags[3] else None
+    for ev in struct.iter_unpack(fmt, data):
+        d1, r1, d2, r2, tof = parse_castor_ids(ev, pos, ncpr, Shift)
+        c0 = lor[d1 // mash, d2 // mash, 0]
+        c1 = lor[d1 // mash, d2 // mash, 1]
+        sino = SinogramIndex[r1, r2]
+        if sino == -1 or c0 == -1 or c1 == -1:
+            continue
+        if randoms is not None:
+            randoms[c1, c0, sino] = ev[0] * duration / TOF_bins
+        if norm is not None:
+            norm[c1, c0, sino] = ev[1]
+        if abs(tof) > TOF_range:
+            continue
+        tb = (tof + TOF_range) / (2 * TOF_range / TOF_bins)
+        if tb < 0 or tb >= TOF_bins:
+            continue
+        with np.errstate(over='ignore'):   # uint16 counts wrap, as the reference's
+            prompts[c1, c0, sino, int(tb)] += np.uint16(1)
+    return prompts, randoms, norm
+
+
+def inject_scatters_loops(data, flags, duration, LORCoordinates, RingTable, scatter_sinograms, Scale, TOF_range,
+                          mash=1, Shift=0):
+    """The event loop of InjectScatters (Datafile.py:161-230) on the datafile
+    bytes: returns the bytes of the new datafile."""
+    import struct
+    fmt, pos = listmode_event_format(*flags)
+    out_flags = (flags[0], 1, flags[2], flags[3], flags[4], flags[5])
+    out_fmt, out_pos = listmode_event_format(*out_flags)
+    lor = np.asarray(LORCoordinates)
+    ncpr = lor.shape[0] * mash
+    n_bins = len(Scale)
+    scale = np.array(Scale, dtype=np.float32, copy=True)
+    scale /= (2 * TOF_range / n_bins)
+    sf = 1. / (0.299792458 / 2)
+    out = bytearray()
+    for ev in struct.iter_unpack(fmt, data):
+        d1, r1, d2, r2, tof = parse_castor_ids(ev, pos, ncpr, Shift)
+        if abs(tof) > TOF_range:
+            continue
+        c0 = lor[d1 // mash, d2 // mash, 0]
+        c1 = lor[d1 // mash, d2 // mash, 1]
+        sino = RingTable[r1, r2]
+        if sino == -1 or c0 == -1 or c1 == -1:
+            continue
+        tb = (tof + TOF_range) / (2 * TOF_range / n_bins)
+        if tb < 0 or tb >= n_bins:
+            continue
+        tb = int(tb)
+        rate = scale[tb] * sf / duration * scatter_sinograms[tb, c1, c0, sino]
+        out += struct.pack(out_fmt, *ev[:out_pos[0]], rate, *ev[out_pos[0]:])
+    return bytes(out)
