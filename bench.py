@@ -507,10 +507,79 @@ def sinogram_stages(w, out32, out64, oracle_pair=None, accel=2):
                                  geometry_buffers=gb)[0]
             res['tail_scale_rel_err_vs_oracle'] = abs(fa - fo) / abs(fo)
             res['tail_scale_oracle_sample'] = f'bin {b}, the cpu_baseline sample\'s estimates'
+        scales = np.zeros(B, np.float32)
+        for i, b in enumerate(fitted):
+            scales[b] = all_scales[i]
+        res['listmode'] = listmode_leg(ctx, w, SinogramLUT(NR, 19, 9)[2], i32.ptr.value, B, scales)
         return res
     finally:
         for b in bufs:
             b.free()
+
+
+def listmode_leg(ctx, w, lut, scatter_ptr, n_bins, scales, n_events=50_000_000, seed=7):
+    """CASToR list mode at the 760 x 80 scanner: n_events synthetic events
+    (ring differences within +-9, opposite detectors +-N/4, TOF normal with
+    sigma 0.4 of the range) histogrammed into the span 19 / MRD 9 prompts
+    (761 x 380 x 159 x B uint16) and injected with the interpolated FP32
+    estimate's scatter rates, device-resident; host clock around the
+    synchronous calls (best of three)."""
+    from opensss_amd import _native, castor
+    lib = _native.load_library()
+    NR, N = w.geometry.shape[0], w.geometry.shape[1]
+    lor = np.ascontiguousarray(w.lor_coordinates, dtype=np.int16)
+    lut16 = np.ascontiguousarray(lut, dtype=np.int16)
+    h = castor.listmode_header('bench_df.Cdf', n_events, duration=600.0, tof_range=float(w.tof_range))
+    rng = np.random.default_rng(seed)
+    words = castor.event_words(h)
+    idw = castor.GenerateEventClass(h)[1][4]
+    tofw = castor.GenerateEventClass(h)[1][3]
+    ev = np.zeros((n_events, words), np.uint32)
+    ev[:, 0] = rng.integers(0, 600000, n_events, dtype=np.uint32)
+    ra = rng.integers(0, NR, n_events)
+    rb = np.clip(ra + rng.integers(-9, 10, n_events), 0, NR - 1)
+    da = rng.integers(0, N, n_events)
+    db = (da + N // 2 + rng.integers(-(N // 4), N // 4 + 1, n_events)) % N
+    ev[:, idw] = (ra * N + da).astype(np.uint32)
+    ev[:, idw + 1] = (rb * N + db).astype(np.uint32)
+    del ra, rb, da, db
+    ev[:, tofw] = (rng.normal(0.0, 0.4, n_events) * w.tof_range).astype(np.float32).view(np.uint32)
+    n_rad, n_ang, n_planes = N + 1, N // 2, int(lut16.max()) + 1
+    factors = castor.injection_factors(scales, h['tof_range'], h['duration'])
+    lay = castor._layout(h, scatter_word=castor.GenerateEventClass(dict(h, scatter_flag=True))[1][0])
+    bufs = [_native.DeviceBuffer.from_array(ctx, a) for a in (ev, lor, lut16, factors)]
+    prompts = _native.DeviceBuffer(ctx, 2 * n_rad * n_ang * n_planes * n_bins)
+    out = _native.DeviceBuffer(ctx, 4 * (words + 1) * n_events)
+    bufs += [prompts, out]
+    del ev
+    n_out = _native.ctypes.c_int64(0)
+    try:
+        t_h, t_i = [], []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            _native.check(lib.sss_listmode_histogram(
+                ctx.handle, bufs[0].ptr, n_events, _native.ctypes.byref(lay), N, 0, 1, N, bufs[1].ptr, NR,
+                bufs[2].ptr, n_rad, n_ang, n_planes, n_bins, h['tof_range'], h['duration'], _native.SSS_LM_DEVICE,
+                prompts.ptr, None, None), 'sss_listmode_histogram')
+            t_h.append(time.perf_counter() - t0)
+        for _ in range(3):
+            t0 = time.perf_counter()
+            _native.check(lib.sss_listmode_inject(
+                ctx.handle, bufs[0].ptr, n_events, _native.ctypes.byref(lay), N, 0, 1, N, bufs[1].ptr, NR,
+                bufs[2].ptr, n_rad, n_ang, n_planes, n_bins, h['tof_range'], bufs[3].ptr,
+                _native.ctypes.c_void_p(scatter_ptr), _native.SSS_LM_DEVICE, out.ptr, _native.ctypes.byref(n_out)),
+                'sss_listmode_inject')
+            t_i.append(time.perf_counter() - t0)
+    finally:
+        for b in bufs:
+            b.free()
+    th, ti = min(t_h), min(t_i)
+    return {
+        'workload': f'{n_events} events ({4 * words} B), 760 x 80 scanner, span 19 MRD 9 prompts, {n_bins} TOF bins',
+        'histogram_ms': th * 1e3, 'histogram_events_per_s': n_events / th,
+        'inject_ms': ti * 1e3, 'inject_events_per_s': n_events / ti, 'injected_events': int(n_out.value),
+        'host_clock': True,
+    }
 
 
 def main():
@@ -612,7 +681,9 @@ def main():
         print(f"[bench] sinogram stages: interpolate {stages['interpolate_ms']:.2f} ms, mask "
               f"{stages['tail_mask_ms']:.1f} ms, fit {stages['tail_fit_ms_per_bin']:.1f} ms/bin "
               f"({stages['tail_fit_all_bins_ms']:.1f} ms for {len(stages['fitted_bins'])} bins in one call, "
-              f"identical {stages['tail_fit_all_bins_identical']}), tail scale "
+              f"identical {stages['tail_fit_all_bins_identical']}), list mode "
+              f"{stages['listmode']['histogram_events_per_s'] / 1e9:.2f} G events/s histogram, "
+              f"{stages['listmode']['inject_events_per_s'] / 1e9:.2f} G events/s inject, tail scale "
               f"FP32 vs FP64 {stages['tail_scale_max_rel_err_fp32_vs_fp64']:.2e}", file=sys.stderr, flush=True)
 
     lat_ms = np.array(lat) * 1e3
