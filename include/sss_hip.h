@@ -26,6 +26,7 @@
  *                          with the LORCounts division and the mask threshold)
  *   sss_listmode_histogram Python/Timber/Datafile.py:29-115 (ExportPrompts)
  *   sss_listmode_inject    Python/Timber/Datafile.py:117-240 (InjectScatters)
+ *   sss_crop_and_downscale Python/openSSS/Utils.py:212-267 (CropAndDownscale)
  */
 #ifndef SSS_HIP_H
 #define SSS_HIP_H
@@ -320,6 +321,30 @@ int sss_listmode_inject(sss_ctx* ctx, const void* events, int64_t n_events, cons
                         const int16_t* lor_coords, int n_rings_index, const int16_t* ring_index, int n_rad,
                         int n_ang, int n_planes, int n_bins, double tof_range, const float* factor,
                         const float* scatter, int flags, void* events_out, int64_t* n_out);
+
+/* Image preparation (Python/openSSS/Utils.py:212-267, CropAndDownscale, and
+ * the attenuation floor of PrepareImages :72), host pointers, volumes
+ * x-fastest (the Fortran order of (X, Y, Z)).  The cropped volume's voxel c
+ * is input voxel c + crop_start (0 outside the input: the reference's crop
+ * window or zero pad, resolved by the caller).  It is then resized as
+ * scikit-image's resize(order=3, mode='edge', anti_aliasing=True,
+ * preserve_range=True) states it: along each axis with radius[k] >= 0 a
+ * Gaussian of the 2 radius[k] + 1 taps weights[k] (edge values repeated,
+ * float64 sums stored float32); cubic B-spline coefficients of the result
+ * padded by 12 repeated edge voxels (float64); at each output voxel the
+ * spline's value at (o + 1/2) crop/out - 1/2; with clip, the output clipped
+ * to the cropped volume's range; values below zero_below set to 0. */
+typedef struct {
+    int64_t in_shape[3];
+    int64_t crop_start[3];
+    int64_t crop_shape[3];
+    int64_t out_shape[3];
+    int32_t radius[3];          /* Gaussian half width per axis, -1: no smoothing along it */
+    const double* weights[3];   /* host, 2 radius + 1 taps */
+    int32_t clip;
+    float zero_below;
+} sss_resize_desc;
+int sss_crop_and_downscale(sss_ctx* ctx, const float* image, const sss_resize_desc* desc, float* out);
 
 /* Synthetic prompts (no counterpart in the reference, which reads measured
  * prompts; the per-LOR trace follows Python/openSSS/TailsMask.py:139-145).

@@ -331,6 +331,14 @@ hipError_t launch_lm_select(const uint32_t* events, int64_t n_events, const List
 hipError_t launch_lm_write(const uint32_t* events, int64_t n_events, const ListmodeLayout& lay, const int32_t* keep,
                            const int64_t* pos, const float* rate, uint32_t* out, hipStream_t st);
 
+// Crop and resize of an x-fastest volume (Utils.py:212-267); scratch of
+// resize_scratch_bytes(crop) bytes (256-byte aligned).
+int64_t resize_scratch_bytes(const int64_t crop[3]);
+hipError_t launch_crop_and_downscale(const float* in, const int64_t in_shape[3], const int64_t crop_start[3],
+                                     const int64_t crop[3], const int64_t out_shape[3], const double* const weights[3],
+                                     const int radius[3], int clip, float zero_below, void* scratch, float* out,
+                                     hipStream_t st);
+
 // Device address of this library's no-allocation fit scratch on the current
 // device (looked up once per device at context creation).
 int noalloc_scratch_init(int device);

@@ -17,6 +17,8 @@ Functions mirror the reference entry points they restate:
 * ``scale_scatters_to_prompts`` -> the scaling loop of ``Scaling.py:60-113``
 * ``parse_castor_ids``, ``export_prompts_loops``, ``inject_scatters_loops``
   -> ``Timber/Coincidence.py:45-159``, ``Timber/Datafile.py:66-113,161-230``
+* ``crop_and_downscale`` -> ``Utils.py:212-267`` (scikit-image's resize as
+  it states it, through scipy.ndimage: parity with scikit-image unpinned)
 """
 from __future__ import annotations
 
@@ -585,3 +587,43 @@ def inject_scatters_loops(data, flags, duration, LORCoordinates, RingTable, scat
         rate = scale[tb] * sf / duration * scatter_sinograms[tb, c1, c0, sino]
         out += struct.pack(out_fmt, *ev[:out_pos[0]], rate, *ev[out_pos[0]:])
     return bytes(out)
+
+
+# ---- image preparation (Utils.py:212-267) ----
+
+def crop_and_downscale(image, voxel_size, crop_size, downscaled_dimensions, anti_aliasing=True, clip=True):
+    """Restatement of CropAndDownscale (Utils.py:212-267): the crop loop as
+    written there, then scikit-image's resize(order=3, mode='edge',
+    preserve_range=True) as that library states it, through the scipy.ndimage
+    routines it calls: gaussian_filter (sigma = max(0, (f - 1) / 2), mode
+    'nearest'), zoom (order 3, mode 'nearest', grid_mode=True) and the clip
+    to the input's range.  float32 in, float32 out."""
+    from scipy import ndimage as ndi
+    cropped = crop(image, voxel_size, crop_size)
+    factors = np.divide(cropped.shape, downscaled_dimensions)
+    filtered = cropped
+    if anti_aliasing:
+        filtered = ndi.gaussian_filter(cropped, np.maximum(0, (factors - 1) / 2), mode='nearest')
+    out = ndi.zoom(filtered, [1 / f for f in factors], order=3, mode='nearest', grid_mode=True)
+    if clip:
+        np.clip(out, np.min(cropped), np.max(cropped), out=out)
+    return out
+
+
+def crop(image, voxel_size, crop_size):
+    """The crop / zero-pad loop of CropAndDownscale (Utils.py:243-262)."""
+    image = np.asarray(image, dtype=np.float32)
+    half = np.ceil(np.array(crop_size) / np.array(voxel_size) / 2).astype(int)
+    center = np.array(image.shape) // 2
+    cropped = image
+    for dim in range(3):
+        start, end = center[dim] - half[dim], center[dim] + half[dim]
+        if start < 0 and end > image.shape[dim]:
+            pw = [(0, 0)] * 3
+            pw[dim] = (half[dim] - center[dim], half[dim] - center[dim])
+            cropped = np.pad(cropped, pw, mode='constant', constant_values=0)
+        else:
+            sl = [slice(None)] * 3
+            sl[dim] = slice(start, end)
+            cropped = cropped[tuple(sl)]
+    return cropped
