@@ -511,10 +511,30 @@ def sinogram_stages(w, out32, out64, oracle_pair=None, accel=2):
         for i, b in enumerate(fitted):
             scales[b] = all_scales[i]
         res['listmode'] = listmode_leg(ctx, w, SinogramLUT(NR, 19, 9)[2], i32.ptr.value, B, scales)
+        res['image_preparation'] = image_leg(ctx, w)
         return res
     finally:
         for b in bufs:
             b.free()
+
+
+def image_leg(ctx, w, ct_shape=(512, 512, 300), ct_voxel=(0.9765625, 0.9765625, 1.0), seed=8):
+    """CropAndDownscale of a CT-sized float32 volume (512 x 512 x 300 of
+    ~1 mm) to the workload's grid over its field of view (mm), host arrays in
+    and out (the upload of the volume included); best of three."""
+    from opensss_amd import images
+    rng = np.random.default_rng(seed)
+    # x fastest, as LoadImage's transposed view of the interfile data
+    ct = np.asfortranarray(rng.uniform(0.0, 0.1, size=ct_shape).astype(np.float32))
+    size_mm = [10.0 * (w.image_size[k + 3] - w.image_size[k]) for k in range(3)]
+    dims = list(w.attenuation.shape)
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        images.CropAndDownscale(ct, ct_voxel, size_mm, dims, zero_below=0.001, context=ctx)
+        ts.append(time.perf_counter() - t0)
+    return {'workload': f'{ct_shape} float32 of {ct_voxel} mm -> {tuple(dims)} over {[round(v) for v in size_mm]} mm',
+            'ms': min(ts) * 1e3, 'input_bytes': int(ct.nbytes), 'host_clock': True}
 
 
 def listmode_leg(ctx, w, lut, scatter_ptr, n_bins, scales, n_events=50_000_000, seed=7):
@@ -683,7 +703,8 @@ def main():
               f"({stages['tail_fit_all_bins_ms']:.1f} ms for {len(stages['fitted_bins'])} bins in one call, "
               f"identical {stages['tail_fit_all_bins_identical']}), list mode "
               f"{stages['listmode']['histogram_events_per_s'] / 1e9:.2f} G events/s histogram, "
-              f"{stages['listmode']['inject_events_per_s'] / 1e9:.2f} G events/s inject, tail scale "
+              f"{stages['listmode']['inject_events_per_s'] / 1e9:.2f} G events/s inject, image preparation "
+              f"{stages['image_preparation']['ms']:.1f} ms, tail scale "
               f"FP32 vs FP64 {stages['tail_scale_max_rel_err_fp32_vs_fp64']:.2e}", file=sys.stderr, flush=True)
 
     lat_ms = np.array(lat) * 1e3
