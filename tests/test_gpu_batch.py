@@ -66,3 +66,31 @@ def test_resident_subject_between_others():
         assert np.allclose(twice, 2.0 * ref, rtol=1e-6, atol=1e-12 * np.abs(ref).max())
     finally:
         s.close()
+
+
+@pytest.mark.parametrize('seeds', [(3, 11, 14, 1)])
+def test_full_size_random_instances_batched_equal_one_instance(seeds):
+    """Instances of the uneven batch's distribution at their full sizes
+    (grids 32^3/2 .. 128^2 x 64, 20-32 detectors x 8-20 rings, 5-20 bins,
+    430-2480 scatter points): the batched path over two contexts returns the
+    one-instance outputs bit for bit."""
+    ws = [phantoms.random_instance(s) for s in seeds]
+    plain = run_plain(ws, WATER)
+    got = run_batch(ws, WATER, n_streams=2)
+    for w, a, b in zip(ws, got, plain):
+        assert a.shape == b.shape and np.array_equal(a, b), w.name
+        assert np.abs(a).max() > 0
+
+
+def test_random_instance_subset_vs_oracle(oracle):
+    """One full-size instance of the batch distribution (96x96x48 grid, 48
+    detectors x 12 rings, 21 bins) on a scatter-point subset against the
+    oracle, to the FP32 bar."""
+    from tests.sss_cases import gpu_compact, oracle_compact, rel_err_above
+    w = phantoms.random_instance(8)
+    pts = w.scatter_points[::40]
+    got = gpu_compact(w, 'fp32', points=pts)
+    want = oracle_compact(oracle, w, points=pts)
+    rel, absn = rel_err_above(got, want)
+    print(f'{w.name} subset {len(pts)} sp: max rel err {rel:.3e}, max abs/peak {absn:.3e}')
+    assert rel <= 1e-4 and absn <= 1e-6

@@ -111,3 +111,27 @@ def test_tile_shapes_give_identical_results_and_autotune_serves_them():
         server.close()
     for r in reqs:
         assert np.array_equal(r.output, _alone(subs, r))
+
+
+def test_short_random_stream_served_equals_alone():
+    """A short Poisson stream over three subjects of the served distribution
+    (full-size random instances, activity x (1 + 0.05 N(0,1)) per request):
+    every served output equals its run alone."""
+    from opensss_amd.serve import Request, ScatterServer, serve
+    subs = {f'r{s}': phantoms.random_instance(s) for s in (11, 14, 3)}
+    rng = np.random.default_rng(5)
+    keys = list(subs)
+    t, reqs = 0.0, []
+    for i in range(9):
+        t += float(rng.exponential(1 / 50.0))
+        k = keys[int(rng.integers(len(keys)))]
+        lam = subs[k].activity * (1.0 + 0.05 * rng.standard_normal(subs[k].activity.shape))
+        reqs.append(Request(key=k, activity=lam, arrival=t, tag=i))
+    server = ScatterServer(subs, WATER, n_contexts=2, max_inflight=3)
+    try:
+        serve(server, reqs)
+    finally:
+        server.close()
+    for r in reqs:
+        assert r.output is not None
+        assert np.array_equal(r.output, _alone(subs, r)), f'request {r.tag} ({r.key})'
