@@ -81,6 +81,7 @@ struct HalfPathTable {
     float* cosabs;     // |cos| of the angle to the detector normal
     int32_t* status;   // 0 ok, 1 walk overflow
     struct PairHalf* rec;   // per activity image: what the FP32 pair evaluation reads
+    uint32_t* len_min; // smallest positive stored sample length (float bits), set by the fill
 };
 
 // One half path as the FP32 pair evaluation reads it: two 16-byte loads.
@@ -215,7 +216,15 @@ hipError_t launch_halfpath_fill(const Grid& g, const double* sp_xyz, int n_sp, c
 // (then the caller scans the padded counts into off_fold), and the copy of
 // the non-zero terms into the fold table.
 hipError_t launch_activity_check(const HalfPathTable t, SampleTable s, int64_t n_hp, const double* lam,
-                                 int32_t* changed, hipStream_t st);
+                                 int32_t* changed, hipStream_t st, const int32_t* run_if = nullptr);
+// Voxel test of a new activity image against the one the fold table was
+// built from (fast[0]: set on any difference; fast[1]: the table image was
+// unfit for the test; tab: its non-zero voxels), and the record of that image
+// when the table is rebuilt (run_if as below).
+hipError_t launch_vox_check(const double* lam, int64_t n_vox, const uint8_t* tab, const uint32_t* len_min,
+                            int32_t* fast, hipStream_t st);
+hipError_t launch_vox_tab(const double* lam, int64_t n_vox, uint8_t* tab, const uint32_t* len_min, int32_t* fast,
+                          hipStream_t st, const int32_t* run_if);
 hipError_t launch_fold_weights(SampleTable s, const int64_t* n_records, int64_t cap, const double* lam,
                                int32_t* changed, hipStream_t st);
 // run_if: when not null, the kernel does its work only if *run_if != 0 (the
@@ -227,7 +236,9 @@ hipError_t launch_activity_sum(const HalfPathTable t, SampleTable s, int64_t n_h
                                const double* lam, hipStream_t st);
 hipError_t launch_fold_compact(const HalfPathTable t, SampleTable s, int64_t n_hp, hipStream_t st,
                                const int32_t* run_if = nullptr);
-hipError_t launch_pair_half(const HalfPathTable t, int64_t n_hp, bool one_bin, hipStream_t st);
+// run_if_a, run_if_b: when both are given, the records are rebuilt only if either is non-zero
+hipError_t launch_pair_half(const HalfPathTable t, int64_t n_hp, bool one_bin, hipStream_t st,
+                            const int32_t* run_if_a = nullptr, const int32_t* run_if_b = nullptr);
 hipError_t launch_trace_segments(const Grid& g, const double* segs, int64_t n, int cap,
                                  int32_t* counts, double* L, int32_t* idx, double* s,
                                  hipStream_t st);

@@ -359,6 +359,10 @@ struct sss_subject {
     int32_t* d_dets = nullptr;
     int32_t* d_flag = nullptr;
     int32_t* d_fold_changed = nullptr;    // set when a new activity changes which terms are zero
+    // voxel test of a new activity image: [0] failed, [1] the table's image
+    // is unfit for it, [2] smallest positive sample length (float bits)
+    int32_t* d_fast = nullptr;
+    uint8_t* d_tab = nullptr;             // non-zero voxels of the fold table's image
     int64_t fold_cap = 0;                 // fold records allocated
     bool fold_valid = false;              // the fold table's layout matches the current tables
     int refreshes = 0;                    // activity images since the last walks
@@ -399,6 +403,7 @@ struct sss_subject {
         pool_free(c, d_upload);   // d_mu (x fastest) .. d_dets live in it
         pool_free(c, d_mu_own);
         pool_free(c, d_lam); pool_free(c, d_flag); pool_free(c, d_fold_changed); pool_free(c, d_scan_tmp);
+        pool_free(c, d_fast); pool_free(c, d_tab);
         pool_free(c, hp.off); pool_free(c, hp.n); pool_free(c, hp.off_fold); pool_free(c, hp.n_fold);
         pool_free(c, hp.rec); pool_free(c, hp.blk); pool_free(c, hp.blk_off); pool_free(c, hp.dist);
         pool_free(c, hp.att); pool_free(c, hp.lnatt); pool_free(c, hp.act); pool_free(c, hp.cosabs);
@@ -774,6 +779,7 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
     if ((s->z_fastest && (rc = pool_alloc(ctx, &s->d_mu_own, (size_t)s->n_vox, &t))) ||
         (rc = pool_alloc(ctx, &s->d_upload, up_bytes, &t)) || (rc = pool_alloc(ctx, &s->d_lam, (size_t)s->n_vox, &t)) ||
         (rc = pool_alloc(ctx, &s->d_flag, 1, &t)) || (rc = pool_alloc(ctx, &s->d_fold_changed, 1, &t)) ||
+        (rc = pool_alloc(ctx, &s->d_fast, 4, &t)) || (rc = pool_alloc(ctx, &s->d_tab, (size_t)s->n_vox + 1, &t)) ||
         (rc = pool_alloc(ctx, &s->hp.off, (size_t)s->n_hp + 1, &t)) || (rc = pool_alloc(ctx, &s->hp.n, (size_t)s->n_hp + 1, &t)) ||
         (rc = pool_alloc(ctx, &s->hp.off_fold, (size_t)s->n_hp + 1, &t)) ||
         (rc = pool_alloc(ctx, &s->hp.n_fold, (size_t)s->n_hp + 1, &t)) ||
@@ -791,6 +797,7 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
         delete s;
         return rc;
     }
+    s->hp.len_min = reinterpret_cast<uint32_t*>(s->d_fast + 2);
     hipStream_t st = ctx->stream;
     // The upload is queued behind whatever the context's stream holds, from
     // the pinned block that the subject keeps until destroy: nothing here
@@ -894,8 +901,14 @@ static int refresh_fold(sss_subject* s)
     SampleTable smp = s->smp;
     if (s->refreshes++ == 0) smp.fold_src = nullptr;
     if (s->fold_valid && smp.fold_src) {
+        // A voxel test against the table's image decides first: when it
+        // passes, the same samples keep non-zero terms and the same half
+        // paths non-zero integrals, so the per-sample count check and the
+        // pair records are skipped (on the device, no wait here)
         SSS_HIP(hipMemsetAsync(s->d_fold_changed, 0, sizeof(int32_t), st));
-        SSS_HIP(launch_activity_check(s->hp, smp, s->n_hp, s->d_lam, s->d_fold_changed, st));
+        SSS_HIP(hipMemsetAsync(s->d_fast, 0, sizeof(int32_t), st));
+        SSS_HIP(launch_vox_check(s->d_lam, s->n_vox, s->d_tab, s->hp.len_min, s->d_fast, st));
+        SSS_HIP(launch_activity_check(s->hp, smp, s->n_hp, s->d_lam, s->d_fold_changed, st, s->d_fast));
         SSS_HIP(launch_fold_weights(smp, s->hp.off_fold + s->n_hp, s->fold_cap, s->d_lam, s->d_fold_changed, st));
         run_if = s->d_fold_changed;
     }
@@ -903,7 +916,9 @@ static int refresh_fold(sss_subject* s)
     const auto padded = rocprim::make_transform_iterator(s->hp.n_fold, PaddedRecords{});
     if (int rc = scan_counts(s, padded, s->hp.off_fold, s->n_hp, st)) return rc;
     SSS_HIP(launch_fold_compact(s->hp, smp, s->n_hp, st, run_if));
-    SSS_HIP(launch_pair_half(s->hp, s->n_hp, false, st));
+    // the image a rebuilt table comes from, for the next images' voxel test
+    SSS_HIP(launch_vox_tab(s->d_lam, s->n_vox, s->d_tab, s->hp.len_min, s->d_fast, st, run_if));
+    SSS_HIP(launch_pair_half(s->hp, s->n_hp, false, st, run_if ? s->d_fast : nullptr, run_if));
     s->fold_valid = smp.fold_src != nullptr;
     return SSS_OK;
 }
@@ -977,6 +992,8 @@ int sss_subject_trace_end(sss_subject* s)
     s->n_samples = total;
     s->fold_valid = false;   // new walks: the next activity builds the fold table in full
     s->refreshes = 0;
+    // the fill keeps the smallest positive length (0x7f7f7f7f: 3.4e38 as float bits)
+    SSS_HIP(hipMemsetAsync(s->hp.len_min, 0x7f, sizeof(uint32_t), st));
     SSS_HIP(launch_halfpath_fill(s->grid, s->d_sp, s->n_sp, s->d_detpos, s->nru, s->ndu, s->d_mu, s->hp,
                                  s->smp, st));
     // truncated walks are counted (read by sss_subject_info_get), not an error

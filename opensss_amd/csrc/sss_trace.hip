@@ -10,6 +10,8 @@
 // reference's float64 arithmetic.
 #include "sss_walk.h"
 
+#include <algorithm>
+
 // distances, angles and the line-integral sums below keep float64 operation
 // order too (no fused multiply-add)
 #pragma clang fp contract(off)
@@ -84,19 +86,25 @@ __global__ void k_halfpath_fill(Grid g, const double* __restrict__ sp_xyz, int n
         // overlaps the next step (same terms, same order: the first added
         // term is 0 * 0)
         double mu_prev = 0, len_prev = 0;
+        float lmin = __builtin_inff();
         const int got = walk_run(g, w, [&](int k, int vox, double L, double smp) {
             const int64_t i = base + (int64_t)k * kWave;
             // q * 2^32 is exact in float64; rint drops bits below 2^-32 ps
             s.nq[i] = -(int64_t)rint(smp * two_over_c * kFixOne);
             s.vox[i] = vox;
-            if (s.len32) s.len32[i] = (float)L;
+            const float lf = (float)L;
+            if (s.len32) s.len32[i] = lf;
             else s.len[i] = L;
+            if (lf > 0.0f) lmin = fminf(lmin, lf);
             mu_int += mu_prev * len_prev;
             mu_prev = mu[vox];
             len_prev = L;
         }, &truncated);
         mu_int += mu_prev * len_prev;
         if (truncated) status = 1;
+        // positive floats order as their bits (the float of a float64 length
+        // may round up: the voxel test keeps a 2^10 margin on it)
+        if (t.len_min && lmin < __builtin_inff()) atomicMin(t.len_min, __float_as_uint(lmin));
         t.n[h] = got;   // <= the count the offsets were built from
     }
     t.lnatt[h] = mu_int;
@@ -199,9 +207,62 @@ __global__ __launch_bounds__(kRefreshWaves * kWave) void k_activity_refresh(Half
 __global__ __launch_bounds__(kRefreshWaves * kWave) void k_activity_check(HalfPathTable t, SampleTable s,
                                                                           int64_t n_hp,
                                                                           const double* __restrict__ lam,
-                                                                          int32_t* changed)
+                                                                          int32_t* changed, const int32_t* run_if)
 {
+    if (run_if && *run_if == 0) return;   // the same value for every thread
     activity_block<kActCheck>(t, s, n_hp, lam, changed);
+}
+
+// A new activity image has the fold table's terms when every voxel is zero
+// exactly where the table's image was, and every other voxel is a finite
+// value >= 2^-90 / (smallest positive sample length): then every product
+// lam L of a positive length is at least about 2^-90 (non-zero in float32
+// and float64), so the same samples keep non-zero terms and the same half
+// paths a non-zero integral.  tab and fast[1] come from the table's image
+// (k_vox_tab).  Any voxel that fails sets fast[0].
+__device__ __forceinline__ double vox_lam_safe(const uint32_t* len_min)
+{
+    return ldexp(1.0, -90) / (double)__uint_as_float(*len_min);
+}
+
+__global__ void k_vox_check(const double* __restrict__ lam, int64_t n_vox, const uint8_t* __restrict__ tab,
+                            const uint32_t* __restrict__ len_min, int32_t* fast)
+{
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (fast[1]) {
+        if (i0 == 0) fast[0] = 1;
+        return;
+    }
+    const double safe = vox_lam_safe(len_min);
+    bool bad = false;
+    for (int64_t v = i0; v < n_vox; v += (int64_t)gridDim.x * blockDim.x) {
+        const double l = lam[v];
+        bad |= tab[v] ? !(isfinite(l) && l >= safe) : !(l == 0.0);
+    }
+    if (bad) fast[0] = 1;
+}
+
+// The table's image: its non-zero voxels, and fast[1] = 1 when one of them is
+// not a finite value >= the safe level.  One workgroup, so the flag is
+// written once, after every voxel was seen.
+constexpr int kVoxTabThreads = 1024;
+
+__global__ __launch_bounds__(kVoxTabThreads) void k_vox_tab(const double* __restrict__ lam, int64_t n_vox,
+                                                            uint8_t* __restrict__ tab,
+                                                            const uint32_t* __restrict__ len_min, int32_t* fast,
+                                                            const int32_t* run_if)
+{
+    if (run_if && *run_if == 0) return;   // the same value for every thread
+    const double safe = vox_lam_safe(len_min);
+    int bad = 0;
+    for (int64_t v = threadIdx.x; v < n_vox; v += kVoxTabThreads) {
+        const double l = lam[v];
+        const bool nz = !(l == 0.0);
+        tab[v] = nz ? 1 : 0;
+        if (nz && !(isfinite(l) && l >= safe)) bad = 1;
+    }
+    bad = __syncthreads_or(bad);
+    if (threadIdx.x == 0) fast[1] = bad ? 1 : 0;
 }
 
 // Weights-only refresh of the fold table (same layout: the same terms are
@@ -381,11 +442,11 @@ hipError_t launch_activity_refresh(const HalfPathTable t, SampleTable s, int64_t
 }
 
 hipError_t launch_activity_check(const HalfPathTable t, SampleTable s, int64_t n_hp, const double* lam,
-                                 int32_t* changed, hipStream_t st)
+                                 int32_t* changed, hipStream_t st, const int32_t* run_if)
 {
     if (n_hp == 0) return hipSuccess;
     hipLaunchKernelGGL(k_activity_check, dim3(blocks_for(n_hp, kWave)), dim3(kRefreshWaves * kWave), 0, st, t, s,
-                       n_hp, lam, changed);
+                       n_hp, lam, changed, run_if);
     return hipGetLastError();
 }
 
@@ -400,8 +461,10 @@ hipError_t launch_fold_weights(SampleTable s, const int64_t* n_records, int64_t 
 
 // The FP32 pair evaluation's record of each half path (after the fold
 // table's offsets are scanned).
-__global__ void k_pair_half(HalfPathTable t, int64_t n_hp, bool one_bin)
+__global__ void k_pair_half(HalfPathTable t, int64_t n_hp, bool one_bin, const int32_t* run_if_a,
+                            const int32_t* run_if_b)
 {
+    if (run_if_a && run_if_b && *run_if_a == 0 && *run_if_b == 0) return;
     const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= n_hp) return;
     PairHalf r;
@@ -417,10 +480,28 @@ __global__ void k_pair_half(HalfPathTable t, int64_t n_hp, bool one_bin)
     t.rec[h] = r;
 }
 
-hipError_t launch_pair_half(const HalfPathTable t, int64_t n_hp, bool one_bin, hipStream_t st)
+hipError_t launch_pair_half(const HalfPathTable t, int64_t n_hp, bool one_bin, hipStream_t st,
+                            const int32_t* run_if_a, const int32_t* run_if_b)
 {
     if (n_hp == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_pair_half, dim3(blocks_for(n_hp, 256)), dim3(256), 0, st, t, n_hp, one_bin);
+    hipLaunchKernelGGL(k_pair_half, dim3(blocks_for(n_hp, 256)), dim3(256), 0, st, t, n_hp, one_bin, run_if_a,
+                       run_if_b);
+    return hipGetLastError();
+}
+
+hipError_t launch_vox_check(const double* lam, int64_t n_vox, const uint8_t* tab, const uint32_t* len_min,
+                            int32_t* fast, hipStream_t st)
+{
+    if (n_vox == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_vox_check, dim3((unsigned)std::min<int64_t>(1024, blocks_for(n_vox, 256))), dim3(256), 0, st,
+                       lam, n_vox, tab, len_min, fast);
+    return hipGetLastError();
+}
+
+hipError_t launch_vox_tab(const double* lam, int64_t n_vox, uint8_t* tab, const uint32_t* len_min, int32_t* fast,
+                          hipStream_t st, const int32_t* run_if)
+{
+    hipLaunchKernelGGL(k_vox_tab, dim3(1), dim3(kVoxTabThreads), 0, st, lam, n_vox, tab, len_min, fast, run_if);
     return hipGetLastError();
 }
 

@@ -431,3 +431,34 @@ def test_work_stats_count_the_reference_work(oracle):
         s.close()
     assert (got_pairs, got_samples) == (pairs, samples)
     assert 0 < pairs < w.n_rings_used ** 2 * w.n_det_used ** 2 * len(xyz)
+
+
+@pytest.mark.parametrize('precision', ['fp32', 'fp64'])
+def test_voxel_test_of_resident_images(precision):
+    """The voxel test that lets a rescaled image skip the per-sample checks:
+    images that keep the zero voxels take it; an image with a voxel far
+    below the safe level (a term that is zero in float32, not in float64), a
+    negative voxel, or a negative zero must fall back to the exact checks.
+    Every result equals a fresh subject's, bit for bit."""
+    w = phantoms.small_tof(seed=9, n_sp=24)
+    lam = w.activity
+    X, Y, Z = lam.shape
+    vox = np.argwhere(lam > 0)[len(np.argwhere(lam > 0)) // 2]
+    tiny = lam.copy()
+    tiny[tuple(vox)] = 1e-300
+    neg = lam.copy()
+    neg[tuple(vox)] = -0.5
+    negzero = np.where(lam == 0.0, -0.0, lam)
+    images = [lam, lam * 1.05, lam * 0.7, tiny, tiny * 1.3, lam * 0.9, neg, lam, negzero, lam * 1.2]
+    s = gpu_subject(w, precision)
+    try:
+        s.run(images[0])
+        for k, img in enumerate(images):
+            if k:
+                s.set_activity(img)
+                s.scatter()
+            got = s.output_compact()
+            want = gpu_compact(w, precision, lam=img)
+            assert np.array_equal(got, want), f'image {k}'
+    finally:
+        s.close()
