@@ -201,6 +201,46 @@ int sss_subject_query(sss_subject* s, int32_t* done);
  * lor_coords (N, N, 2) int16 = (angular, radial). */
 int sss_subject_output_dense(sss_subject* s, const int16_t* lor_coords, float* host_out);
 
+/* Batches: several traced subjects of one context driven as one unit, for
+ * the repeated estimates of reconstruction (a new activity image per
+ * iteration for every subject).  No counterpart in the reference, where each
+ * SingleScatterSimulationTOF call (SingleScatterSimulationTOF.py:36-208) is
+ * one instance that is traced and summed anew.
+ *
+ * One batch launch queues, in order: every member's staged activity upload
+ * and refresh; one persistent scatter launch per kernel variant present over
+ * all members of that variant (their workgroup blocks in 8 per-XCD work
+ * queues, the members with the most work per block first); one chunk-reduce
+ * launch per precision; every member's compact output download into pinned
+ * host memory.  sss_batch_graph_launch captures that sequence in a HIP graph
+ * on its first call (and again after a member's walks were redone) and
+ * replays the graph; sss_batch_submit queues the same sequence directly.
+ * Each member's output equals its own sss_subject_scatter output bit for bit.
+ *
+ * Members: traced, with a host activity image (sss_subject_set_activity),
+ * all on `ctx`; they must outlive the batch (sss_subject_destroy fails for a
+ * member of a live batch), and while a launch is in flight only the batch
+ * functions may use them. */
+typedef struct sss_batch sss_batch;
+int sss_batch_create(sss_ctx* ctx, sss_subject* const* members, int n_members, sss_batch** out);
+/* Member i's next activity image (host, X*Y*Z f64 in its volume_order),
+ * copied into its pinned staging before the call returns (after the batch's
+ * previous launch has read the last one). */
+int sss_batch_set_activity(sss_batch* b, int i, const double* lam);
+int sss_batch_submit(sss_batch* b);
+int sss_batch_graph_launch(sss_batch* b);
+/* Waits for the last launch. */
+int sss_batch_wait(sss_batch* b);
+/* Member i's compact output of the last launch (its layout and type as
+ * sss_subject_output_compact; waits for the launch). */
+int sss_batch_output(sss_batch* b, int i, void* host_out);
+/* Duration of the last launch (HIP events around it on the context stream). */
+int sss_batch_elapsed_ms(sss_batch* b, float* ms);
+/* Scatter launches per batch launch (kernel variants present), and whether
+ * a captured graph is current (1) or not (0). */
+int sss_batch_info(sss_batch* b, int32_t* n_scatter_launches, int32_t* graph_ready);
+int sss_batch_destroy(sss_batch* b);
+
 /* Device buffers on the context's device (synchronous copies). */
 int sss_buffer_alloc(sss_ctx* ctx, int64_t bytes, void** dev);
 int sss_buffer_free(sss_ctx* ctx, void* dev);

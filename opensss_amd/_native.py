@@ -103,6 +103,16 @@ SIGNATURES = [
     ('sss_subject_output_compact', ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ('sss_subject_output_compact_async', ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ('sss_subject_wait', ctypes.c_int, [ctypes.c_void_p]),
+    ('sss_batch_create', ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_void_p)]),
+    ('sss_batch_set_activity', ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    ('sss_batch_submit', ctypes.c_int, [ctypes.c_void_p]),
+    ('sss_batch_graph_launch', ctypes.c_int, [ctypes.c_void_p]),
+    ('sss_batch_wait', ctypes.c_int, [ctypes.c_void_p]),
+    ('sss_batch_output', ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    ('sss_batch_elapsed_ms', ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    ('sss_batch_info', ctypes.c_int, [ctypes.c_void_p, _c_i32p, _c_i32p]),
+    ('sss_batch_destroy', ctypes.c_int, [ctypes.c_void_p]),
     ('sss_subject_query', ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
     ('sss_subject_output_dense', ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ('sss_buffer_alloc', ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)]),

@@ -248,6 +248,33 @@ hipError_t launch_line_integrals(const Grid& g, const double* segs, int64_t n, c
 hipError_t launch_scatter(const ScatterArgs& a, bool fp64, bool trivial, hipStream_t st);
 hipError_t launch_pair_stats(const ScatterArgs& a, unsigned long long* out, hipStream_t st);
 hipError_t launch_chunk_reduce(const ScatterArgs& a, bool fp64, hipStream_t st);
+
+// Several subjects in one persistent scatter launch (k_scatter_queue).  The
+// work is workgroup blocks {subject, block of its item list}; block b of a
+// subject's list was laid out for XCD b % 8, so the blocks sit in 8 queues,
+// one per XCD, back to back in `blocks` (queue x: [q_begin[x], q_begin[x+1])).
+constexpr int kXcdCount = 8;
+struct BatchQueue {
+    const ScatterArgs* args;   // per subject
+    const int2* blocks;
+    const int32_t* q_begin;    // kXcdCount + 1
+    int32_t* counters;         // kXcdCount, zero before the launch
+};
+// Kernel variant of a subject's scatter (subjects of one variant can share a
+// launch), the persistent grid of a queue launch, and the launch itself
+// (a: any subject of the launch, for the variant).
+int scatter_variant(const ScatterArgs& a, bool fp64, bool trivial);
+hipError_t scatter_queue_grid(const ScatterArgs& a, bool fp64, bool trivial, int max_bins, int n_cu, int n_blocks,
+                              int* grid);
+hipError_t launch_scatter_queue(const ScatterArgs& a, bool fp64, bool trivial, const BatchQueue& q, int max_bins,
+                                int grid, hipStream_t st);
+// Chunk reduces of several subjects: blocks[x] = {subject, block of 256 of
+// its lane slots}; reduce_blocks gives a subject's block count.
+int reduce_blocks(const ScatterArgs& a);
+hipError_t launch_chunk_reduce_queue(const ScatterArgs* args, const int2* blocks, int n_blocks, int max_bins,
+                                     bool fp64, hipStream_t st);
+// n int32 values set to v by a kernel (graph captures record no memset)
+hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
 hipError_t launch_interp(const float* sampled, int n_bins, int n_sampled_planes, int64_t cells_s,
                          const int32_t* cell_map, int64_t n_cells_out, const int32_t* plane_map,
                          int n_out_planes, float* out, float* sum, hipStream_t st);
@@ -361,6 +388,35 @@ hipStream_t ctx_stream(const sss_ctx* ctx);
 // context's stream, after what was queued with it).
 int ctx_pool_get(sss_ctx* ctx, size_t bytes, void** p);
 void ctx_pool_put(sss_ctx* ctx, void* p);
+
+// What one subject brings to a batch launch (sss_batch.hip).  The subject's
+// tables must stay as they are while a batch holds it: `generation` changes
+// whenever its walks are redone, and a batch rebuilds its lists then.
+struct BatchMember {
+    ScatterArgs args;
+    bool fp64 = false, trivial = false;
+    int variant = -1;
+    int n_bins = 0;
+    int n_blocks = 0;          // workgroup blocks of its scatter item list
+    size_t out_bytes = 0;      // compact output
+    const void* d_out = nullptr;
+    double work = 0;           // estimated scatter work (for the queue order)
+    int generation = 0;
+};
+// Checks that the subject is traced and has a host activity image on its
+// context, brings its activity refresh to the form every later image takes
+// (queued refreshes of the current image), and describes it.
+int subject_batch_prepare(sss_subject* s, sss_ctx* ctx, BatchMember* m);
+int subject_generation(const sss_subject* s);
+// Batch membership count: a subject in a live batch is not destroyed.
+void subject_batch_ref(sss_subject* s, int delta);
+// Copies a host activity image into the subject's pinned staging copy
+// (after any queued upload from it has finished).
+int subject_stage_activity(sss_subject* s, const double* lam);
+// Queued on the subject's context stream: the staged image to the device
+// (transposed when the subject takes C-order volumes), then the activity
+// refresh.
+int subject_queue_activity(sss_subject* s);
 
 hipError_t launch_dense_scatter_typed(const void* compact, bool fp64, int n_bins, int nru, int ndu,
                                       int n_det, const int32_t* dets, const int16_t* lor,

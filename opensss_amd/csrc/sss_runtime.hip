@@ -388,6 +388,8 @@ struct sss_subject {
     hipEvent_t ev_count = nullptr;
     hipEvent_t ev_out = nullptr;          // after the last output download
     bool trace_begun = false;
+    int generation = 0;                   // walks done (a batch holding the subject rebuilds on change)
+    int batch_refs = 0;                   // live batches holding the subject
 
     ~sss_subject()
     {
@@ -846,6 +848,7 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
 int sss_subject_destroy(sss_subject* s)
 {
     if (!s) return SSS_OK;
+    if (s->batch_refs > 0) return fail(SSS_ERR_INVALID, "sss_subject_destroy: the subject is in a live batch");
     (void)hipSetDevice(s->ctx->device);
     // the device blocks go back to the context's cache: their next users are
     // queued behind this subject's work on the same stream; the pinned count
@@ -905,8 +908,9 @@ static int refresh_fold(sss_subject* s)
         // passes, the same samples keep non-zero terms and the same half
         // paths non-zero integrals, so the per-sample count check and the
         // pair records are skipped (on the device, no wait here)
-        SSS_HIP(hipMemsetAsync(s->d_fold_changed, 0, sizeof(int32_t), st));
-        SSS_HIP(hipMemsetAsync(s->d_fast, 0, sizeof(int32_t), st));
+        // flags cleared by a kernel: batches capture this sequence in a graph
+        SSS_HIP(launch_fill_i32(s->d_fold_changed, 1, 0, st));
+        SSS_HIP(launch_fill_i32(s->d_fast, 1, 0, st));
         SSS_HIP(launch_vox_check(s->d_lam, s->n_vox, s->d_tab, s->hp.len_min, s->d_fast, st));
         SSS_HIP(launch_activity_check(s->hp, smp, s->n_hp, s->d_lam, s->d_fold_changed, st, s->d_fast));
         SSS_HIP(launch_fold_weights(smp, s->hp.off_fold + s->n_hp, s->fold_cap, s->d_lam, s->d_fold_changed, st));
@@ -990,6 +994,7 @@ int sss_subject_trace_end(sss_subject* s)
         s->samples_cap = total;
     }
     s->n_samples = total;
+    ++s->generation;
     s->fold_valid = false;   // new walks: the next activity builds the fold table in full
     s->refreshes = 0;
     // the fill keeps the smallest positive length (0x7f7f7f7f: 3.4e38 as float bits)
@@ -1636,3 +1641,83 @@ int sss_scale_to_prompts(sss_ctx* ctx, const int32_t grid[3], const double bound
 }
 
 }  // extern "C"
+
+namespace sss {
+
+namespace {
+__global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+}  // namespace
+
+hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+int subject_batch_prepare(sss_subject* s, sss_ctx* ctx, BatchMember* m)
+{
+    if (s->ctx != ctx) return fail(SSS_ERR_INVALID, "batch: every subject must be on the batch's context");
+    if (!s->traced || s->trace_begun) return fail(SSS_ERR_INVALID, "batch: every subject must be traced");
+    if (!s->has_activity || !s->h_lam)
+        return fail(SSS_ERR_INVALID, "batch: every subject needs a host activity image (sss_subject_set_activity)");
+    SSS_HIP(hipSetDevice(ctx->device));
+    // the first two images after the walks build the fold table (the second
+    // one with its record -> sample map); every later image takes the same
+    // queued sequence, which is what a batch replays
+    while (!s->trivial && s->refreshes < 2)
+        if (int rc = refresh_fold(s)) return rc;
+    const bool fp64 = s->precision == SSS_PRECISION_FP64;
+    m->args = scatter_args(s);
+    m->fp64 = fp64;
+    m->trivial = s->trivial;
+    m->variant = scatter_variant(m->args, fp64, s->trivial);
+    m->n_bins = s->n_bins;
+    m->n_blocks = s->n_items / kScatterWaves;
+    m->out_bytes = (size_t)s->out_elems * (fp64 ? sizeof(double) : sizeof(float));
+    m->d_out = s->d_out;
+    // pairs x samples: one-bin subjects evaluate pairs; the others fold every
+    // sample of a half path once per detector cell it pairs with
+    const double cells = (double)s->nru * s->ndu;
+    m->work = (s->trivial ? (double)s->n_hp : (double)s->n_samples) * cells;
+    m->generation = s->generation;
+    return SSS_OK;
+}
+
+int subject_generation(const sss_subject* s) { return s->generation; }
+
+void subject_batch_ref(sss_subject* s, int delta) { s->batch_refs += delta; }
+
+int subject_stage_activity(sss_subject* s, const double* lam)
+{
+    SSS_HIP(hipSetDevice(s->ctx->device));
+    if (s->lam_queued) {
+        SSS_HIP(hipEventSynchronize(s->ev_lam));
+        s->lam_queued = false;
+    }
+    if (!s->h_lam) {
+        void* p = nullptr;
+        if (int rc = host_block(s->ctx, (size_t)s->n_vox * sizeof(double), &p)) return rc;
+        s->h_lam = static_cast<double*>(p);
+    }
+    std::memcpy(s->h_lam, lam, (size_t)s->n_vox * sizeof(double));
+    return SSS_OK;
+}
+
+int subject_queue_activity(sss_subject* s)
+{
+    hipStream_t st = s->ctx->stream;
+    SSS_HIP(hipMemcpyAsync(s->z_fastest ? s->d_vol_in : s->d_lam, s->h_lam, (size_t)s->n_vox * sizeof(double),
+                           hipMemcpyHostToDevice, st));
+    if (s->z_fastest)
+        SSS_HIP(launch_volume_x_fastest(s->d_vol_in, s->d_lam, s->grid.nx, s->grid.ny, s->grid.nz, st));
+    s->has_activity = true;
+    return refresh_fold(s);
+}
+
+}  // namespace sss

@@ -835,32 +835,21 @@ __device__ __forceinline__ void rel_path(const ScatterArgs& a, float* __restrict
     rel_flush(w, coef);
 }
 
-// At least 6 waves per SIMD (<= 80 VGPRs): the values that do not fit are
-// spilled outside the sample loops (per slice and per pair only).  Measured
-// on config 2: 4 waves 240 ms, 5 waves 203 ms, 6 waves 189 ms.
+// One wave's work item (tile, chunk) of the scatter sum.  LDS: one column
+// per lane; the one-bin FP32 path reads the energy table, which the caller
+// has placed at the start of LDS.  Padding items (tile -1) return at once.
 template <typename R, int HM, int TM, bool TRIVIAL, bool RANGE>
-__global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArgs a)
+__device__ __forceinline__ void scatter_item(const ScatterArgs& a, unsigned char* smem_raw, int wave,
+                                             int lane, int2 item)
 {
-    extern __shared__ unsigned char smem_raw[];
     constexpr bool SLIDE = TM == kTofSlide;
     constexpr bool REL = TM == kTofSlideRel || TM == kTofSlideRelWide;
     constexpr bool WIDE = TM == kTofSlideRelWide;
     static_assert(!SLIDE || (sizeof(R) == 4 && HM == 1 && !RANGE), "sliding window: float, h = 1, no range test");
     static_assert(!REL || (sizeof(R) == 4 && (HM == 1 || HM == 2) && !RANGE && (!WIDE || HM == 1)),
                   "relative sliding window: float, h = 1 or 2 (wide: h = 1), no range test");
-    // the wave index is uniform: held in a scalar register, so the work item,
-    // chunk and scatter-point loop below are scalar too
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int lane = threadIdx.x % kWave;
-    // LDS: one column per lane; the one-bin FP32 path: the energy table
     constexpr bool ONE_BIN_REC = TRIVIAL && sizeof(R) == 4;
-    float2* et = reinterpret_cast<float2*>(smem_raw);
-    if constexpr (ONE_BIN_REC) {
-        for (int i = threadIdx.x; i < kEnergyRows; i += blockDim.x) et[i] = a.etab[i];
-        __syncthreads();   // every wave of the workgroup, before any exits
-    }
-    // (tile, chunk) of this wave; padding items (tile -1) exit at once
-    const int2 item = a.items[blockIdx.x * kScatterWaves + wave];
+    const float2* et = reinterpret_cast<const float2*>(smem_raw);
     if (item.x < 0) return;
     const int tile = item.x;
     const int chunk = item.y;
@@ -982,6 +971,99 @@ __global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArg
     a.part_cnt[(int64_t)chunk * n_slots + slot] = cnt;
 }
 
+// At least 6 waves per SIMD (<= 80 VGPRs): the values that do not fit are
+// spilled outside the sample loops (per slice and per pair only).  Measured
+// on config 2: 4 waves 240 ms, 5 waves 203 ms, 6 waves 189 ms.  One work
+// item per wave.
+template <typename R, int HM, int TM, bool TRIVIAL, bool RANGE>
+__global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter(ScatterArgs a)
+{
+    // 16-byte aligned: the columns hold doubles (8-byte LDS atomics) and
+    // the energy table float2
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    // the wave index is uniform: held in a scalar register, so the work item,
+    // chunk and scatter-point loop are scalar too
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int lane = threadIdx.x % kWave;
+    if constexpr (TRIVIAL && sizeof(R) == 4) {
+        float2* et = reinterpret_cast<float2*>(smem_raw);
+        for (int i = threadIdx.x; i < kEnergyRows; i += blockDim.x) et[i] = a.etab[i];
+        __syncthreads();   // every wave of the workgroup, before any exits
+    }
+    scatter_item<R, HM, TM, TRIVIAL, RANGE>(a, smem_raw, wave, lane, a.items[blockIdx.x * kScatterWaves + wave]);
+}
+
+// Read-only device data that every lane of a wave reads at one address:
+// through the constant address space, so the loads are scalar (s_load) and
+// the values sit in scalar registers, as kernel arguments do.
+template <typename T>
+using ConstPtr = const __attribute__((address_space(4))) T*;
+
+template <typename T>
+__device__ __forceinline__ ConstPtr<T> as_const(const T* p)
+{
+    return (ConstPtr<T>)p;
+}
+
+// p[i] read through the constant address space, one dword at a time (the
+// copy constructors take generic references).
+template <typename T>
+__device__ __forceinline__ T load_const(const T* p, int64_t i)
+{
+    static_assert(sizeof(T) % 4 == 0, "whole dwords");
+    T v;
+    ConstPtr<uint32_t> src = as_const(reinterpret_cast<const uint32_t*>(p + i));
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&v);
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 4); ++k) dst[k] = src[k];
+    return v;
+}
+
+// Several subjects' scatter sums in one persistent launch (kernel variant
+// given by the template; every member of the launch uses it).  The work is
+// whole workgroup blocks of the members' own item lists (kScatterWaves waves
+// of one subject), in 8 queues, one per XCD: block b of a member's list was
+// laid out for XCD b % 8 (its tiles share rings there), and a workgroup takes
+// blocks from its own XCD's queue (workgroups reach the XCDs round-robin by
+// id, and the grid is a multiple of 8), then from the others once its own is
+// empty.  Each queue lists its members' blocks member by member, the members
+// with the most work per block first, so the short blocks fill the end of
+// the launch.  Results are those of k_scatter: a block's waves do exactly
+// the work they do there.
+template <typename R, int HM, int TM, bool TRIVIAL, bool RANGE>
+__global__ __launch_bounds__(kScatterWaves * kWave, 6) void k_scatter_queue(BatchQueue q)
+{
+    // after the static s_next: aligned as in k_scatter
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ int s_next;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int lane = threadIdx.x % kWave;
+    const int home = blockIdx.x % kXcdCount;
+    ConstPtr<int32_t> q_begin = as_const(q.q_begin);
+    for (int v = 0; v < kXcdCount; ++v) {
+        const int x = (home + v) % kXcdCount;
+        const int begin = q_begin[x];
+        const int n = q_begin[x + 1] - begin;
+        for (;;) {
+            if (threadIdx.x == 0) s_next = atomicAdd(q.counters + x, 1);
+            __syncthreads();
+            const int k = __builtin_amdgcn_readfirstlane(s_next);
+            __syncthreads();   // every wave has read s_next before it is written again
+            if (k >= n) break;
+            const int2 bl = load_const(q.blocks, begin + k);
+            const ScatterArgs a = load_const(q.args, bl.x);
+            if constexpr (TRIVIAL && sizeof(R) == 4) {
+                float2* et = reinterpret_cast<float2*>(smem_raw);
+                for (int i = threadIdx.x; i < kEnergyRows; i += blockDim.x) et[i] = a.etab[i];
+                __syncthreads();
+            }
+            scatter_item<R, HM, TM, TRIVIAL, RANGE>(a, smem_raw, wave, lane,
+                                                    a.items[bl.y * kScatterWaves + wave]);
+            __syncthreads();   // LDS (energy table, columns) is reused by the next block
+        }
+    }
+}
+
 // Counts the (scatter point, LOR) pairs that pass the reference's filters and
 // the voxel samples of their half paths with activity: the work the
 // algorithmic FLOP count F = sum_pairs [2 B (n1 [act1 != 0] + n2 [act2 != 0])
@@ -1042,17 +1124,18 @@ __device__ __forceinline__ void kahan_add(R& s, R& c, R v)
 // groups per slot keep enough waves in flight for the chunk loop's loads.
 constexpr int kReduceBins = 4;
 
+// One lane slot's cell, bins [bg kReduceBins, bg kReduceBins + kReduceBins).
 template <typename R>
-__global__ void k_chunk_reduce(ScatterArgs a)
+__device__ __forceinline__ void reduce_slot(const ScatterArgs& a, int64_t slot, int bg)
 {
     constexpr int BMAX = kReduceBins;
     const int64_t n_slots = (int64_t)a.n_tiles * kWave;
-    const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= n_slots) return;
+    const int b0 = bg * kReduceBins;
+    const int B = a.n_bins;
+    if (b0 >= B) return;
     const LaneCell lc = lane_cell(a, a.tiles[slot / kWave], (int)(slot % kWave));
     if (!lc.valid) return;
-    const int b0 = blockIdx.y * kReduceBins;
-    const int B = a.n_bins;
     const int nb = min(kReduceBins, B - b0);
     const R* __restrict__ part_sum = reinterpret_cast<const R*>(a.part_sum) + (int64_t)b0 * n_slots;
     R total[BMAX], comp[BMAX], num[BMAX];
@@ -1107,6 +1190,23 @@ __global__ void k_chunk_reduce(ScatterArgs a)
 }
 
 template <typename R>
+__global__ void k_chunk_reduce(ScatterArgs a)
+{
+    reduce_slot<R>(a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
+}
+
+// Several subjects' chunk reduces in one launch: block x reduces slots
+// [256 m.y, 256 m.y + 256) of subject m.x, m = blocks[x]; blockIdx.y is the
+// bin group (groups past a subject's bins return).
+template <typename R>
+__global__ void k_chunk_reduce_queue(const ScatterArgs* args, const int2* blocks)
+{
+    const int2 m = load_const(blocks, blockIdx.x);
+    const ScatterArgs a = load_const(args, m.x);
+    reduce_slot<R>(a, (int64_t)m.y * blockDim.x + threadIdx.x, blockIdx.y);
+}
+
+template <typename R>
 hipError_t launch_reduce_typed(const ScatterArgs& a, hipStream_t st)
 {
     const int64_t n_slots = (int64_t)a.n_tiles * kWave;
@@ -1141,61 +1241,148 @@ __global__ void k_dense_scatter(const R* __restrict__ compact, int n_bins, int n
     dense[((b * (n_det + 1) + rad) * na + ang) * planes + plane] = (float)compact[i];
 }
 
-template <typename R, int HM, int TM, bool TRIVIAL, bool RANGE>
-hipError_t launch_variant(const ScatterArgs& a, hipStream_t st)
-{
-    // one workgroup per kScatterWaves work items (the item list is padded to that)
-    dim3 grid((unsigned)(a.n_items / kScatterWaves));
-    size_t lds = TRIVIAL ? (sizeof(R) == 4 ? kEnergyRows * sizeof(float2) : 0)
-                         : (size_t)kScatterWaves * (a.n_bins + 2 * HM) * kWave * sizeof(R);
-    hipLaunchKernelGGL((k_scatter<R, HM, TM, TRIVIAL, RANGE>), grid,
-                       dim3(kScatterWaves * kWave), lds, st, a);
-    return hipGetLastError();
-}
+// The kernel variant of a subject's scatter sum: calls
+// f.template run<R, HM, TM, TRIVIAL, RANGE>() for the template that serves it.
+template <typename F>
+hipError_t dispatch_variant(const ScatterArgs& a, bool fp64, bool trivial, F& f);
 
-template <typename R, bool RANGE>
-hipError_t launch_hm(const ScatterArgs& a, hipStream_t st)
+template <typename R, bool RANGE, typename F>
+hipError_t dispatch_hm(const ScatterArgs& a, F& f)
 {
     const int h = a.half_window;
-    if (h == 0) return launch_variant<R, 0, kTofGeneral, false, RANGE>(a, st);
+    if (h == 0) return f.template run<R, 0, kTofGeneral, false, RANGE>();
     if constexpr (sizeof(R) == 4 && !RANGE) {
         if (a.tof_mode == kTofSlideRel) {
-            if (h == 1) return launch_variant<R, 1, kTofSlideRel, false, false>(a, st);
-            if (h == 2) return launch_variant<R, 2, kTofSlideRel, false, false>(a, st);
+            if (h == 1) return f.template run<R, 1, kTofSlideRel, false, false>();
+            if (h == 2) return f.template run<R, 2, kTofSlideRel, false, false>();
         }
-        if (a.tof_mode == kTofSlideRelWide && h == 1)
-            return launch_variant<R, 1, kTofSlideRelWide, false, false>(a, st);
+        if (a.tof_mode == kTofSlideRelWide && h == 1) return f.template run<R, 1, kTofSlideRelWide, false, false>();
     }
     if (h == 1) {
         if constexpr (!RANGE) {
             if (a.tof_mode == kTofOneExpInner) {
                 if constexpr (sizeof(R) == 4)
-                    return launch_variant<R, 1, kTofSlide, false, false>(a, st);
+                    return f.template run<R, 1, kTofSlide, false, false>();
                 else
-                    return launch_variant<R, 1, kTofOneExpInner, false, false>(a, st);
+                    return f.template run<R, 1, kTofOneExpInner, false, false>();
             }
         }
-        if (a.tof_mode == kTofOneExp) return launch_variant<R, 1, kTofOneExp, false, RANGE>(a, st);
-        return launch_variant<R, 1, kTofGeneral, false, RANGE>(a, st);
+        if (a.tof_mode == kTofOneExp) return f.template run<R, 1, kTofOneExp, false, RANGE>();
+        return f.template run<R, 1, kTofGeneral, false, RANGE>();
     }
-    if (h <= 2) return launch_variant<R, 2, kTofGeneral, false, RANGE>(a, st);
-    if (h <= 4) return launch_variant<R, 4, kTofGeneral, false, RANGE>(a, st);
-    return launch_variant<R, 8, kTofGeneral, false, RANGE>(a, st);
+    if (h <= 2) return f.template run<R, 2, kTofGeneral, false, RANGE>();
+    if (h <= 4) return f.template run<R, 4, kTofGeneral, false, RANGE>();
+    return f.template run<R, 8, kTofGeneral, false, RANGE>();
 }
 
-template <typename R>
-hipError_t launch_typed(const ScatterArgs& a, bool trivial, hipStream_t st)
+template <typename R, typename F>
+hipError_t dispatch_typed(const ScatterArgs& a, bool trivial, F& f)
 {
-    if (trivial) return launch_variant<R, 1, kTofGeneral, true, false>(a, st);
-    return a.range_check ? launch_hm<R, true>(a, st) : launch_hm<R, false>(a, st);
+    if (trivial) return f.template run<R, 1, kTofGeneral, true, false>();
+    return a.range_check ? dispatch_hm<R, true>(a, f) : dispatch_hm<R, false>(a, f);
 }
+
+template <typename F>
+hipError_t dispatch_variant(const ScatterArgs& a, bool fp64, bool trivial, F& f)
+{
+    return fp64 ? dispatch_typed<double>(a, trivial, f) : dispatch_typed<float>(a, trivial, f);
+}
+
+// Dynamic LDS of one scatter workgroup: the energy table (one-bin FP32), or
+// one column of B + 2 HM rows per lane.
+template <typename R, int HM, bool TRIVIAL>
+size_t scatter_lds(int n_bins)
+{
+    return TRIVIAL ? (sizeof(R) == 4 ? kEnergyRows * sizeof(float2) : 0)
+                   : (size_t)kScatterWaves * (n_bins + 2 * HM) * kWave * sizeof(R);
+}
+
+struct LaunchOne {
+    const ScatterArgs& a;
+    hipStream_t st;
+    template <typename R, int HM, int TM, bool TRIVIAL, bool RANGE>
+    hipError_t run()
+    {
+        // one workgroup per kScatterWaves work items (the item list is padded to that)
+        const dim3 grid((unsigned)(a.n_items / kScatterWaves));
+        const size_t lds = scatter_lds<R, HM, TRIVIAL>(a.n_bins);
+        hipLaunchKernelGGL((k_scatter<R, HM, TM, TRIVIAL, RANGE>), grid, dim3(kScatterWaves * kWave), lds, st, a);
+        return hipGetLastError();
+    }
+};
+
+struct VariantId {
+    int id = -1;
+    template <typename R, int HM, int TM, bool TRIVIAL, bool RANGE>
+    hipError_t run()
+    {
+        id = (sizeof(R) == 8 ? 1 : 0) + (TRIVIAL ? 2 : 0) + (RANGE ? 4 : 0) + 8 * HM + 128 * TM;
+        return hipSuccess;
+    }
+};
+
+struct QueueResident {
+    int max_bins;
+    int per_cu = 0;
+    template <typename R, int HM, int TM, bool TRIVIAL, bool RANGE>
+    hipError_t run()
+    {
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_scatter_queue<R, HM, TM, TRIVIAL, RANGE>,
+                                                            kScatterWaves * kWave,
+                                                            scatter_lds<R, HM, TRIVIAL>(max_bins));
+    }
+};
+
+struct LaunchQueue {
+    const BatchQueue& q;
+    int max_bins;
+    int grid;
+    hipStream_t st;
+    template <typename R, int HM, int TM, bool TRIVIAL, bool RANGE>
+    hipError_t run()
+    {
+        const size_t lds = scatter_lds<R, HM, TRIVIAL>(max_bins);
+        hipLaunchKernelGGL((k_scatter_queue<R, HM, TM, TRIVIAL, RANGE>), dim3((unsigned)grid),
+                           dim3(kScatterWaves * kWave), lds, st, q);
+        return hipGetLastError();
+    }
+};
 
 }  // namespace
 
 hipError_t launch_scatter(const ScatterArgs& a, bool fp64, bool trivial, hipStream_t st)
 {
     if (a.n_tiles == 0 || a.n_items < kScatterWaves) return hipSuccess;
-    return fp64 ? launch_typed<double>(a, trivial, st) : launch_typed<float>(a, trivial, st);
+    LaunchOne f{a, st};
+    return dispatch_variant(a, fp64, trivial, f);
+}
+
+int scatter_variant(const ScatterArgs& a, bool fp64, bool trivial)
+{
+    VariantId f;
+    (void)dispatch_variant(a, fp64, trivial, f);
+    return f.id;
+}
+
+hipError_t scatter_queue_grid(const ScatterArgs& a, bool fp64, bool trivial, int max_bins, int n_cu, int n_blocks,
+                              int* grid)
+{
+    QueueResident f{max_bins};
+    hipError_t e = dispatch_variant(a, fp64, trivial, f);
+    if (e != hipSuccess) return e;
+    // every resident workgroup, a multiple of 8 so that workgroup id % 8 is
+    // the XCD; no more than the blocks to run (rounded up to 8)
+    int g = std::max(1, f.per_cu) * std::max(1, n_cu);
+    g = std::min(g, (n_blocks + kXcdCount - 1) / kXcdCount * kXcdCount);
+    *grid = std::max(kXcdCount, g / kXcdCount * kXcdCount);
+    return hipSuccess;
+}
+
+hipError_t launch_scatter_queue(const ScatterArgs& a, bool fp64, bool trivial, const BatchQueue& q, int max_bins,
+                                int grid, hipStream_t st)
+{
+    LaunchQueue f{q, max_bins, grid, st};
+    return dispatch_variant(a, fp64, trivial, f);
 }
 
 hipError_t launch_pair_stats(const ScatterArgs& a, unsigned long long* out, hipStream_t st)
@@ -1210,6 +1397,24 @@ hipError_t launch_chunk_reduce(const ScatterArgs& a, bool fp64, hipStream_t st)
     if (a.n_tiles == 0) return hipSuccess;
     if (a.n_bins > kMaxBins) return hipErrorInvalidValue;
     return fp64 ? launch_reduce_typed<double>(a, st) : launch_reduce_typed<float>(a, st);
+}
+
+int reduce_blocks(const ScatterArgs& a)
+{
+    return (int)(((int64_t)a.n_tiles * kWave + 255) / 256);
+}
+
+hipError_t launch_chunk_reduce_queue(const ScatterArgs* args, const int2* blocks, int n_blocks, int max_bins,
+                                     bool fp64, hipStream_t st)
+{
+    if (n_blocks == 0) return hipSuccess;
+    if (max_bins > kMaxBins) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)n_blocks, (unsigned)((max_bins + kReduceBins - 1) / kReduceBins));
+    if (fp64)
+        hipLaunchKernelGGL(k_chunk_reduce_queue<double>, grid, dim3(256), 0, st, args, blocks);
+    else
+        hipLaunchKernelGGL(k_chunk_reduce_queue<float>, grid, dim3(256), 0, st, args, blocks);
+    return hipGetLastError();
 }
 
 hipError_t launch_dense_scatter_typed(const void* compact, bool fp64, int n_bins, int nru, int ndu,

@@ -168,7 +168,8 @@ class ScatterSubject:
     # -- lifecycle -------------------------------------------------------
     def close(self):
         if getattr(self, 'handle', None):
-            self._lib.sss_subject_destroy(self.handle)
+            # fails (and keeps the subject) while a live batch holds it
+            _native.check(self._lib.sss_subject_destroy(self.handle), 'sss_subject_destroy')
             self.handle = None
 
     def __del__(self):
