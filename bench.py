@@ -100,6 +100,7 @@ def batch_reading(kind, n, streams, fp64_sample=4):
         c.close()
     print(f'[bench] batched path {t_batch:.3f} s', file=sys.stderr, flush=True)
     same = all(np.array_equal(a, b) for a, b in zip(plain, batched))
+    graph = graph_reading(ws, water, plain, evals, t_plain)
     errs = []
     for i in range(0, len(ws), max(1, len(ws) // max(1, fp64_sample)))[:fp64_sample]:
         ref = run_plain([ws[i]], water, precision='fp64')[0]
@@ -112,8 +113,107 @@ def batch_reading(kind, n, streams, fp64_sample=4):
         'one_instance_path': {'value': evals / t_plain, 'seconds': t_plain},
         'batched_path': {'value': evals / t_batch, 'seconds': t_batch},
         'speedup': t_plain / t_batch, 'outputs_identical': bool(same),
+        'graph_batch': graph,
         'max_rel_err_vs_fp64': max(e['max_rel_err_cells_above_1pct_peak'] for e in errs) if errs else None,
         'fp64_checked_instances': len(errs),
+    }
+
+
+def graph_reading(ws, water, plain, evals, t_plain, iterations=3):
+    """The graph batch (opensss_amd.batch.run_graph_batch) on the same
+    instances: resident groups under 60% of the free device memory, one
+    persistent scatter launch per kernel variant per group, replayed as a
+    HIP graph.  Launch 1 takes the instances' own images (outputs compared
+    bit for bit with the one-instance path); launches 2..`iterations` take
+    new images (activity x (1 + 0.05 N(0,1)), the repeated estimates of a
+    reconstruction), timed against the one-instance path run once per
+    image."""
+    from opensss_amd import _native
+    from opensss_amd.batch import run_graph_batch
+    rng = np.random.default_rng(11)
+    fields = {}
+
+    def image(k, j):
+        if j == 0:
+            return ws[k].activity
+        if (k, j) not in fields:
+            fields[(k, j)] = ws[k].activity * np.clip(1.0 + 0.05 * rng.standard_normal(ws[k].activity.shape), 0.0, None)
+        return fields[(k, j)]
+
+    ctx = _native.Context(0)
+    try:
+        run_graph_batch(ws[:2], water, context=ctx)   # warm: kernels, graph code paths
+        timings = []
+        t0 = time.perf_counter()
+        got = run_graph_batch(ws, water, context=ctx, iterations=iterations, activity=image, timings=timings)
+        wall = time.perf_counter() - t0
+    finally:
+        ctx.close()
+    # the outputs are the last launch's: compare them with the one-instance
+    # path on those images for a few instances (launch 1 equals `plain` by
+    # the same kernels; tests/test_gpu_graph_batch.py checks every launch)
+    from opensss_amd.batch import run_plain
+    import dataclasses
+    check = list(range(0, len(ws), max(1, len(ws) // 4)))[:4]
+    same = all(np.array_equal(got[k], run_plain([dataclasses.replace(ws[k], activity=image(k, iterations - 1))],
+                                                water)[0]) for k in check)
+    first = sum(t['launch_ms'][0] for t in timings) / 1e3
+    later = sum(sum(t['launch_ms'][1:]) for t in timings) / 1e3
+    setup = sum(t['setup_s'] for t in timings)
+    per_image = later / max(1, iterations - 1)
+    print(f'[bench] graph batch: {len(timings)} resident groups, set-up {setup:.3f} s, first launches '
+          f'{first:.3f} s, later launches {per_image:.3f} s per image set', file=sys.stderr, flush=True)
+    return {
+        'groups': len(timings), 'iterations': iterations, 'wall_seconds': wall, 'setup_seconds': setup,
+        'first_launch_seconds': first, 'first_launch_value': evals / first,
+        'reestimate_seconds_per_image_set': per_image, 'reestimate_value': evals / per_image,
+        'reestimate_ratio_to_one_instance_path': t_plain / per_image,
+        'one_shot_ratio_to_one_instance_path': t_plain / (setup + first),
+        'outputs_identical_to_one_instance_path': bool(same), 'checked_instances': len(check),
+    }
+
+
+def config1_graph_reading(n=64, iterations=5):
+    """n config-1 subjects (one-bin, non-TOF; seeds 0..n-1) in one graph
+    batch: per image set, the graph launch against the one-instance path."""
+    from opensss_amd import _native, phantoms
+    from opensss_amd.batch import GraphBatch, _subject, run_plain
+    water = np.load(os.path.join(ROOT, 'opensss_amd', 'data', 'attenuation_water.npy'))
+    ws = [phantoms.config1(seed=seed) for seed in range(n)]
+    evals = sum(instance_evals(w) for w in ws)
+    run_plain(ws[:1], water)
+    t0 = time.perf_counter()
+    plain = run_plain(ws, water)
+    t_plain = time.perf_counter() - t0
+    ctx = _native.Context(0)
+    subs, batch = [], None
+    try:
+        for w in ws:
+            s = _subject(w, water, ctx, 'fp32')
+            subs.append(s)
+            s.set_activity(w.activity)
+            s.trace()
+        batch = GraphBatch(subs, ctx)
+        ms = []
+        for _ in range(max(2, iterations)):
+            batch.launch(graph=True)
+            ms.append(batch.elapsed_ms())
+        same = all(np.array_equal(batch.output(i), plain[i]) for i in range(n))
+        info = batch.info
+    finally:
+        if batch is not None:
+            batch.close()
+        for s in subs:
+            s.close()
+        ctx.close()
+    launch = float(np.median(ms[1:])) / 1e3
+    print(f'[bench] {n} config-1 subjects: one-instance path {t_plain:.4f} s, graph launch {launch * 1e3:.2f} ms',
+          file=sys.stderr, flush=True)
+    return {
+        'workload': f'config1-subjects-0..{n - 1}', 'subjects': n, 'evals': int(evals),
+        'one_instance_path': {'value': evals / t_plain, 'seconds': t_plain},
+        'graph_launch': {'value': evals / launch, 'seconds': launch, 'scatter_launches': info['scatter_launches']},
+        'ratio_to_one_instance_path': t_plain / launch, 'outputs_identical': bool(same),
     }
 
 
@@ -692,6 +792,7 @@ def main():
     batched = served = None
     if args.workload == 'config2' and args.default_batch > 0:
         batched = batch_reading('random', args.default_batch, args.batch_streams)
+        batched['config1_subjects'] = config1_graph_reading()
     if args.workload == 'config2' and args.default_serve > 0:
         served = serve_reading('random', args.subjects, args.rate, args.default_serve, args.streams)
 

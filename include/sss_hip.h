@@ -177,6 +177,12 @@ int sss_subject_scatter(sss_subject* s);
 /* Trace (if needed) + scatter in one call: the plain one-instance path. */
 int sss_subject_run(sss_subject* s);
 int sss_subject_info_get(sss_subject* s, sss_subject_info* out);
+/* Re-tiles the subject's scatter launch without redoing its walks:
+ * tile_rings and n_chunks as in sss_subject_desc (0: the defaults).  The
+ * tile shape does not change any output value; n_chunks changes the
+ * grouping in which a slice's chunk sums are added (FP32 rounding only).
+ * For tile-shape tuners (no counterpart in the reference). */
+int sss_subject_set_tiling(sss_subject* s, int tile_rings, int n_chunks);
 /* Pairs passing the reference's filters (A != 0, some activity, efficiency
  * != 0; SingleScatterSimulationTOF.py:451-509) and the voxel samples of their
  * half paths with activity (n1 [Act1 != 0] + n2 [Act2 != 0], every voxel as

@@ -103,6 +103,7 @@ SIGNATURES = [
     ('sss_subject_output_compact', ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ('sss_subject_output_compact_async', ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ('sss_subject_wait', ctypes.c_int, [ctypes.c_void_p]),
+    ('sss_subject_set_tiling', ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     ('sss_batch_create', ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_void_p)]),
     ('sss_batch_set_activity', ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),

@@ -282,7 +282,7 @@ def resident_groups(instances, memory_budget):
 
 
 def run_graph_batch(instances, attenuation_table, precision='fp32', context=None, memory_budget=None,
-                    graph=True, iterations=1, activity=None, timings=None):
+                    graph=True, iterations=1, activity=None, timings=None, tuner=None):
     """Compact outputs of every instance through `GraphBatch`.
 
     Instances go in resident groups (``resident_groups``; the budget
@@ -293,7 +293,9 @@ def run_graph_batch(instances, attenuation_table, precision='fp32', context=None
     image for launch j being ``activity(k, j)`` (default: the instance's own
     image), as the repeated estimates of a reconstruction do; the outputs
     are those of the last launch.  ``timings``, a list, receives one dict per
-    group: {'members', 'setup_s', 'launch_ms': [per launch]}.
+    group: {'members', 'setup_s', 'launch_ms': [per launch]}.  With a
+    ``tuner`` (``autotune.TileTuner``) each subject takes the tile shape
+    tuned for its shape key (timed on the first subject of a new key).
     """
     import time
     own = context is None
@@ -313,6 +315,8 @@ def run_graph_batch(instances, attenuation_table, precision='fp32', context=None
                     subs.append(s)
                     s.set_activity(inst.activity if activity is None else activity(k, 0))
                     s.trace()
+                    if tuner is not None:
+                        tuner.apply(s, inst, precision)
                 batch = GraphBatch(subs, ctx)
                 setup = time.perf_counter() - t0
                 ms = []

@@ -375,6 +375,9 @@ struct sss_subject {
     float* d_part_cnt = nullptr;          // [chunk][slot]
     int64_t n_slots = 0;
     std::vector<int32_t> dets;
+    std::vector<int32_t> slice_ptr_h;     // n_slices + 1 scatter-point bounds (host)
+    unsigned char* d_tiling = nullptr;    // tiles, items and chunks after sss_subject_set_tiling
+    size_t part_sum_cap = 0, part_cnt_cap = 0;   // elements allocated for the chunk partials
     // pinned host sources of the queued uploads (kept until destroy)
     void* h_upload = nullptr;
     double* h_lam = nullptr;
@@ -412,9 +415,77 @@ struct sss_subject {
         pool_free(c, hp.status);
         pool_free(c, smp.nq); pool_free(c, smp.vox); pool_free(c, smp.len); pool_free(c, smp.len32); pool_free(c, smp.a);
         pool_free(c, smp.a64); pool_free(c, smp.fold); pool_free(c, smp.fold_a64); pool_free(c, smp.fold_src);
-        pool_free(c, d_out); pool_free(c, d_part_sum); pool_free(c, d_part_cnt);
+        pool_free(c, d_out); pool_free(c, d_part_sum); pool_free(c, d_part_cnt); pool_free(c, d_tiling);
     }
 };
+
+namespace sss {
+namespace {
+
+// Scatter-launch layout of a subject: wave tiles, scatter-point chunks and
+// the XCD-aware work items.
+struct Tiling {
+    std::vector<int4> tiles;
+    std::vector<int32_t> chunk_sp, chunk_slice;
+    std::vector<int2> items;
+    int tile_db_log2 = 0;
+};
+
+}  // namespace
+}  // namespace sss
+
+static void make_tiling(const sss_subject* s, int tile_rings, int n_chunks, Tiling* t)
+{
+    // wave tiles: rows and columns are blocks of rb rings x db detectors
+    // (rb * db = 8, rb the largest power of two <= min(4, NRU) by default);
+    // detector blocks cover the upper triangle i1 <= i2
+    int rb = 1;
+    if (tile_rings > 0) {
+        rb = tile_rings;
+    } else {
+        while (rb * 2 <= std::min(kTileMaxRings, s->nru)) rb *= 2;
+    }
+    const int db = kTileRows / rb;
+    t->tile_db_log2 = 0;
+    while ((1 << t->tile_db_log2) < db) ++t->tile_db_log2;
+    t->tiles.clear();
+    const int ndb = (s->ndu + db - 1) / db;
+    for (int r1 = 0; r1 < s->nru; r1 += rb)
+        for (int r2 = 0; r2 < s->nru; r2 += rb)
+            for (int ti = 0; ti < ndb; ++ti)
+                for (int tj = ti; tj < ndb; ++tj) t->tiles.push_back(make_int4(r1, r2, ti * db, tj * db));
+    // scatter-point chunks: groups of at most G consecutive points of one
+    // slice (n_chunks > 0: about n_chunks groups in all).  Every chunk costs
+    // one partial-sum and one count array (at most 16 GiB in all).
+    const size_t elem = s->precision == SSS_PRECISION_FP64 ? sizeof(double) : sizeof(float);
+    const int64_t n_slots = (int64_t)t->tiles.size() * kWave;
+    const int64_t chunk_bytes = std::max<int64_t>(1, n_slots * (s->n_bins * (int64_t)elem + (int64_t)sizeof(float)));
+    const int64_t max_chunks = std::max<int64_t>(1, (int64_t(16) << 30) / chunk_bytes);
+    int64_t group = n_chunks > 0 ? (s->n_sp + n_chunks - 1) / n_chunks : kAutoChunkPoints;
+    group = std::max<int64_t>({group, 1, (s->n_sp + max_chunks - 1) / max_chunks});
+    t->chunk_sp.clear();
+    t->chunk_slice.clear();
+    const int n_slices = (int)s->slice_ptr_h.size() - 1;
+    for (int sl = 0; sl < n_slices; ++sl) {
+        const int64_t b = s->slice_ptr_h[sl], m = s->slice_ptr_h[sl + 1] - b;
+        const int64_t k = (m + group - 1) / group;
+        for (int64_t j = 0; j < k; ++j) {
+            t->chunk_sp.push_back((int32_t)(b + j * m / k));
+            t->chunk_slice.push_back(sl);
+        }
+    }
+    t->chunk_sp.push_back(s->n_sp);
+    t->items = xcd_work_items(t->tiles, s->nru, (int)t->chunk_slice.size(), kScatterWaves);
+}
+
+static void apply_tiling_sizes(sss_subject* s, const Tiling& t)
+{
+    s->tile_db_log2 = t.tile_db_log2;
+    s->n_tiles = (int)t.tiles.size();
+    s->n_slots = (int64_t)s->n_tiles * kWave;
+    s->n_chunks = (int)t.chunk_slice.size();
+    s->n_items = (int)t.items.size();
+}
 
 extern "C" {
 
@@ -691,51 +762,16 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
     const double tof_bound = tof_offset_bound(g, d->sp_xyz, d->n_sp, detpos.data(), d->nru * d->ndu);
     s->tof_bound = tof_bound;
     s->range_check = !(s->t_lo == -kTofHalf && s->t_hi == kTofHalf && tof_bound < kTofHalf - 2);
-    // wave tiles: rows and columns are blocks of rb rings x db detectors
-    // (rb * db = 8, rb the largest power of two <= min(4, NRU)); detector
-    // blocks cover the upper triangle i1 <= i2
-    int rb = 1;
-    if (d->tile_rings > 0) {
-        rb = d->tile_rings;
-    } else {
-        while (rb * 2 <= std::min(kTileMaxRings, d->nru)) rb *= 2;
-    }
-    const int db = kTileRows / rb;
-    s->tile_db_log2 = 0;
-    while ((1 << s->tile_db_log2) < db) ++s->tile_db_log2;
-    std::vector<int4> tiles;
-    const int ndb = (d->ndu + db - 1) / db;
-    for (int r1 = 0; r1 < d->nru; r1 += rb)
-        for (int r2 = 0; r2 < d->nru; r2 += rb)
-            for (int ti = 0; ti < ndb; ++ti)
-                for (int tj = ti; tj < ndb; ++tj)
-                    tiles.push_back(make_int4(r1, r2, ti * db, tj * db));
-    s->n_tiles = (int)tiles.size();
-    // scatter-point chunks: groups of at most G consecutive points of one
-    // slice.  The launch walks chunk-major, so small groups keep the half
-    // paths that the resident waves read inside each XCD's L2; every chunk
-    // costs one partial-sum and one count array (at most 16 GiB in all).
     const size_t elem = s->precision == SSS_PRECISION_FP64 ? sizeof(double) : sizeof(float);
     s->n_cells = (int64_t)d->nru * d->nru * s->n_cells_plane;
-    s->n_slots = (int64_t)s->n_tiles * kWave;
-    const int64_t chunk_bytes =
-        std::max<int64_t>(1, s->n_slots * (d->n_bins * (int64_t)elem + (int64_t)sizeof(float)));
-    const int64_t max_chunks = std::max<int64_t>(1, (int64_t(16) << 30) / chunk_bytes);
-    int64_t group = d->n_chunks > 0 ? (d->n_sp + d->n_chunks - 1) / d->n_chunks : kAutoChunkPoints;
-    group = std::max<int64_t>({group, 1, (d->n_sp + max_chunks - 1) / max_chunks});
-    std::vector<int32_t> chunk_sp, chunk_slice;
-    for (int sl = 0; sl < d->n_slices; ++sl) {
-        const int64_t b = d->slice_ptr[sl], m = d->slice_ptr[sl + 1] - b;
-        const int64_t k = (m + group - 1) / group;
-        for (int64_t j = 0; j < k; ++j) {
-            chunk_sp.push_back((int32_t)(b + j * m / k));
-            chunk_slice.push_back(sl);
-        }
-    }
-    chunk_sp.push_back(d->n_sp);
-    s->n_chunks = (int)chunk_slice.size();
-    const std::vector<int2> items = xcd_work_items(tiles, d->nru, s->n_chunks, kScatterWaves);
-    s->n_items = (int)items.size();
+    s->slice_ptr_h.assign(d->slice_ptr, d->slice_ptr + d->n_slices + 1);
+    Tiling tl;
+    make_tiling(s, d->tile_rings, d->n_chunks, &tl);
+    const std::vector<int4>& tiles = tl.tiles;
+    const std::vector<int32_t>& chunk_sp = tl.chunk_sp;
+    const std::vector<int32_t>& chunk_slice = tl.chunk_slice;
+    const std::vector<int2>& items = tl.items;
+    apply_tiling_sizes(s, tl);
     std::vector<float> ratio32(d->ratio, d->ratio + d->n_energy);
 
     // energy terms of the one-bin scatter: the rows a pair can reach, then row 0
@@ -800,6 +836,8 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
         return rc;
     }
     s->hp.len_min = reinterpret_cast<uint32_t*>(s->d_fast + 2);
+    s->part_sum_cap = (size_t)s->n_slots * s->n_bins * elem * s->n_chunks + 1;
+    s->part_cnt_cap = (size_t)s->n_slots * s->n_chunks + 1;
     hipStream_t st = ctx->stream;
     // The upload is queued behind whatever the context's stream holds, from
     // the pinned block that the subject keeps until destroy: nothing here
@@ -842,6 +880,77 @@ int sss_subject_create(sss_ctx* ctx, const sss_subject_desc* d, sss_subject** ou
         return hip_fail(e, "sss_subject_create: upload");
     }
     *out = s;
+    return SSS_OK;
+}
+
+int sss_subject_set_tiling(sss_subject* s, int tile_rings, int n_chunks)
+{
+    if (!s) return fail(SSS_ERR_INVALID, "sss_subject_set_tiling: NULL subject");
+    if (tile_rings != 0 && tile_rings != 1 && tile_rings != 2 && tile_rings != 4 && tile_rings != 8)
+        return fail(SSS_ERR_INVALID, "sss_subject_set_tiling: tile_rings must be 0, 1, 2, 4 or 8");
+    if (n_chunks < 0) return fail(SSS_ERR_INVALID, "sss_subject_set_tiling: n_chunks must be >= 0");
+    SSS_HIP(hipSetDevice(s->ctx->device));
+    sss_ctx* c = s->ctx;
+    hipStream_t st = c->stream;
+    Tiling t;
+    make_tiling(s, tile_rings, n_chunks, &t);
+    // one device block: tiles | items | chunk_sp | chunk_slice (256-byte slots)
+    auto slot = [](size_t b) { return (b + 16 + 255) / 256 * 256; };
+    const size_t b_tiles = t.tiles.size() * sizeof(int4), b_items = t.items.size() * sizeof(int2);
+    const size_t b_sp = t.chunk_sp.size() * sizeof(int32_t), b_sl = t.chunk_slice.size() * sizeof(int32_t);
+    const size_t o_items = slot(b_tiles), o_sp = o_items + slot(b_items), o_sl = o_sp + slot(b_sp);
+    std::vector<unsigned char> img(o_sl + slot(b_sl), 0);
+    std::memcpy(img.data(), t.tiles.data(), b_tiles);
+    std::memcpy(img.data() + o_items, t.items.data(), b_items);
+    std::memcpy(img.data() + o_sp, t.chunk_sp.data(), b_sp);
+    std::memcpy(img.data() + o_sl, t.chunk_slice.data(), b_sl);
+    unsigned char* blk = nullptr;
+    if (int rc = pool_alloc(c, &blk, img.size(), &s->device_bytes)) return rc;
+    const size_t elem = s->precision == SSS_PRECISION_FP64 ? sizeof(double) : sizeof(float);
+    const size_t n_slots = t.tiles.size() * kWave, n_ch = t.chunk_slice.size();
+    const size_t need_sum = n_slots * s->n_bins * elem * n_ch + 1, need_cnt = n_slots * n_ch + 1;
+    unsigned char* sum = nullptr;
+    float* cnt = nullptr;
+    if (need_sum > s->part_sum_cap) {
+        if (int rc = pool_alloc(c, &sum, need_sum, &s->device_bytes)) {
+            pool_free(c, blk);
+            return rc;
+        }
+    }
+    if (need_cnt > s->part_cnt_cap) {
+        if (int rc = pool_alloc(c, &cnt, need_cnt, &s->device_bytes)) {
+            pool_free(c, blk);
+            pool_free(c, sum);
+            return rc;
+        }
+    }
+    hipError_t e = hipMemcpyAsync(blk, img.data(), img.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);   // img is pageable and local
+    if (e != hipSuccess) {
+        pool_free(c, blk);
+        pool_free(c, sum);
+        pool_free(c, cnt);
+        return hip_fail(e, "sss_subject_set_tiling: upload");
+    }
+    // the old blocks go back to the cache; their queued readers ran above
+    pool_free(c, s->d_tiling);
+    s->d_tiling = blk;
+    if (sum) {
+        pool_free(c, s->d_part_sum);
+        s->d_part_sum = sum;
+        s->part_sum_cap = need_sum;
+    }
+    if (cnt) {
+        pool_free(c, s->d_part_cnt);
+        s->d_part_cnt = cnt;
+        s->part_cnt_cap = need_cnt;
+    }
+    s->d_tiles = reinterpret_cast<int4*>(blk);
+    s->d_items = reinterpret_cast<int2*>(blk + o_items);
+    s->d_chunk_sp = reinterpret_cast<int32_t*>(blk + o_sp);
+    s->d_chunk_slice = reinterpret_cast<int32_t*>(blk + o_sl);
+    apply_tiling_sizes(s, t);
+    ++s->generation;   // batches holding the subject rebuild their lists
     return SSS_OK;
 }
 

@@ -18,8 +18,9 @@ free up (continuous batching, not a queue drained batch by batch):
 - a subject is made resident only when its device-memory footprint fits the
   budget; idle subjects are evicted least recently used first to make room,
   and a request whose subject cannot fit yet waits;
-- with ``autotune``, a subject's first load times the scatter launch for
-  each candidate tile shape (``tune_tile_rings``) and keeps the fastest.
+- with ``autotune`` (or a ``tuner``), a subject takes the tile shape tuned
+  for its shape key (``autotune.TileTuner``); the first subject of a key
+  times the candidates on its resident tables.
 
 Every request's result is the one-instance path's result for the same
 subject and activity, bit for bit: the same kernels in the same order on
@@ -36,6 +37,7 @@ from typing import Any, Callable, Optional
 import numpy as np
 
 from . import _native
+from .autotune import TileTuner
 from .sss import ScatterSubject
 
 SSS_ERR_NOMEM = -3
@@ -65,38 +67,14 @@ def footprint_estimate(inst) -> int:
     return 4 * n_vox * 8 + n_hp * 120 + n_hp * walk * 48 + 3 * out
 
 
-def tune_tile_rings(inst, attenuation_table, candidates=(2, 4), precision='fp32', context=None, repeats=2):
-    """Tile shape of the scatter launch for one subject, picked by timing:
-    for each candidate (rings per tile block; 8 / rings detectors) a subject
-    is created, traced and given the instance's activity, and its scatter
-    kernel is timed (HIP events, best of `repeats`).  Results do not depend
-    on the tile shape.  Returns (best, {candidate: ms})."""
-    times = {}
-    for rb in candidates:
-        s = ScatterSubject(inst.attenuation, inst.image_size, inst.geometry, inst.normals, inst.det_size,
-                           attenuation_table, inst.energy_resolution, inst.energy_threshold,
-                           inst.n_rings_used, inst.n_det_used, inst.tof_resolution, inst.tof_range,
-                           inst.n_bins, scatter_points=inst.scatter_points, precision=precision,
-                           context=context, tile_rings=rb)
-        try:
-            s.run(inst.activity)
-            ms = []
-            for _ in range(max(1, repeats)):
-                s.scatter()
-                ms.append(s.scatter_kernel_ms())
-            times[rb] = min(ms)
-        finally:
-            s.close()
-    return min(times, key=times.get), times
-
-
 class ScatterServer:
     def __init__(self, instances: dict, attenuation_table, n_contexts: int = 2,
                  max_inflight: Optional[int] = None, hbm_budget: Optional[int] = None,
-                 precision: str = 'fp32', contexts=None, autotune: bool = False):
+                 precision: str = 'fp32', contexts=None, autotune: bool = False,
+                 tuner: Optional[TileTuner] = None):
         self.instances = dict(instances)
-        self.autotune = autotune
-        self.tile_rings: dict = {}          # key -> tile rings picked by the tuner
+        # tile shapes by timing (autotune), per subject shape
+        self.tuner = tuner if tuner is not None else (TileTuner() if autotune else None)
         self.table = attenuation_table
         self.precision = precision
         self._own_ctx = contexts is None
@@ -143,9 +121,6 @@ class ScatterServer:
                 return None
         if need > self.hbm_budget:
             raise MemoryError(f'subject {key!r} needs about {need} bytes, budget {self.hbm_budget}')
-        if self.autotune and key not in self.tile_rings:
-            self.tile_rings[key] = tune_tile_rings(inst, self.table, precision=self.precision,
-                                                   context=self._context(key))[0]
         while True:
             s = None
             try:
@@ -153,8 +128,10 @@ class ScatterServer:
                                    self.table, inst.energy_resolution, inst.energy_threshold, inst.n_rings_used,
                                    inst.n_det_used, inst.tof_resolution, inst.tof_range, inst.n_bins,
                                    scatter_points=inst.scatter_points, precision=self.precision,
-                                   context=self._context(key), tile_rings=self.tile_rings.get(key, 0))
+                                   context=self._context(key))
                 s.trace()
+                if self.tuner is not None:
+                    self.tuner.apply(s, inst, self.precision, activity=inst.activity)
                 break
             except _native.SssError as e:
                 if s is not None:

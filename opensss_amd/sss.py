@@ -192,6 +192,12 @@ class ScatterSubject:
         _native.check(self._lib.sss_subject_trace_end(self.handle), 'sss_subject_trace_end')
         self._traced = True
 
+    def set_tiling(self, tile_rings: int = 0, n_chunks: int = 0):
+        """Re-tile the scatter launch (rings per tile block, scatter-point
+        chunks; 0 = defaults) without redoing the walks."""
+        _native.check(self._lib.sss_subject_set_tiling(self.handle, int(tile_rings), int(n_chunks)),
+                      'sss_subject_set_tiling')
+
     def set_activity(self, ActivityMap):
         lam = np.asarray(ActivityMap)
         if tuple(lam.shape) != self.grid:

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from opensss_amd import _native, phantoms
+from opensss_amd.autotune import TileTuner
 from opensss_amd.batch import GraphBatch, resident_groups, run_graph_batch, run_plain
 from tests.sss_cases import WATER, gpu_compact, gpu_subject
 
@@ -161,8 +162,10 @@ def test_run_graph_batch_matches_one_instance_path():
     budget = 2 * max(footprint_estimate(w) for w in ws)
     assert len(resident_groups(ws, budget)) >= 2
     timings = []
-    got = run_graph_batch(ws, WATER, memory_budget=budget, iterations=2, timings=timings)
-    assert sum(len(t['members']) for t in timings) == len(ws)
+    # tile shapes from the tuner (whole-slice chunks: the same sums in the same order)
+    tuner = TileTuner(chunk_points=(0,))
+    got = run_graph_batch(ws, WATER, memory_budget=budget, iterations=2, timings=timings, tuner=tuner)
+    assert sum(len(t['members']) for t in timings) == len(ws) and tuner.timed_subjects >= 1
     for a, b in zip(got, plain):
         assert np.array_equal(a, b)
     big = [phantoms.random_instance(s) for s in (11, 34, 49)]

@@ -79,38 +79,65 @@ def test_admission_by_footprint_evicts_and_reloads():
         assert np.array_equal(r.output, _alone(subs, r)), f'request {r.tag} (subject {r.key})'
 
 
-def test_tile_shapes_give_identical_results_and_autotune_serves_them():
+def test_tile_shapes_give_identical_results_and_autotune_serves_them(tmp_path):
     """Every tile shape gives the same bits (each cell keeps its order of
-    sums); the server with autotune picks one by timing and serves the
-    one-instance path's results."""
-    from opensss_amd.batch import run_plain
-    from opensss_amd.serve import ScatterServer, serve, tune_tile_rings
+    sums), whether set at create or by re-tiling a traced subject; the tuner
+    times the candidates on a resident subject and keeps the fastest per
+    shape key (saved and reloaded through its JSON file); the server with
+    autotune serves the one-instance path's results with the tuned shape."""
+    from opensss_amd.autotune import TileTuner, shape_key
+    from opensss_amd.serve import ScatterServer, serve
     from opensss_amd.sss import ScatterSubject
     w = phantoms.small_tof(seed=21, n_sp=30, n_bins=7, n_rings=8, nru=8, ndu=16)
+
+    def subject(**kw):
+        return ScatterSubject(w.attenuation, w.image_size, w.geometry, w.normals, w.det_size, WATER,
+                              w.energy_resolution, w.energy_threshold, w.n_rings_used, w.n_det_used,
+                              w.tof_resolution, w.tof_range, w.n_bins, scatter_points=w.scatter_points, **kw)
+
     outs = []
     for rb in (1, 2, 4, 8):
-        s = ScatterSubject(w.attenuation, w.image_size, w.geometry, w.normals, w.det_size, WATER,
-                           w.energy_resolution, w.energy_threshold, w.n_rings_used, w.n_det_used,
-                           w.tof_resolution, w.tof_range, w.n_bins, scatter_points=w.scatter_points,
-                           tile_rings=rb)
+        s = subject(tile_rings=rb)
         try:
             s.run(w.activity)
             outs.append(s.output_compact())
         finally:
             s.close()
     assert all(np.array_equal(outs[0], o) for o in outs[1:])
-    best, times = tune_tile_rings(w, WATER)
-    assert best in (2, 4) and set(times) == {2, 4} and all(t > 0 for t in times.values())
+    s = subject()
+    try:
+        s.run(w.activity)
+        for rb in (8, 1, 2, 4, 0):
+            s.set_tiling(rb, 0)
+            s.scatter()
+            assert np.array_equal(s.output_compact(), outs[0]), rb
+        s.set_tiling(0, 10)   # shorter chunks: the slice sums regroup (FP32 rounding)
+        s.scatter()
+        assert s.info['n_chunks'] >= 10
+        np.testing.assert_allclose(s.output_compact(), outs[0], rtol=1e-5, atol=1e-6 * outs[0].max())
+        path = str(tmp_path / 'tiles.json')
+        tuner = TileTuner(path)
+        best = tuner.tune(s, w)
+        key = shape_key(w)
+        assert tuner.best[key] == best and len(tuner.times[key]) == len(tuner.candidates(w)) == 8
+        assert all(t > 0 for t in tuner.times[key].values())
+        assert TileTuner(path).best == {key: best}
+    finally:
+        s.close()
     subs = {'a': w}
     reqs = _requests(subs, 3, seed=5)
-    server = ScatterServer(subs, WATER, n_contexts=1, autotune=True)
+    server = ScatterServer(subs, WATER, n_contexts=1, tuner=TileTuner(path))
     try:
         serve(server, reqs)
-        assert server.tile_rings['a'] in (2, 4)
+        assert server.tuner.timed_subjects == 0   # the key was tuned above
     finally:
         server.close()
     for r in reqs:
-        assert np.array_equal(r.output, _alone(subs, r))
+        want = _alone(subs, r)
+        if best[1] == 0:
+            assert np.array_equal(r.output, want)
+        else:
+            np.testing.assert_allclose(r.output, want, rtol=1e-5, atol=1e-6 * want.max())
 
 
 def test_short_random_stream_served_equals_alone():

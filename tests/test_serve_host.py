@@ -41,6 +41,7 @@ class FakeSubject:
     download completes on the third poll after it is queued."""
     live = {}
     log = []
+    timed = []   # keys of the subjects whose scatter was timed
     sizes = {}   # device bytes per subject key (default 1000)
 
     def __init__(self, attenuation, *args, context=None, scatter_points=None, precision='fp32',
@@ -69,7 +70,11 @@ class FakeSubject:
     def scatter_kernel_ms(self):
         # the stand-in's fastest tile shape: 2 rings for even keys, 4 for odd
         best = 2 if self.key % 2 == 0 else 4
+        FakeSubject.timed.append(self.key)
         return 1.0 if self.tile_rings == best else 2.0
+
+    def set_tiling(self, tile_rings=0, n_chunks=0):
+        self.tile_rings = tile_rings
 
     def set_activity(self, lam):
         assert not self.busy, 'activity changed while a request is in flight'
@@ -115,6 +120,7 @@ def fake(monkeypatch):
     monkeypatch.setattr(serve, 'ScatterSubject', FakeSubject)
     FakeSubject.live = {}
     FakeSubject.log = []
+    FakeSubject.timed = []
     FakeSubject.sizes = {}
     return [FakeContext(), FakeContext()]
 
@@ -191,14 +197,20 @@ def test_subjects_spread_over_contexts(fake):
 
 
 def test_autotune_keeps_the_fastest_tile_shape(fake):
+    """Subjects of one shape key share the tuned shape: subject 2 has subject
+    0's key (8 rings x 16 detectors) and is not timed; subject 1 (24
+    detectors) has its own key."""
+    from opensss_amd.autotune import shape_key
     subs = {k: _instance(k) for k in range(3)}
-    for w in subs.values():
+    for k, w in subs.items():
         w.activity = np.ones((4, 4, 2))
+        w.n_rings_used, w.n_det_used = 8, (24 if k % 2 else 16)
     reqs = _requests([0, 1, 2, 0])
     server = serve.ScatterServer(subs, None, contexts=fake, autotune=True)
     serve.serve(server, reqs)
-    assert server.tile_rings == {0: 2, 1: 4, 2: 2}
+    assert server.tuner.best == {shape_key(subs[0]): (2, 0), shape_key(subs[1]): (4, 0)}
     assert [server.resident[k][0].tile_rings for k in range(3)] == [2, 4, 2]
+    assert 2 not in FakeSubject.timed and server.tuner.timed_subjects == 2
     for r in reqs:
         assert np.all(r.output == np.float32((r.key + 1) * float(r.activity.sum())))
     server.close()
