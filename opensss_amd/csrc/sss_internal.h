@@ -135,9 +135,11 @@ struct SampleTable {
     double* a64;       // lam * len in float64 (FP64 validation path only, else null)
     uint32_t* fold;    // kFoldGroupWords per group of kSampleGroup records
     double* fold_a64;  // FP64 path only
-    int32_t* fold_src; // per fold record: its sample entry (-1: padding), so a
-                       // new activity image with the same zero terms only
-                       // rewrites the weights
+    int2* fold_map;    // per fold record {voxel, FP32 subjects: the sample's
+                       // float length bits; FP64 subjects: its sample entry}
+                       // ({-1, 0}: padding), so a new activity image with the
+                       // same zero terms only rewrites the weights, reading
+                       // one coalesced word per record
 };
 constexpr int kFoldGroupWords = 3 * kSampleGroup;
 

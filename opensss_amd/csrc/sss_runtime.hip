@@ -414,7 +414,7 @@ struct sss_subject {
         pool_free(c, hp.att); pool_free(c, hp.lnatt); pool_free(c, hp.act); pool_free(c, hp.cosabs);
         pool_free(c, hp.status);
         pool_free(c, smp.nq); pool_free(c, smp.vox); pool_free(c, smp.len); pool_free(c, smp.len32); pool_free(c, smp.a);
-        pool_free(c, smp.a64); pool_free(c, smp.fold); pool_free(c, smp.fold_a64); pool_free(c, smp.fold_src);
+        pool_free(c, smp.a64); pool_free(c, smp.fold); pool_free(c, smp.fold_a64); pool_free(c, smp.fold_map);
         pool_free(c, d_out); pool_free(c, d_part_sum); pool_free(c, d_part_cnt); pool_free(c, d_tiling);
     }
 };
@@ -1011,8 +1011,8 @@ static int refresh_fold(sss_subject* s)
     // kept from the second image after the walks on (a subject that takes
     // one image per trace, as one reference call does, never needs it)
     SampleTable smp = s->smp;
-    if (s->refreshes++ == 0) smp.fold_src = nullptr;
-    if (s->fold_valid && smp.fold_src) {
+    if (s->refreshes++ == 0) smp.fold_map = nullptr;
+    if (s->fold_valid && smp.fold_map) {
         // A voxel test against the table's image decides first: when it
         // passes, the same samples keep non-zero terms and the same half
         // paths non-zero integrals, so the per-sample count check and the
@@ -1032,7 +1032,7 @@ static int refresh_fold(sss_subject* s)
     // the image a rebuilt table comes from, for the next images' voxel test
     SSS_HIP(launch_vox_tab(s->d_lam, s->n_vox, s->d_tab, s->hp.len_min, s->d_fast, st, run_if));
     SSS_HIP(launch_pair_half(s->hp, s->n_hp, false, st, run_if ? s->d_fast : nullptr, run_if));
-    s->fold_valid = smp.fold_src != nullptr;
+    s->fold_valid = smp.fold_map != nullptr;
     return SSS_OK;
 }
 
@@ -1076,7 +1076,7 @@ int sss_subject_trace_end(sss_subject* s)
         pool_free(c, s->smp.nq); pool_free(c, s->smp.vox); pool_free(c, s->smp.len); pool_free(c, s->smp.len32);
         pool_free(c, s->smp.a);
         pool_free(c, s->smp.a64); pool_free(c, s->smp.fold); pool_free(c, s->smp.fold_a64);
-        pool_free(c, s->smp.fold_src);
+        pool_free(c, s->smp.fold_map);
         s->samples_cap = 0;
         s->fold_cap = 0;
         // the fold table holds at most every sample plus the group padding,
@@ -1097,7 +1097,7 @@ int sss_subject_trace_end(sss_subject* s)
                               (rc = pool_alloc(c, &s->smp.fold_a64, fold_cap, tally)))) ||
             // sample entries of the fold records (int32 entries), for
             // weights-only refreshes
-            (fold && total < INT32_MAX && (rc = pool_alloc(c, &s->smp.fold_src, fold_cap, tally))))
+            (fold && total < INT32_MAX && (rc = pool_alloc(c, &s->smp.fold_map, fold_cap, tally))))
             return rc;
         s->fold_cap = fold ? (int64_t)fold_cap : 0;
         s->samples_cap = total;

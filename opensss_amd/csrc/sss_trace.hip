@@ -215,14 +215,18 @@ __global__ __launch_bounds__(kRefreshWaves * kWave) void k_activity_check(HalfPa
 
 // A new activity image has the fold table's terms when every voxel is zero
 // exactly where the table's image was, and every other voxel is a finite
-// value >= 2^-90 / (smallest positive sample length): then every product
-// lam L of a positive length is at least about 2^-90 (non-zero in float32
-// and float64), so the same samples keep non-zero terms and the same half
-// paths a non-zero integral.  tab and fast[1] come from the table's image
-// (k_vox_tab).  Any voxel that fails sets fast[0].
+// value >= 2^-148 / (smallest positive float32 sample length): then every
+// product lam L of such a length is at least 2^-148, so its float32 term is
+// not zero (these kernels keep float32 denormals: (float)(lam L) is zero
+// only for lam L <= 2^-150) and its float64 term neither, and the same
+// samples keep non-zero terms and the same half paths a non-zero integral.
+// Walks can produce lengths of 1e-30 cm (a ray that starts on a voxel
+// plane), so the level must not be much above the float32 underflow.  tab
+// and fast[1] come from the table's image (k_vox_tab).  Any voxel that fails
+// sets fast[0].
 __device__ __forceinline__ double vox_lam_safe(const uint32_t* len_min)
 {
-    return ldexp(1.0, -90) / (double)__uint_as_float(*len_min);
+    return ldexp(1.0, -148) / (double)__uint_as_float(*len_min);
 }
 
 __global__ void k_vox_check(const double* __restrict__ lam, int64_t n_vox, const uint8_t* __restrict__ tab,
@@ -267,18 +271,20 @@ __global__ __launch_bounds__(kVoxTabThreads) void k_vox_tab(const double* __rest
 
 // Weights-only refresh of the fold table (same layout: the same terms are
 // non-zero): record r gets the FP32 (and FP64) term of the sample it was
-// copied from.  A record whose term became zero sets *changed (with the
-// count check in k_activity_check, equal counts and all records non-zero
-// mean the same set of terms).
+// copied from, lam[voxel] * length as k_activity_refresh forms it.  A
+// record whose term became zero sets *changed (with the count check in
+// k_activity_check, equal counts and all records non-zero mean the same set
+// of terms).
 __global__ __launch_bounds__(256) void k_fold_weights(SampleTable s, const int64_t* __restrict__ n_records,
                                                       int64_t cap, const double* __restrict__ lam,
                                                       int32_t* changed)
 {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= cap || r >= *n_records) return;
-    const int32_t src = s.fold_src[r];
-    if (src < 0) return;   // padding record: weight 0 already
-    const double ar = lam[s.vox[src]] * sample_len(s, src);
+    const int2 m = s.fold_map[r];
+    if (m.x < 0) return;   // padding record: weight 0 already
+    const double len = s.fold_a64 ? s.len[m.y] : (double)__int_as_float(m.y);
+    const double ar = lam[m.x] * len;
     const bool keep = s.fold_a64 ? ar != 0.0 : (float)ar != 0.0f;
     if (!keep) atomicOr(changed, 1);
     s.fold[(r / kSampleGroup) * kFoldGroupWords + 2 * kSampleGroup + r % kSampleGroup] = __float_as_uint((float)ar);
@@ -354,7 +360,10 @@ __global__ __launch_bounds__(256) void k_fold_compact(HalfPathTable t, SampleTab
             const int at = kept + __popc(gm & ((1u << j) - 1));
             fold_store(s, dst + at, nq, a);
             if (s.fold_a64) s.fold_a64[dst + at] = a64;
-            if (s.fold_src) s.fold_src[dst + at] = (int32_t)(base + (int64_t)k * kWave);
+            if (s.fold_map) {
+                const int64_t i = base + (int64_t)k * kWave;
+                s.fold_map[dst + at] = make_int2(s.vox[i], s.fold_a64 ? (int32_t)i : __float_as_int(s.len32[i]));
+            }
         }
         if (gm) last_nq = __shfl(nq, g0 + 31 - __clz(gm), kWave);
         kept += __popc(gm);
@@ -363,7 +372,7 @@ __global__ __launch_bounds__(256) void k_fold_compact(HalfPathTable t, SampleTab
     if (kept + j < padded) {
         fold_store(s, dst + kept + j, last_nq, 0.0f);
         if (s.fold_a64) s.fold_a64[dst + kept + j] = 0.0;
-        if (s.fold_src) s.fold_src[dst + kept + j] = -1;
+        if (s.fold_map) s.fold_map[dst + kept + j] = make_int2(-1, 0);
     }
 }
 
