@@ -48,6 +48,9 @@ constexpr int kTofSlideRel = 4;
 // kTofSlideRel's window keeping only the side weight that ku points to,
 // 2^(|ku| - beta2) (the other is below 2^-95 of the centre)
 constexpr int kTofSlideRelWide = 5;
+// float only: kTofSlide with the row path's fold records shared by the four
+// lanes of a quad (each lane loads a third of a group, DPP broadcasts the rest)
+constexpr int kTofSlideQuad = 6;
 constexpr int kMaxHalfWindow = 8;
 
 void set_error(const std::string& msg);

@@ -1322,7 +1322,8 @@ int sss_subject_info_get(sss_subject* s, sss_subject_info* out)
     out->n_truncated_walks = trunc;
     const ScatterArgs a = scatter_args(s);
     out->tof_fold = (s->trivial || a.half_window == 0) ? kTofGeneral : a.tof_mode;
-    if (out->tof_fold == kTofOneExpInner && s->precision != SSS_PRECISION_FP64) out->tof_fold = kTofSlide;
+    if (out->tof_fold == kTofOneExpInner && s->precision != SSS_PRECISION_FP64)
+        out->tof_fold = kTofSlideQuad;
     return SSS_OK;
 }
 

@@ -220,12 +220,9 @@ __device__ __forceinline__ void pair_coef(const ScatterArgs& a, const PairHalf& 
 // pair_eval for the FP32 fold, from the two half paths' PairHalf records
 // (two 16-byte loads each, then the energy tables): the same filters and
 // the same float terms.
-__device__ __forceinline__ bool pair_eval_rec(const ScatterArgs& a, int sp, int d1, int d2, double lor2,
-                                              double eff_ref, Pair<float>& p)
+__device__ __forceinline__ bool pair_terms_rec(const ScatterArgs& a, const PairHalf& r1, const PairHalf& r2,
+                                               double lor2, double eff_ref, Pair<float>& p)
 {
-    const int64_t per_sp = (int64_t)a.nru * a.ndu;
-    const PairHalf r1 = a.hp_rec[sp * per_sp + d1];
-    const PairHalf r2 = a.hp_rec[sp * per_sp + d2];
     const int32_t f = r1.n_flags | r2.n_flags;
     if ((f & kHalfAttZero) || !(f & kHalfActive)) return false;
     double carg, s1q, s2q;
@@ -240,6 +237,13 @@ __device__ __forceinline__ bool pair_eval_rec(const ScatterArgs& a, int sp, int 
     p.b2 = r2.off;
     p.act1 = p.act2 = 0.0f;   // the one-bin path reads pair_eval's
     return true;
+}
+
+__device__ __forceinline__ bool pair_eval_rec(const ScatterArgs& a, int sp, int d1, int d2, double lor2,
+                                              double eff_ref, Pair<float>& p)
+{
+    const int64_t per_sp = (int64_t)a.nru * a.ndu;
+    return pair_terms_rec(a, a.hp_rec[sp * per_sp + d1], a.hp_rec[sp * per_sp + d2], lor2, eff_ref, p);
 }
 
 // Entry of the energy table in LDS for an energy row: rows kEnergyRowLo..
@@ -527,12 +531,13 @@ __device__ __forceinline__ float slide_kc_f(const ScatterArgs& a, int b)
 // move: a voxel step moves t by far less than a bin) adds the bin leaving the
 // window into LDS and shifts the other two; a longer move (after a run of
 // zero terms) flushes the window.
-template <int SIGN>
-__device__ __forceinline__ void slide_sample(const ScatterArgs& a, const TofConst<float>& d,
-                                             float* col, SlideWindow& win, int64_t x,
-                                             int64_t nq, float av, float coef, float ccoef, float skap)
+// av(): the sample's emission term, read after any centre move (the quad
+// fold's DPP broadcast then sits next to the multiply that takes it).
+template <int SIGN, typename AV>
+__device__ __forceinline__ void slide_step(const ScatterArgs& a, const TofConst<float>& d, float* col,
+                                           SlideWindow& win, int hi, AV av, float coef, float ccoef,
+                                           float skap)
 {
-    const int hi = (int)((x + nq) >> 32);
     if (hi < win.thr) {
         int b = win.cur + SIGN;
         int thr = slide_thr_f<SIGN>(a, b);
@@ -565,11 +570,19 @@ __device__ __forceinline__ void slide_sample(const ScatterArgs& a, const TofCons
     const float ku = __builtin_fmaf((float)hi, skap, win.kc);
     const float E = fast_exp2(ku);
     const float D = __builtin_fmaf(E, __builtin_fmaf(d.c, E, 1.0f), d.c);
-    const float g = av * fast_rcp(D);
+    const float g = av() * fast_rcp(D);
     const float gE = g * E;
     win.A += g;
     win.M += gE;
     win.P = __builtin_fmaf(gE, E, win.P);
+}
+
+template <int SIGN>
+__device__ __forceinline__ void slide_sample(const ScatterArgs& a, const TofConst<float>& d,
+                                             float* col, SlideWindow& win, int64_t x,
+                                             int64_t nq, float av, float coef, float ccoef, float skap)
+{
+    slide_step<SIGN>(a, d, col, win, (int)((x + nq) >> 32), [av] { return av; }, coef, ccoef, skap);
 }
 
 // Centre bin of the path's first sample (offset t, high word hi): the float
@@ -636,6 +649,154 @@ __device__ __forceinline__ void slide_path(const ScatterArgs& a, const TofConst<
         if (k <= 0) break;
     }
     slide_flush(win, coef, ccoef);
+}
+
+// Quad-shared sliding window (kTofSlideQuad): slide_path's arithmetic with
+// the fold records loaded once per quad.  The 8 lanes of a tile row read the
+// same 48-byte groups of their row path; here the four lanes of a quad each
+// load a third of every group (three of its 12 words, one 12-byte load
+// instead of three 16-byte loads) and take the other words from each other
+// by DPP quad broadcasts.  The four lanes of a quad must run the fold
+// together: the caller keeps whole quads active through it (same path, same
+// length).
+struct QuadWords {
+    uint32_t x, y, z;
+};
+
+// Word v of quad lane K, for every lane of the quad.
+template <int K>
+__device__ __forceinline__ uint32_t quad_bcast(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, K * 0x55, 0xf, 0xf, true);
+}
+
+// Or of a flag over the quad.
+__device__ __forceinline__ int quad_any(int f)
+{
+    f |= __builtin_amdgcn_mov_dpp(f, 0xb1, 0xf, 0xf, true);   // quad_perm [1,0,3,2]
+    f |= __builtin_amdgcn_mov_dpp(f, 0x4e, 0xf, 0xf, true);   // quad_perm [2,3,0,1]
+    return f;
+}
+
+// High word of x + nq for the 32.32 offset nq = (qhi of quad lane KH, qlo
+// of quad lane KL): the add with carry of the low words, then of the high
+// words, each reading its quad lane's word through DPP (the compiler does
+// not fold a DPP move into an add with carry).  The s_nop gives the two wait
+// states a DPP read needs after a VALU write of its source.
+template <int KL, int KH>
+__device__ __forceinline__ int quad_hi(uint32_t xlo, uint32_t xhi, uint32_t qlo, uint32_t qhi)
+{
+    uint32_t lo, hi;
+    asm volatile("s_nop 1\n\t"
+                 "v_add_co_u32_dpp %0, vcc, %2, %3 quad_perm:[%6,%6,%6,%6] row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+                 "v_addc_co_u32_dpp %1, vcc, %4, %5, vcc quad_perm:[%7,%7,%7,%7] row_mask:0xf bank_mask:0xf bound_ctrl:1"
+                 : "=&v"(lo), "=v"(hi)
+                 : "v"(qlo), "v"(xlo), "v"(qhi), "v"(xhi), "i"(KL), "i"(KH)
+                 : "vcc");
+    (void)lo;
+    return (int)hi;
+}
+
+// Sample J of a group from the quad's words: lane j holds words 3j..3j+2 of
+// {nq0 lo, nq0 hi, nq1 lo, nq1 hi, nq2 lo, nq2 hi, nq3 lo, nq3 hi, a0, a1, a2, a3}.
+template <int SIGN, int J>
+__device__ __forceinline__ void quad_sample(const ScatterArgs& a, const TofConst<float>& d, float* col,
+                                            SlideWindow& win, uint32_t xlo, uint32_t xhi, const QuadWords& w,
+                                            float coef, float ccoef, float skap)
+{
+    int hi;
+    uint32_t aw;
+    if constexpr (J == 0) {
+        hi = quad_hi<0, 0>(xlo, xhi, w.x, w.y); aw = w.z;
+    } else if constexpr (J == 1) {
+        hi = quad_hi<0, 1>(xlo, xhi, w.z, w.x); aw = w.x;
+    } else if constexpr (J == 2) {
+        hi = quad_hi<1, 1>(xlo, xhi, w.y, w.z); aw = w.y;
+    } else {
+        hi = quad_hi<2, 2>(xlo, xhi, w.x, w.y); aw = w.z;
+    }
+    constexpr int KA = J == 0 ? 2 : 3;
+    slide_step<SIGN>(a, d, col, win, hi, [aw] { return __uint_as_float(quad_bcast<KA>(aw)); }, coef, ccoef,
+                     skap);
+}
+
+template <int SIGN>
+__device__ __forceinline__ void quad_group(const ScatterArgs& a, const TofConst<float>& d, float* col,
+                                           SlideWindow& win, uint32_t xlo, uint32_t xhi, const QuadWords& w,
+                                           float coef, float ccoef, float skap)
+{
+    quad_sample<SIGN, 0>(a, d, col, win, xlo, xhi, w, coef, ccoef, skap);
+    quad_sample<SIGN, 1>(a, d, col, win, xlo, xhi, w, coef, ccoef, skap);
+    quad_sample<SIGN, 2>(a, d, col, win, xlo, xhi, w, coef, ccoef, skap);
+    quad_sample<SIGN, 3>(a, d, col, win, xlo, xhi, w, coef, ccoef, skap);
+}
+
+// slide_path for a quad: qj = lane % 4.  Groups are kFoldGroupWords = 4
+// QuadWords apart; two groups are in registers and the loads run two groups
+// past the path, as in slide_path.
+template <int SIGN>
+__device__ __forceinline__ void quad_path(const ScatterArgs& a, const TofConst<float>& d,
+                                          float* __restrict__ col, SlideWindow& win, int64_t base, int n,
+                                          int64_t shift, float coef, int qj)
+{
+    constexpr int kStride = kFoldGroupWords / 3;
+    const QuadWords* __restrict__ g =
+        reinterpret_cast<const QuadWords*>(a.s_fold + (base / kSampleGroup) * kFoldGroupWords) + qj;
+    const int64_t x = SIGN < 0 ? shift : ((int64_t)0xffffffffLL - shift);
+    const uint32_t xlo = (uint32_t)x, xhi = (uint32_t)((uint64_t)x >> 32);
+    const float skap = SIGN < 0 ? d.kap : -d.kap;
+    const float ccoef = coef * d.c;
+    QuadWords c = g[0];
+    QuadWords nx = g[kStride];
+    {
+        const int hi = quad_hi<0, 0>(xlo, xhi, c.x, c.y);
+        win.cur = slide_first<SIGN>(a, hi);
+        win.thr = slide_thr_f<SIGN>(a, win.cur);
+        win.kc = slide_kc_f(a, win.cur);
+        win.wp = col + (win.cur - 1 + a.slide_row0) * kWave;
+    }
+    for (int k = n;; ) {
+        quad_group<SIGN>(a, d, col, win, xlo, xhi, c, coef, ccoef, skap);
+        c = g[2 * kStride];
+        k -= kSampleGroup;
+        g += kStride;
+        if (k <= 0) break;
+        quad_group<SIGN>(a, d, col, win, xlo, xhi, nx, coef, ccoef, skap);
+        nx = g[2 * kStride];
+        k -= kSampleGroup;
+        g += kStride;
+        if (k <= 0) break;
+    }
+    slide_flush(win, coef, ccoef);
+}
+
+// One scatter point of a kTofSlideQuad wave item.  Every lane loads its
+// cell's two records (clamped detector indices keep invalid cells' loads in
+// bounds) and evaluates its pair.  The row path is folded by quads: the four
+// lanes of a quad share the cell row, so they fold one half path, and lanes
+// whose pair the reference skips fold with coefficient 0 and shift 0 so that
+// the quad stays whole.  The column path (8 lanes of a tile column, 8 apart,
+// which DPP quad broadcasts cannot join) is folded per lane as slide_path
+// does it.  Sharing the loads in one of the two folds balances the kernel:
+// with per-lane loads in both it is bound by the texture address and data
+// path; with quads in both (the column fold on the transposed cells) it is
+// bound by the VALU, which the DPP adds with carry cost one instruction a
+// sample more (38.2 and 38.8 ms at config 2, 36.3 ms as here).
+__device__ __forceinline__ void quad_point(const ScatterArgs& a, const TofConst<float>& tc, float* col,
+                                           SlideWindow& sl, int sp, int d1, int d2, double lor2, double eff_ref,
+                                           bool valid, int qj, float& cnt)
+{
+    const int64_t per_sp = (int64_t)a.nru * a.ndu;
+    const PairHalf r1 = a.hp_rec[sp * per_sp + d1];
+    const PairHalf r2 = a.hp_rec[sp * per_sp + d2];
+    Pair<float> p;
+    bool found = false;
+    if (valid) found = pair_terms_rec(a, r1, r2, lor2, eff_ref, p);
+    if (found) cnt += p.counts;
+    const int n1 = (r1.n_flags & kHalfActive) ? (r1.n_flags & kHalfCountMask) : 0;
+    if (n1 > 0 && quad_any(found ? 1 : 0))
+        quad_path<-1>(a, tc, col, sl, r1.off, n1, found ? p.shift : 0, found ? p.coef1 : 0.0f, qj);
+    if (found && p.n2 > 0) slide_path<+1>(a, tc, col, sl, p.b2, p.n2, p.shift, p.coef2);
 }
 
 // Sliding window in relative form (kTofSlideRel, float, h = H in {1, 2}).
@@ -842,7 +1003,8 @@ template <typename R, int HM, int TM, bool TRIVIAL, bool RANGE>
 __device__ __forceinline__ void scatter_item(const ScatterArgs& a, unsigned char* smem_raw, int wave,
                                              int lane, int2 item)
 {
-    constexpr bool SLIDE = TM == kTofSlide;
+    constexpr bool QUAD = TM == kTofSlideQuad;
+    constexpr bool SLIDE = TM == kTofSlide || QUAD;
     constexpr bool REL = TM == kTofSlideRel || TM == kTofSlideRelWide;
     constexpr bool WIDE = TM == kTofSlideRelWide;
     static_assert(!SLIDE || (sizeof(R) == 4 && HM == 1 && !RANGE), "sliding window: float, h = 1, no range test");
@@ -931,7 +1093,12 @@ __device__ __forceinline__ void scatter_item(const ScatterArgs& a, unsigned char
             }
         }
     }
-    if constexpr (!ONE_BIN_REC) {
+    if constexpr (QUAD) {
+        // every lane takes every point (whole quads through the row fold)
+        for (int sp = p_begin; sp < p_end; ++sp)
+            quad_point(a, tc, col, sl, sp, d1, d2, lor2, eff_ref, valid, lane % 4, cnt);
+    }
+    if constexpr (!ONE_BIN_REC && !QUAD) {
         for (int sp = p_begin; sp < p_end; ++sp) {
             Pair<R> p;
             bool found = false;
@@ -1262,7 +1429,7 @@ hipError_t dispatch_hm(const ScatterArgs& a, F& f)
         if constexpr (!RANGE) {
             if (a.tof_mode == kTofOneExpInner) {
                 if constexpr (sizeof(R) == 4)
-                    return f.template run<R, 1, kTofSlide, false, false>();
+                    return f.template run<R, 1, kTofSlideQuad, false, false>();
                 else
                     return f.template run<R, 1, kTofOneExpInner, false, false>();
             }

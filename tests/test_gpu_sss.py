@@ -108,12 +108,13 @@ def test_small_tof_workload_vs_oracle(oracle, precision, tol):
 
 # (bins, TOF resolution ps, TOF range ps) -> TOF fold of the FP32 and FP64 launches
 # (0 one exp2 per window bin, 1 one exp2 with end-bin masks, 2 one exp2 with the
-# window centre inside the bins, 3 as 2 with the sliding window).  The FP64
+# window centre inside the bins, 4 and 5 the relative sliding windows, 6 the
+# sliding window of 2 with the records shared per quad).  The FP64
 # path drops bins below 1e-16 of the nearest one (FP32: 1e-9), so its window
 # is wider at 214 ps.
 TOF_VARIANTS = {
-    'h1-15bins': ((15, 214.0, 3335.64), 3, 0),
-    'h1-14bins-even': ((14, 214.0, 3335.64), 3, 0),
+    'h1-15bins': ((15, 214.0, 3335.64), 6, 0),
+    'h1-14bins-even': ((14, 214.0, 3335.64), 6, 0),
     'h2-21bins': ((21, 214.0, 3335.64), 4, 0),
     'h2-17bins': ((17, 214.0, 3335.64), 4, 0),
     'h1-8bins': ((8, 214.0, 3335.64), 4, 0),
