@@ -113,7 +113,9 @@ typedef struct {
                                     on the nearest bin, one exp2 per sample (FP32, h = 1 or
                                     2), 5 as 4 keeping only the side weight that the
                                     offset points to (FP32, h = 1, bins far wider than the
-                                    kernel) */
+                                    kernel), 6 as 3 with the fold records of each cell row's
+                                    half path shared by the four lanes of a quad (FP32; what
+                                    the FP32 launch of 3's subjects runs) */
 } sss_subject_info;
 
 const char* sss_last_error(void);

@@ -38,7 +38,8 @@ constexpr int kTofGeneral = 0;            // one exp2 per window bin
 constexpr int kTofOneExp = 1;             // h = 1: one exp2, end bins masked
 constexpr int kTofOneExpInner = 2;        // h = 1: one exp2, window centre kept inside the bins
 // float only: kTofOneExpInner with a window that slides along a half path
-// (the centre bin moves one bin at a time, in one direction)
+// (the centre bin moves one bin at a time, in one direction); launched in its
+// kTofSlideQuad form
 constexpr int kTofSlide = 3;
 // float only, h = 1 or 2: a sliding window centred on the nearest bin with
 // the side weights in relative form, 2^(+-ku - beta2) (two exp2 per sample):

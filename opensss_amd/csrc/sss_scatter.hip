@@ -1004,7 +1004,7 @@ __device__ __forceinline__ void scatter_item(const ScatterArgs& a, unsigned char
                                              int lane, int2 item)
 {
     constexpr bool QUAD = TM == kTofSlideQuad;
-    constexpr bool SLIDE = TM == kTofSlide || QUAD;
+    constexpr bool SLIDE = QUAD;
     constexpr bool REL = TM == kTofSlideRel || TM == kTofSlideRelWide;
     constexpr bool WIDE = TM == kTofSlideRelWide;
     static_assert(!SLIDE || (sizeof(R) == 4 && HM == 1 && !RANGE), "sliding window: float, h = 1, no range test");
@@ -1110,9 +1110,6 @@ __device__ __forceinline__ void scatter_item(const ScatterArgs& a, unsigned char
                 cnt += p.counts;
                 if constexpr (TRIVIAL) {
                     num += p.coef1 * p.act1 + p.coef2 * p.act2;
-                } else if constexpr (SLIDE) {
-                    if (p.n1 > 0) slide_path<-1>(a, tc, col, sl, p.b1, p.n1, p.shift, p.coef1);
-                    if (p.n2 > 0) slide_path<+1>(a, tc, col, sl, p.b2, p.n2, p.shift, p.coef2);
                 } else if constexpr (REL) {
                     if (p.n1 > 0) rel_path<HM, -1, WIDE>(a, col, rw, p.b1, p.n1, p.shift, p.coef1);
                     if (p.n2 > 0) rel_path<HM, +1, WIDE>(a, col, rw, p.b2, p.n2, p.shift, p.coef2);
