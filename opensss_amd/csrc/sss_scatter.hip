@@ -851,12 +851,11 @@ __device__ __forceinline__ void rel_flush(RelWindow<H>& w, float coef)
     }
 }
 
-template <int H, int SIGN, bool WIDE>
-__device__ __forceinline__ void rel_sample(const ScatterArgs& a, float* col,
-                                           RelWindow<H>& w, int64_t x, int64_t nq, float av, float coef,
-                                           float skap)
+// av(): the sample's emission term, read after any centre move (as slide_step).
+template <int H, int SIGN, bool WIDE, typename AV>
+__device__ __forceinline__ void rel_step(const ScatterArgs& a, float* col, RelWindow<H>& w, int hi, AV av,
+                                         float coef, float skap)
 {
-    const int hi = (int)((x + nq) >> 32);
     if (hi < w.thr) {
         int b = w.cur + SIGN;
         int thr = slide_thr_f<SIGN, true>(a, b);
@@ -895,7 +894,7 @@ __device__ __forceinline__ void rel_sample(const ScatterArgs& a, float* col,
         const float ku = __builtin_fmaf(fh, skap, w.kc);
         const bool up = ku > 0.0f;
         const float t = fast_exp2(fminf(__builtin_fabsf(ku) - a.f_beta2, 30.0f)) * (up ? w.mp1 : w.mm1);
-        const float g = av * fast_rcp(1.0f + t);
+        const float g = av() * fast_rcp(1.0f + t);
         const float gt = g * t;
         w.A[1] += g;
         w.A[2] += up ? gt : 0.0f;
@@ -910,7 +909,7 @@ __device__ __forceinline__ void rel_sample(const ScatterArgs& a, float* col,
         const float E = fast_exp2(ku);
         const float tp = (w.cp1 * E) * E;
         const float D = (w.cm1 + E) + tp;
-        const float g = av * fast_rcp(D);
+        const float g = av() * fast_rcp(D);
         w.A[0] = __builtin_fmaf(g, w.cm1, w.A[0]);
         w.A[1] = __builtin_fmaf(g, E, w.A[1]);
         w.A[2] = __builtin_fmaf(g, tp, w.A[2]);
@@ -927,13 +926,21 @@ __device__ __forceinline__ void rel_sample(const ScatterArgs& a, float* col,
         const float tp1 = (w.cp1 * E2) * E;
         const float tp2 = (w.c4p * E2) * E2;
         const float D = ((w.c4m + tm1) + E2) + (tp1 + tp2);
-        const float g = av * fast_rcp(D);
+        const float g = av() * fast_rcp(D);
         w.A[0] = __builtin_fmaf(g, w.c4m, w.A[0]);
         w.A[1] = __builtin_fmaf(g, tm1, w.A[1]);
         w.A[2] = __builtin_fmaf(g, E2, w.A[2]);
         w.A[3] = __builtin_fmaf(g, tp1, w.A[3]);
         w.A[4] = __builtin_fmaf(g, tp2, w.A[4]);
     }
+}
+
+template <int H, int SIGN, bool WIDE>
+__device__ __forceinline__ void rel_sample(const ScatterArgs& a, float* col,
+                                           RelWindow<H>& w, int64_t x, int64_t nq, float av, float coef,
+                                           float skap)
+{
+    rel_step<H, SIGN, WIDE>(a, col, w, (int)((x + nq) >> 32), [av] { return av; }, coef, skap);
 }
 
 // Nearest bin of the path's first sample: the float estimate, then the
@@ -960,6 +967,18 @@ __device__ __forceinline__ void rel_group(const ScatterArgs& a, float* col,
     rel_sample<H, SIGN, WIDE>(a, col, w, x, nq_hi(q23), __uint_as_float(av.w), coef, skap);
 }
 
+// The window start of a relative-window path at high word hi.
+template <int H, int SIGN, bool WIDE>
+__device__ __forceinline__ void rel_start(const ScatterArgs& a, float* col, RelWindow<H>& w, int hi)
+{
+    const int b = rel_first<SIGN>(a, hi);
+    w.thr = slide_thr_f<SIGN, true>(a, b);
+    w.wp = col + (b - H + a.slide_row0) * kWave;
+    rel_centre<H, WIDE>(a, w, b);
+#pragma unroll
+    for (int i = 0; i < 2 * H + 1; ++i) w.A[i] = 0.0f;
+}
+
 // fold_path for kTofSlideRel: groups of 4 records, two groups in registers
 // (as slide_path).
 template <int H, int SIGN, bool WIDE>
@@ -972,15 +991,7 @@ __device__ __forceinline__ void rel_path(const ScatterArgs& a, float* __restrict
     const float skap = SIGN < 0 ? a.f_kap : -a.f_kap;
     uint4 c0 = g[0], c1 = g[1], c2 = g[2];
     uint4 n0 = g[3], n1 = g[4], n2 = g[5];
-    {
-        const int hi = (int)((x + nq_lo(c0)) >> 32);
-        const int b = rel_first<SIGN>(a, hi);
-        w.thr = slide_thr_f<SIGN, true>(a, b);
-        w.wp = col + (b - H + a.slide_row0) * kWave;
-        rel_centre<H, WIDE>(a, w, b);
-#pragma unroll
-        for (int i = 0; i < 2 * H + 1; ++i) w.A[i] = 0.0f;
-    }
+    rel_start<H, SIGN, WIDE>(a, col, w, (int)((x + nq_lo(c0)) >> 32));
     for (int k = n;; ) {
         rel_group<H, SIGN, WIDE>(a, col, w, x, c0, c1, c2, coef, skap);
         c0 = g[6]; c1 = g[7]; c2 = g[8];
@@ -994,6 +1005,84 @@ __device__ __forceinline__ void rel_path(const ScatterArgs& a, float* __restrict
         if (k <= 0) break;
     }
     rel_flush(w, coef);
+}
+
+// rel_path for a quad (quad_path's loads and broadcasts, rel_step's window).
+template <int H, int SIGN, bool WIDE, int J>
+__device__ __forceinline__ void quad_rel_sample(const ScatterArgs& a, float* col, RelWindow<H>& w, uint32_t xlo,
+                                                uint32_t xhi, const QuadWords& q, float coef, float skap)
+{
+    int hi;
+    uint32_t aw;
+    if constexpr (J == 0) {
+        hi = quad_hi<0, 0>(xlo, xhi, q.x, q.y); aw = q.z;
+    } else if constexpr (J == 1) {
+        hi = quad_hi<0, 1>(xlo, xhi, q.z, q.x); aw = q.x;
+    } else if constexpr (J == 2) {
+        hi = quad_hi<1, 1>(xlo, xhi, q.y, q.z); aw = q.y;
+    } else {
+        hi = quad_hi<2, 2>(xlo, xhi, q.x, q.y); aw = q.z;
+    }
+    constexpr int KA = J == 0 ? 2 : 3;
+    rel_step<H, SIGN, WIDE>(a, col, w, hi, [aw] { return __uint_as_float(quad_bcast<KA>(aw)); }, coef, skap);
+}
+
+template <int H, int SIGN, bool WIDE>
+__device__ __forceinline__ void quad_rel_group(const ScatterArgs& a, float* col, RelWindow<H>& w, uint32_t xlo,
+                                               uint32_t xhi, const QuadWords& q, float coef, float skap)
+{
+    quad_rel_sample<H, SIGN, WIDE, 0>(a, col, w, xlo, xhi, q, coef, skap);
+    quad_rel_sample<H, SIGN, WIDE, 1>(a, col, w, xlo, xhi, q, coef, skap);
+    quad_rel_sample<H, SIGN, WIDE, 2>(a, col, w, xlo, xhi, q, coef, skap);
+    quad_rel_sample<H, SIGN, WIDE, 3>(a, col, w, xlo, xhi, q, coef, skap);
+}
+
+template <int H, int SIGN, bool WIDE>
+__device__ __forceinline__ void quad_rel_path(const ScatterArgs& a, float* __restrict__ col, RelWindow<H>& w,
+                                              int64_t base, int n, int64_t shift, float coef, int qj)
+{
+    constexpr int kStride = kFoldGroupWords / 3;
+    const QuadWords* __restrict__ g =
+        reinterpret_cast<const QuadWords*>(a.s_fold + (base / kSampleGroup) * kFoldGroupWords) + qj;
+    const int64_t x = SIGN < 0 ? shift : ((int64_t)0xffffffffLL - shift);
+    const uint32_t xlo = (uint32_t)x, xhi = (uint32_t)((uint64_t)x >> 32);
+    const float skap = SIGN < 0 ? a.f_kap : -a.f_kap;
+    QuadWords c = g[0];
+    QuadWords nx = g[kStride];
+    rel_start<H, SIGN, WIDE>(a, col, w, quad_hi<0, 0>(xlo, xhi, c.x, c.y));
+    for (int k = n;; ) {
+        quad_rel_group<H, SIGN, WIDE>(a, col, w, xlo, xhi, c, coef, skap);
+        c = g[2 * kStride];
+        k -= kSampleGroup;
+        g += kStride;
+        if (k <= 0) break;
+        quad_rel_group<H, SIGN, WIDE>(a, col, w, xlo, xhi, nx, coef, skap);
+        nx = g[2 * kStride];
+        k -= kSampleGroup;
+        g += kStride;
+        if (k <= 0) break;
+    }
+    rel_flush(w, coef);
+}
+
+// quad_point for the relative windows: the row path on quads, the column
+// path per lane.
+template <int H, bool WIDE>
+__device__ __forceinline__ void quad_rel_point(const ScatterArgs& a, float* col, RelWindow<H>& w, int sp,
+                                               int d1, int d2, double lor2, double eff_ref, bool valid, int qj,
+                                               float& cnt)
+{
+    const int64_t per_sp = (int64_t)a.nru * a.ndu;
+    const PairHalf r1 = a.hp_rec[sp * per_sp + d1];
+    const PairHalf r2 = a.hp_rec[sp * per_sp + d2];
+    Pair<float> p;
+    bool found = false;
+    if (valid) found = pair_terms_rec(a, r1, r2, lor2, eff_ref, p);
+    if (found) cnt += p.counts;
+    const int n1 = (r1.n_flags & kHalfActive) ? (r1.n_flags & kHalfCountMask) : 0;
+    if (n1 > 0 && quad_any(found ? 1 : 0))
+        quad_rel_path<H, -1, WIDE>(a, col, w, r1.off, n1, found ? p.shift : 0, found ? p.coef1 : 0.0f, qj);
+    if (found && p.n2 > 0) rel_path<H, +1, WIDE>(a, col, w, p.b2, p.n2, p.shift, p.coef2);
 }
 
 // One wave's work item (tile, chunk) of the scatter sum.  LDS: one column
@@ -1098,7 +1187,17 @@ __device__ __forceinline__ void scatter_item(const ScatterArgs& a, unsigned char
         for (int sp = p_begin; sp < p_end; ++sp)
             quad_point(a, tc, col, sl, sp, d1, d2, lor2, eff_ref, valid, lane % 4, cnt);
     }
-    if constexpr (!ONE_BIN_REC && !QUAD) {
+    // h = 1 relative windows fold the row path on quads; the h = 2 window's
+    // sample costs more VALU than the quad's offset broadcast saves in loads
+    // (random instances with 17-21 bins 3-6% slower on quads, 5-11 bins 2-7%
+    // faster)
+    constexpr bool QREL = REL && HM == 1;
+    if constexpr (QREL) {
+        // every lane takes every point (whole quads through the row fold)
+        for (int sp = p_begin; sp < p_end; ++sp)
+            quad_rel_point<1, WIDE>(a, col, rw, sp, d1, d2, lor2, eff_ref, valid, lane % 4, cnt);
+    }
+    if constexpr (!ONE_BIN_REC && !QUAD && !QREL) {
         for (int sp = p_begin; sp < p_end; ++sp) {
             Pair<R> p;
             bool found = false;
