@@ -570,7 +570,9 @@ __device__ __forceinline__ void slide_step(const ScatterArgs& a, const TofConst<
     const float ku = __builtin_fmaf((float)hi, skap, win.kc);
     const float E = fast_exp2(ku);
     const float D = __builtin_fmaf(E, __builtin_fmaf(d.c, E, 1.0f), d.c);
-    const float g = av() * fast_rcp(D);
+    float g = av() * fast_rcp(D);
+    // the term's one use: the quad fold's DPP broadcast folds into this multiply
+    asm volatile("" : "+v"(g));
     const float gE = g * E;
     win.A += g;
     win.M += gE;
@@ -894,7 +896,8 @@ __device__ __forceinline__ void rel_step(const ScatterArgs& a, float* col, RelWi
         const float ku = __builtin_fmaf(fh, skap, w.kc);
         const bool up = ku > 0.0f;
         const float t = fast_exp2(fminf(__builtin_fabsf(ku) - a.f_beta2, 30.0f)) * (up ? w.mp1 : w.mm1);
-        const float g = av() * fast_rcp(1.0f + t);
+        float g = av() * fast_rcp(1.0f + t);
+        asm volatile("" : "+v"(g));
         const float gt = g * t;
         w.A[1] += g;
         w.A[2] += up ? gt : 0.0f;
@@ -909,7 +912,8 @@ __device__ __forceinline__ void rel_step(const ScatterArgs& a, float* col, RelWi
         const float E = fast_exp2(ku);
         const float tp = (w.cp1 * E) * E;
         const float D = (w.cm1 + E) + tp;
-        const float g = av() * fast_rcp(D);
+        float g = av() * fast_rcp(D);
+        asm volatile("" : "+v"(g));
         w.A[0] = __builtin_fmaf(g, w.cm1, w.A[0]);
         w.A[1] = __builtin_fmaf(g, E, w.A[1]);
         w.A[2] = __builtin_fmaf(g, tp, w.A[2]);
