@@ -923,19 +923,22 @@ __device__ __forceinline__ void rel_step(const ScatterArgs& a, float* col, RelWi
         // nearest-bin centre |ku| <= beta2, and beta2 < 30 keeps every term
         // a normal float; past the end bins ku is held to +-(beta2 + 10),
         // where the far side's terms weigh under 2^-(2 beta2 + 10).
+        // D by Horner in E, the weights g E^k by a chain of products (the
+        // masked constants multiply last, so a masked side never forms
+        // g E^4 unscaled: every factor stays finite)
         const float ku = __builtin_amdgcn_fmed3f(__builtin_fmaf(fh, skap, w.kc), -a.f_kclamp, a.f_kclamp);
         const float E = fast_exp2(ku);
-        const float E2 = E * E;
-        const float tm1 = w.cm1 * E;
-        const float tp1 = (w.cp1 * E2) * E;
-        const float tp2 = (w.c4p * E2) * E2;
-        const float D = ((w.c4m + tm1) + E2) + (tp1 + tp2);
+        const float D = __builtin_fmaf(
+            E, __builtin_fmaf(E, __builtin_fmaf(E, __builtin_fmaf(E, w.c4p, w.cp1), 1.0f), w.cm1), w.c4m);
         const float g = av() * fast_rcp(D);
+        const float g1 = g * E;
+        const float g2 = g1 * E;
+        const float g3 = g2 * E;
         w.A[0] = __builtin_fmaf(g, w.c4m, w.A[0]);
-        w.A[1] = __builtin_fmaf(g, tm1, w.A[1]);
-        w.A[2] = __builtin_fmaf(g, E2, w.A[2]);
-        w.A[3] = __builtin_fmaf(g, tp1, w.A[3]);
-        w.A[4] = __builtin_fmaf(g, tp2, w.A[4]);
+        w.A[1] = __builtin_fmaf(g1, w.cm1, w.A[1]);
+        w.A[2] += g2;
+        w.A[3] = __builtin_fmaf(g3, w.cp1, w.A[3]);
+        w.A[4] = __builtin_fmaf(g3, E * w.c4p, w.A[4]);
     }
 }
 
