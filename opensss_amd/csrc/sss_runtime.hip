@@ -1030,7 +1030,9 @@ static int refresh_fold(sss_subject* s)
     if (int rc = scan_counts(s, padded, s->hp.off_fold, s->n_hp, st)) return rc;
     SSS_HIP(launch_fold_compact(s->hp, smp, s->n_hp, st, run_if));
     // the image a rebuilt table comes from, for the next images' voxel test
-    SSS_HIP(launch_vox_tab(s->d_lam, s->n_vox, s->d_tab, s->hp.len_min, s->d_fast, st, run_if));
+    // (which needs the record map, so the first image after the walks, whose
+    // table keeps none, does not need it either)
+    if (smp.fold_map) SSS_HIP(launch_vox_tab(s->d_lam, s->n_vox, s->d_tab, s->hp.len_min, s->d_fast, st, run_if));
     SSS_HIP(launch_pair_half(s->hp, s->n_hp, false, st, run_if ? s->d_fast : nullptr, run_if));
     s->fold_valid = smp.fold_map != nullptr;
     return SSS_OK;
