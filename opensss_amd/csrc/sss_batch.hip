@@ -1,6 +1,7 @@
 // Batches of traced subjects on one context (sss_batch_* in sss_hip.h).
 //
-// A batch launch runs every member's activity upload and refresh, then one
+// A batch launch runs every member's activity upload and refresh (the
+// one-bin members' refreshes merged into one launch), then one
 // persistent scatter launch per kernel variant present (k_scatter_queue: the
 // members' workgroup blocks in 8 per-XCD queues), one chunk-reduce launch per
 // precision and the output downloads.  sss_batch_graph_launch captures that
@@ -40,6 +41,9 @@ struct sss_batch {
     std::vector<Group> groups;
     std::vector<Reduce> reduces;
     const ScatterArgs* d_args = nullptr;
+    const OneBinRefresh* d_one_bin = nullptr;   // one-bin members' tables
+    const int2* d_one_bin_blocks = nullptr;     // {one-bin member, block of 64 half paths}
+    int n_one_bin_blocks = 0;
     int32_t* d_counters = nullptr;
     int n_counters = 0;
     void* d_mem = nullptr;           // args, queues, reduce maps, counters
@@ -157,6 +161,22 @@ int build(sss_batch* b)
         b->reduces.push_back(r);
         rplans.emplace_back(off, std::move(blocks));
     }
+    // one-bin members' activity refreshes, merged into one launch
+    std::vector<OneBinRefresh> one_bin;
+    std::vector<int2> one_bin_blocks;
+    for (int i = 0; i < n; ++i) {
+        if (!b->m[i].trivial) continue;
+        OneBinRefresh r;
+        if (int rc = subject_one_bin_refresh(b->members[i], &r)) return rc;
+        const int64_t nb = (r.n_hp + kWave - 1) / kWave;
+        for (int64_t k = 0; k < nb; ++k) one_bin_blocks.push_back(make_int2((int)one_bin.size(), (int)k));
+        one_bin.push_back(r);
+    }
+    const size_t ob_off = place(std::max<size_t>(1, one_bin.size()) * sizeof(OneBinRefresh));
+    const size_t obb_off = place(std::max<size_t>(1, one_bin_blocks.size()) * sizeof(int2));
+    if (!one_bin.empty()) std::memcpy(img.data() + ob_off, one_bin.data(), one_bin.size() * sizeof(OneBinRefresh));
+    if (!one_bin_blocks.empty())
+        std::memcpy(img.data() + obb_off, one_bin_blocks.data(), one_bin_blocks.size() * sizeof(int2));
     b->n_counters = (int)b->groups.size() * kXcdCount;
     const size_t cnt_off = place(std::max(1, b->n_counters) * sizeof(int32_t));
 
@@ -167,6 +187,9 @@ int build(sss_batch* b)
     SSS_HIP(hipStreamSynchronize(st));
     unsigned char* base = static_cast<unsigned char*>(b->d_mem);
     b->d_args = reinterpret_cast<const ScatterArgs*>(base + args_off);
+    b->d_one_bin = reinterpret_cast<const OneBinRefresh*>(base + ob_off);
+    b->d_one_bin_blocks = reinterpret_cast<const int2*>(base + obb_off);
+    b->n_one_bin_blocks = (int)one_bin_blocks.size();
     b->d_counters = reinterpret_cast<int32_t*>(base + cnt_off);
     for (size_t r = 0; r < b->reduces.size(); ++r)
         b->reduces[r].blocks = reinterpret_cast<const int2*>(base + rplans[r].first);
@@ -191,8 +214,12 @@ int build(sss_batch* b)
 int queue_all(sss_batch* b)
 {
     hipStream_t st = ctx_stream(b->ctx);
-    for (sss_subject* s : b->members)
-        if (int rc = subject_queue_activity(s)) return rc;
+    for (size_t i = 0; i < b->members.size(); ++i) {
+        // one-bin members: the upload here, their refreshes in one launch below
+        const int rc = b->m[i].trivial ? subject_queue_upload(b->members[i]) : subject_queue_activity(b->members[i]);
+        if (rc) return rc;
+    }
+    SSS_HIP(launch_one_bin_refresh(b->d_one_bin, b->d_one_bin_blocks, b->n_one_bin_blocks, st));
     SSS_HIP(launch_fill_i32(b->d_counters, b->n_counters, 0, st));
     for (const sss_batch::Group& G : b->groups) {
         if (G.n_blocks == 0) continue;

@@ -245,6 +245,16 @@ hipError_t launch_fold_compact(const HalfPathTable t, SampleTable s, int64_t n_h
 // run_if_a, run_if_b: when both are given, the records are rebuilt only if either is non-zero
 hipError_t launch_pair_half(const HalfPathTable t, int64_t n_hp, bool one_bin, hipStream_t st,
                             const int32_t* run_if_a = nullptr, const int32_t* run_if_b = nullptr);
+// The activity refresh of a one-bin subject (ActPath per half path, then its
+// pair record) for many subjects in one launch: blocks[i] = {member, block of
+// 64 half paths}, the member's tables in m.
+struct OneBinRefresh {
+    HalfPathTable t;
+    SampleTable s;
+    int64_t n_hp;
+    const double* lam;
+};
+hipError_t launch_one_bin_refresh(const OneBinRefresh* m, const int2* blocks, int n_blocks, hipStream_t st);
 hipError_t launch_trace_segments(const Grid& g, const double* segs, int64_t n, int cap,
                                  int32_t* counts, double* L, int32_t* idx, double* s,
                                  hipStream_t st);
@@ -423,6 +433,11 @@ int subject_stage_activity(sss_subject* s, const double* lam);
 // (transposed when the subject takes C-order volumes), then the activity
 // refresh.
 int subject_queue_activity(sss_subject* s);
+// The upload half of subject_queue_activity (staged image to the device, in
+// the kernels' layout), for a one-bin subject whose refresh a batch merges.
+int subject_queue_upload(sss_subject* s);
+// A one-bin subject's tables as launch_one_bin_refresh reads them.
+int subject_one_bin_refresh(const sss_subject* s, OneBinRefresh* r);
 
 hipError_t launch_dense_scatter_typed(const void* compact, bool fp64, int n_bins, int nru, int ndu,
                                       int n_det, const int32_t* dets, const int16_t* lor,

@@ -1821,7 +1821,7 @@ int subject_stage_activity(sss_subject* s, const double* lam)
     return SSS_OK;
 }
 
-int subject_queue_activity(sss_subject* s)
+int subject_queue_upload(sss_subject* s)
 {
     hipStream_t st = s->ctx->stream;
     SSS_HIP(hipMemcpyAsync(s->z_fastest ? s->d_vol_in : s->d_lam, s->h_lam, (size_t)s->n_vox * sizeof(double),
@@ -1829,7 +1829,23 @@ int subject_queue_activity(sss_subject* s)
     if (s->z_fastest)
         SSS_HIP(launch_volume_x_fastest(s->d_vol_in, s->d_lam, s->grid.nx, s->grid.ny, s->grid.nz, st));
     s->has_activity = true;
+    return SSS_OK;
+}
+
+int subject_queue_activity(sss_subject* s)
+{
+    if (int rc = subject_queue_upload(s)) return rc;
     return refresh_fold(s);
+}
+
+int subject_one_bin_refresh(const sss_subject* s, OneBinRefresh* r)
+{
+    if (!s->trivial) return fail(SSS_ERR_INVALID, "batch: a merged one-bin refresh needs a one-bin subject");
+    r->t = s->hp;
+    r->s = s->smp;
+    r->n_hp = s->n_hp;
+    r->lam = s->d_lam;
+    return SSS_OK;
 }
 
 }  // namespace sss

@@ -137,14 +137,16 @@ enum ActMode { kActSum, kActRefresh, kActCheck };
 
 template <int MODE>
 __device__ __forceinline__ void activity_block(HalfPathTable t, SampleTable s, int64_t n_hp,
-                                               const double* __restrict__ lam, int32_t* changed = nullptr)
+                                               const double* __restrict__ lam, int32_t* changed = nullptr,
+                                               int64_t blk = -1)
 {
+    if (blk < 0) blk = blockIdx.x;
     constexpr bool FOLD = MODE == kActRefresh;
     constexpr bool COUNT = MODE != kActSum;
     __shared__ double part_act[kRefreshWaves][kWave];
     __shared__ int part_nz[kRefreshWaves][kWave];
     const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
-    const int64_t h = (int64_t)blockIdx.x * kWave + l;
+    const int64_t h = blk * kWave + l;
     double act = 0;
     int nz = 0;
     if (h < n_hp) {
@@ -468,14 +470,10 @@ hipError_t launch_fold_weights(SampleTable s, const int64_t* n_records, int64_t 
     return hipGetLastError();
 }
 
-// The FP32 pair evaluation's record of each half path (after the fold
-// table's offsets are scanned).
-__global__ void k_pair_half(HalfPathTable t, int64_t n_hp, bool one_bin, const int32_t* run_if_a,
-                            const int32_t* run_if_b)
+// The FP32 pair evaluation's record of half path h (after the fold table's
+// offsets are scanned).
+__device__ __forceinline__ void pair_half_record(const HalfPathTable& t, int64_t h, bool one_bin)
 {
-    if (run_if_a && run_if_b && *run_if_a == 0 && *run_if_b == 0) return;
-    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= n_hp) return;
     PairHalf r;
     r.dist = t.dist[h];
     const double act = t.act[h];
@@ -487,6 +485,29 @@ __global__ void k_pair_half(HalfPathTable t, int64_t n_hp, bool one_bin, const i
     r.lnatt = (float)t.lnatt[h];
     r.att = (float)att;
     t.rec[h] = r;
+}
+
+__global__ void k_pair_half(HalfPathTable t, int64_t n_hp, bool one_bin, const int32_t* run_if_a,
+                            const int32_t* run_if_b)
+{
+    if (run_if_a && run_if_b && *run_if_a == 0 && *run_if_b == 0) return;
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= n_hp) return;
+    pair_half_record(t, h, one_bin);
+}
+
+// k_activity_sum then k_pair_half for many one-bin subjects in one launch:
+// workgroup i takes block blocks[i].y of 64 half paths of member
+// blocks[i].x, with the same arithmetic as the two kernels; the lane that
+// stored a path's ActPath then writes its pair record.
+__global__ __launch_bounds__(kRefreshWaves * kWave) void k_one_bin_refresh(const OneBinRefresh* __restrict__ m,
+                                                                           const int2* __restrict__ blocks)
+{
+    const int2 b = blocks[blockIdx.x];
+    const OneBinRefresh& r = m[b.x];
+    activity_block<kActSum>(r.t, r.s, r.n_hp, r.lam, nullptr, b.y);
+    const int64_t h = (int64_t)b.y * kWave + threadIdx.x % kWave;
+    if (threadIdx.x < kWave && h < r.n_hp) pair_half_record(r.t, h, true);
 }
 
 hipError_t launch_pair_half(const HalfPathTable t, int64_t n_hp, bool one_bin, hipStream_t st,
@@ -539,6 +560,13 @@ hipError_t launch_volume_x_fastest(const double* src, double* dst, int X, int Y,
     if ((int64_t)X * Y * Z == 0) return hipSuccess;
     const dim3 grid((unsigned)((Z + 31) / 32), (unsigned)((X + 31) / 32), (unsigned)Y);
     hipLaunchKernelGGL(k_volume_x_fastest, grid, dim3(256), 0, st, src, dst, X, Y, Z);
+    return hipGetLastError();
+}
+
+hipError_t launch_one_bin_refresh(const OneBinRefresh* m, const int2* blocks, int n_blocks, hipStream_t st)
+{
+    if (n_blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_one_bin_refresh, dim3((unsigned)n_blocks), dim3(kRefreshWaves * kWave), 0, st, m, blocks);
     return hipGetLastError();
 }
 
