@@ -20,16 +20,17 @@ pytestmark = pytest.mark.gpu
 def _members():
     """Small subjects covering the kernel variants: sliding window (15 bins),
     relative window h = 2 (21 bins) and its wide-bin form (6 bins), one TOF
-    bin (zero table columns at large offsets), one bin without TOF (the
-    one-bin pair loop), an FP64 subject, and other tile shapes (8 rings x 24
+    bin (zero table columns at large offsets), two subjects of one bin
+    without TOF (the one-bin pair loop; their activity refreshes share one
+    launch), an FP64 subject, and other tile shapes (8 rings x 24
     detectors)."""
     out = []
     for n_bins, precision, n_sp, seed, kw in [(15, 'fp32', 30, 1, {}), (21, 'fp32', 25, 2, {}),
                                              (6, 'fp32', 20, 3, {}), (1, 'fp32', 20, 4, {}),
-                                             (1, 'fp32', 24, 8, {}),
+                                             (1, 'fp32', 24, 8, {}), (1, 'fp32', 17, 9, {'nru': 8, 'ndu': 24}),
                                              (7, 'fp64', 15, 5, {}), (15, 'fp32', 18, 6, {'nru': 8, 'ndu': 24})]:
         w = phantoms.small_tof(seed=seed, n_bins=n_bins, n_sp=n_sp, **kw)
-        if seed == 8:
+        if seed in (8, 9):
             w = dataclasses.replace(w, tof_resolution=1e9, tof_range=1000.0)
         out.append((w, precision))
     return out
@@ -58,9 +59,10 @@ def test_batch_launch_equals_each_subject():
     try:
         want = [s.output_compact() for s in subs]
         batch = GraphBatch(subs)
-        # the two 15-bin sliding-window members share one launch
+        # the two 15-bin sliding-window members share one launch, and so do
+        # the two one-bin members
         variants = {(p, s.info['tof_fold'], w.n_bins, w.tof_resolution) for (w, p), s in zip(ws, subs)}
-        assert batch.info['scatter_launches'] == len(variants) == len(subs) - 1
+        assert batch.info['scatter_launches'] == len(variants) == len(subs) - 2
         for graph in (False, True, True, False):
             batch.launch(graph=graph)
             for i in range(len(subs)):
